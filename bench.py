@@ -1,0 +1,261 @@
+"""Benchmark: synaptic events/s of the 140k-neuron connectome time step on one GPU.
+
+Workload (seeded, synthetic): N=140,000 LIF neurons (tau_m ~ U(10,30) ms per
+neuron, v_th 1, v_reset 0, refractory 2 ms, soft reset), 70/30 E/I,
+out-degree power law (exponent 2.1, k in [100, 5000]; E ~ 50M), lognormal
+weights (I x4 negative), integer delays U{1..20} through
+HeterSynapsePSC(ExponentialPSC tau_s=5 ms, max_delay_steps=21), dt=0.1 ms,
+8 lanes, Poisson background 5 Hz per neuron-lane as a 20.0 current kick.
+A "step" is one simulation time step of all 8 lanes.
+
+Prints one JSON line (rank 0).  value = synaptic events/s over the timed
+steps (events = (lane, edge) deliveries, counted exactly from the raster).
+``roofline`` prices the dominant kernel's algorithmic bytes per launch
+against its HIP-event-timed average duration; ``cpu_baseline`` times the
+CPU oracle (torch-CPU restatement of the reference's native path) on a
+bounded sample of the same workload from the same state, and also checks
+that sample's raster against the GPU's bit for bit.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=140_000)
+    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--kmin", type=int, default=100)
+    ap.add_argument("--kmax", type=int, default=5000)
+    ap.add_argument("--delay-max", type=int, default=20)
+    ap.add_argument("--mode", choices=["ordered", "fast"], default="ordered")
+    ap.add_argument("--kernel-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    return ap.parse_args()
+
+
+def build(args, dev):
+    from btorch_amd import workloads
+    from btorch_amd.models import (
+        LIF, ExponentialPSC, HeterSynapsePSC, RecurrentNN, SparseConn, functional,
+    )
+
+    n, D = args.n, args.delay_max + 1
+    c = workloads.powerlaw_net_fast(n=n, k_min=args.kmin, k_max=args.kmax, exc_frac=0.7,
+                                    delay_lo=1, delay_hi=args.delay_max, seed=args.seed)
+    rows = torch.from_numpy(c.pre * D + np.clip(c.delay, 0, D - 1)).to(dev)
+    cols = torch.from_numpy(c.post).to(dev)
+    w = torch.from_numpy(c.w).to(dev)
+    coo = torch.sparse_coo_tensor(torch.stack([rows, cols]), w, size=(n * D, n))
+    backend = "native" if args.mode == "ordered" else "triton"
+    conn = SparseConn(coo, sparse_backend=backend, device=dev)
+    rng = np.random.default_rng(args.seed + 7)
+    tau = torch.from_numpy(rng.uniform(10.0, 30.0, n).astype(np.float32))
+    syn = HeterSynapsePSC(n, 1, None, conn, base_psc=ExponentialPSC, max_delay_steps=D,
+                          use_circular_buffer=True, tau_syn=5.0)
+    model = RecurrentNN(LIF(n, tau=tau, tau_ref=2.0), syn, step_mode="m",
+                        update_state_names=[]).to(dev)
+    functional.init_net_state(model, batch_size=args.lanes, device=dev, dtype=torch.float32)
+    # per-(neuron, delay bin) out-degree for exact event counting
+    rp = conn._csr_row_ptr
+    deg = (rp[1:] - rp[:-1]).reshape(n, D).to(torch.float64)
+    return model, conn, deg, c.nnz
+
+
+def poisson_kicks(T, lanes, n, dev, seed, rate_hz=5.0, dt_ms=0.1, kick=20.0):
+    g = torch.Generator(device=dev).manual_seed(seed)
+    p = rate_hz * dt_ms * 1e-3
+    x = torch.empty((T, lanes, n), device=dev)
+    for t0 in range(0, T, 256):
+        t1 = min(T, t0 + 256)
+        x[t0:t1] = (torch.rand((t1 - t0, lanes, n), device=dev, generator=g) < p).float() * kick
+    return x
+
+
+def delivery_table(raster, n, deg):
+    """A[t, d] = deliveries at delay d caused by the spikes of step t (summed over lanes)."""
+    from btorch_amd.models._engine import unpack_bits
+    T = raster.shape[0]
+    out = torch.zeros((T, deg.shape[1]), dtype=torch.float64, device=deg.device)
+    act = torch.zeros((T, deg.shape[1]), dtype=torch.float64, device=deg.device)
+    nz = (deg > 0).to(torch.float64)
+    for t0 in range(0, T, 64):
+        s = unpack_bits(raster[t0:t0 + 64], n).to(torch.float64)   # (t, lanes, n)
+        out[t0:t0 + 64] = (s @ deg).sum(1)
+        act[t0:t0 + 64] = (s @ nz).sum(1)
+    return out, act
+
+
+def window_events(tables, lo, hi):
+    """Deliveries (and active virtual rows) landing in steps [lo, hi) of the
+    concatenated history of tables (each row t carries its delays d)."""
+    out, act = tables
+    T, D = out.shape
+    ev = 0.0
+    rows = 0.0
+    for d in range(D):
+        t0, t1 = max(0, lo - d), max(0, hi - d)
+        ev += float(out[t0:t1, d].sum())
+        rows += float(act[t0:t1, d].sum())
+    return ev, rows
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
+              "(the path shards by postsynaptic neuron; see the design notes).", file=sys.stderr)
+        sys.exit(2)
+    from btorch_amd.models import environ
+
+    dev = torch.device("cuda:0")
+    torch.manual_seed(args.seed)
+    t_build = time.perf_counter()
+    model, conn, deg, nnz = build(args, dev)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+    eng = model.fused_engine()
+    n, lanes, D = args.n, args.lanes, args.delay_max + 1
+    W, K, KS = args.warmup, args.steps, args.kernel_steps
+    x = poisson_kicks(W + K + KS, lanes, n, dev, args.seed)
+    torch.cuda.synchronize()
+
+    with environ.context(dt=0.1), torch.no_grad():
+        r_w, _ = eng.run(x[:W].contiguous() if W else None, max(W, 0), lanes, set(),
+                         check_overflow=False) if W else (None, None)
+        torch.cuda.synchronize()
+        # snapshot of the state entering the timed region (for the CPU sample)
+        snap = {"v": model.neuron.v.detach().cpu().clone(),
+                "refractory": model.neuron.refractory.detach().cpu().clone(),
+                "psc": model.synapse.base_psc.psc.detach().cpu().clone(),
+                "history": model.synapse.history.history.detach().cpu().clone(),
+                "cursor": model.synapse.history.cursor}
+        xt = x[W:W + K].contiguous()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r_t, _ = eng.run(xt, K, lanes, set(), check_overflow=False)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        timings: dict = {}
+        r_k, _ = eng.run(x[W + K:W + K + KS].contiguous(), KS, lanes, set(),
+                         check_overflow=False, timings=timings)
+        torch.cuda.synchronize()
+        from btorch_amd import _lib
+        overflow = bool(_lib.load().btx_rsnn_check_overflow(
+            eng._workspace(lanes, n, conn.sparse_config.slots, int(conn.sparse_config.ordered),
+                           dev).data_ptr(), _lib.stream_ptr()))
+
+    rasters = [r for r in (r_w, r_t, r_k) if r is not None]
+    full = torch.cat(rasters, 0)
+    tables = delivery_table(full, n, deg)
+    ev_t, rows_t = window_events(tables, W, W + K)
+    ev_k, rows_k = window_events(tables, W + K, W + K + KS)
+    value = ev_t / elapsed
+    ms_step = 1e3 * elapsed / K
+
+    # roofline of the dominant kernel over the kernel-timing pass
+    per = {k: (ms / max(c, 1), c, ms) for k, (ms, c) in timings.items()}
+    dom = max(("step", "push", "compact"), key=lambda k: per[k][2])
+    nwords = (n + 63) // 64
+    a_rows_per_launch = rows_k / KS
+    ev_per_launch = ev_k / KS
+    bytes_model = {
+        # v, refractory, psc read+write; x and slot count read: 32 B per
+        # neuron-lane; per-neuron tau; raster word; slot records of the
+        # previous step's terms (8 B each)
+        "step": 32.0 * n * lanes + 4.0 * n + 8.0 * nwords * lanes + 8.0 * ev_per_launch,
+        # active index + int64 row_ptr pair per active row; col + weight per
+        # term; slot record + count per term
+        "push": 20.0 * a_rows_per_launch + 8.0 * ev_per_launch + 12.0 * ev_per_launch,
+        # spike words + active indices
+        "compact": 8.0 * nwords * lanes + 4.0 * a_rows_per_launch / max(D - 1, 1),
+    }
+    avg_ms = per[dom][0]
+    achieved = bytes_model[dom] / (avg_ms * 1e-3)
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                "kernel": {"step": "k_lif_exp_step", "push": "k_push",
+                           "compact": "k_compact_bits"}[dom],
+                "bytes_per_launch": bytes_model[dom], "avg_us": avg_ms * 1e3,
+                "kernel_avg_us": {k: round(v[0] * 1e3, 3) for k, v in per.items()}}
+
+    cpu = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        cpu = cpu_baseline(args, conn, snap, x[W:W + args.cpu_steps].cpu(), r_t, deg, tables, W)
+
+    line = {
+        "metric": "synaptic events/s at 140k-neuron connectome",
+        "value": value,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_step,
+        "steps_per_s": K / elapsed,
+        "events_per_step": ev_t / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded power-law connectome, Poisson background)",
+        "config": {"workload": "connectome forward sim, N=140k LIF, power-law out-degree "
+                               "(2.1, 100..5000), delays 1-20 (D=21), 8 lanes, dt 0.1 ms",
+                   "n": n, "synapses": nnz, "lanes": lanes, "delay_bins": D,
+                   "accumulation": args.mode, "build_s": round(t_build, 2),
+                   "overflow": overflow},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(args, conn, snap, x_cpu, r_t, deg, tables, W):
+    """Time the CPU oracle on the first ``cpu_steps`` timed steps from the
+    same state; check its raster against the GPU's."""
+    from btorch_amd.models._engine import unpack_bits
+    from oracle import rsnn
+
+    n, D = args.n, args.delay_max + 1
+    threads = len(os.sched_getaffinity(0))
+    torch.set_num_threads(threads)
+    dst = conn.indices[0].cpu().numpy()
+    src = conn.indices[1].cpu().numpy()
+    w = conn.magnitude.detach().cpu().numpy()
+    mm = rsnn.NativeSparse.from_coalesced(dst, src, w, n * D, n)
+    rng = np.random.default_rng(args.seed + 7)
+    tau = torch.from_numpy(rng.uniform(10.0, 30.0, n).astype(np.float32))
+    net = rsnn.Net(n, mm, rsnn.LIFParams.make(n, tau=tau, tau_ref=2.0), torch.tensor(5.0), 0.1,
+                   delay_mode="hetero", max_delay_steps=D, circular=True)
+    S = x_cpu.shape[0]
+    t0 = time.perf_counter()
+    out = rsnn.simulate(net, x_cpu, state=snap)
+    el = time.perf_counter() - t0
+    gpu_z = unpack_bits(r_t[:S], n).cpu()
+    ev, _ = window_events(tables, W, W + S)
+    return {"value": ev / el, "unit": "events/s", "cores": threads, "kind": "port",
+            "sample": f"{S} steps of the same workload from the timed region's entry state "
+                      f"(torch-CPU oracle, {threads} threads)",
+            "steps_per_s": S / el, "raster_bit_exact_vs_gpu": bool(torch.equal(out["z"], gpu_z))}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+"""Tensor-level wrappers over the C ABI of libbtx.so.
+
+Each wrapper validates shapes/devices on the host (so no kernel is launched
+on operands it does not expect), allocates outputs with torch and enqueues
+the kernel on the current stream.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+
+
+def _i32(t):
+    return t.dtype == torch.int32 and t.is_contiguous()
+
+
+@dataclass
+class CsrLayout:
+    """Source-major CSR of a connection matrix (n_src rows, n_dst columns).
+
+    ``perm[k]`` is the position, in the reference's coalesced (dst, src)
+    order (btorch/models/linear.py:276-291), of CSR edge k; packed weights
+    are ``magnitude[perm]``.
+    """
+
+    row_ptr: torch.Tensor  # (n_src + 1,) int64
+    col: torch.Tensor      # (nnz,) int32, destinations, ascending per row
+    perm: torch.Tensor     # (nnz,) int64
+    n_src: int
+    n_dst: int
+    csc_ptr: torch.Tensor | None = None  # (n_dst + 1,) int64, dst-major row pointer
+    csc_src: torch.Tensor | None = None  # (nnz,) int32, sources in coalesced order
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.numel())
+
+    def to(self, device) -> "CsrLayout":
+        mv = (lambda t: None if t is None else t.to(device))
+        return CsrLayout(self.row_ptr.to(device), self.col.to(device), self.perm.to(device),
+                         self.n_src, self.n_dst, mv(self.csc_ptr), mv(self.csc_src))
+
+
+def build_csr(dst: torch.Tensor, src: torch.Tensor, n_src: int, n_dst: int) -> CsrLayout:
+    """Source-major CSR from coalesced (dst, src) index vectors.
+
+    A stable sort by source keeps destinations ascending inside each row
+    (the input is sorted by (dst, src)).  Runs on the indices' device.
+    """
+    if n_src >= 2**31 or n_dst >= 2**31:
+        raise ValueError("connection dimensions must fit int32 indices")
+    dst = dst.to(torch.int64)
+    src = src.to(torch.int64)
+    perm = torch.argsort(src, stable=True)
+    col = dst[perm].to(torch.int32).contiguous()
+    counts = torch.bincount(src, minlength=n_src)
+    row_ptr = torch.zeros(n_src + 1, dtype=torch.int64, device=dst.device)
+    if counts.numel():
+        row_ptr[1:] = torch.cumsum(counts, 0)
+    dcounts = torch.bincount(dst, minlength=n_dst)
+    csc_ptr = torch.zeros(n_dst + 1, dtype=torch.int64, device=dst.device)
+    if dcounts.numel():
+        csc_ptr[1:] = torch.cumsum(dcounts, 0)
+    return CsrLayout(row_ptr.contiguous(), col, perm.contiguous(), n_src, n_dst,
+                     csc_ptr.contiguous(), src.to(torch.int32).contiguous())
+
+
+def compact_dense(x2d: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """Ascending nonzero columns of every row of ``x2d`` (lanes, n)."""
+    _lib.require_hip(x2d)
+    if x2d.dtype != torch.float32:
+        raise TypeError("compaction expects float32 input")
+    x2d = x2d.contiguous()
+    lanes, n = x2d.shape
+    idx = torch.empty((lanes, max(n, 1)), dtype=torch.int32, device=x2d.device)
+    cnt = torch.empty((lanes,), dtype=torch.int32, device=x2d.device)
+    _lib.check(_lib.load().btx_compact_dense(x2d.data_ptr(), lanes, n, n, idx.data_ptr(),
+                                             idx.shape[1], cnt.data_ptr(), _lib.stream_ptr()),
+               "btx_compact_dense")
+    return idx, cnt
+
+
+def compact_bits(words: torch.Tensor, n: int) -> tuple[torch.Tensor, torch.Tensor]:
+    """Ascending set-bit positions of every row of ``words`` (lanes, ceil(n/64)) int64."""
+    _lib.require_hip(words)
+    lanes = words.shape[0]
+    if words.shape[1] != (n + 63) // 64 or not words.is_contiguous():
+        raise ValueError("words must be contiguous (lanes, ceil(n/64))")
+    idx = torch.empty((lanes, max(n, 1)), dtype=torch.int32, device=words.device)
+    cnt = torch.empty((lanes,), dtype=torch.int32, device=words.device)
+    _lib.check(_lib.load().btx_compact_bits(words.data_ptr(), lanes, n, words.shape[1],
+                                            idx.data_ptr(), idx.shape[1], cnt.data_ptr(),
+                                            _lib.stream_ptr()), "btx_compact_bits")
+    return idx, cnt
+
+
+def spmv_workspace(lanes: int, n_dst: int, slots: int, ordered: bool, device) -> torch.Tensor:
+    nbytes = _lib.load().btx_spmv_workspace_bytes(lanes, n_dst, slots, int(ordered))
+    return torch.zeros(nbytes, dtype=torch.uint8, device=device)
+
+
+def spmv_event(layout: CsrLayout, w_csr: torch.Tensor, x2d: torch.Tensor, active: torch.Tensor,
+               counts: torch.Tensor, *, ordered: bool, slots: int,
+               workspace: torch.Tensor, w_dst: torch.Tensor) -> torch.Tensor:
+    """out (lanes, n_dst) = x2d @ W from the compacted active lists; ``w_dst``
+    holds the same weights in the coalesced (dst, src) order."""
+    _lib.require_hip(x2d, w_csr, active, counts, workspace, w_dst)
+    if layout.csc_ptr is None or w_dst.numel() != layout.nnz or not w_dst.is_contiguous():
+        raise ValueError("destination-major arrays missing or mismatched")
+    lanes = x2d.shape[0]
+    if x2d.shape[1] != layout.n_src or w_csr.numel() != layout.nnz:
+        raise ValueError("operand shapes do not match the connection layout")
+    if not (_i32(active) and _i32(counts)) or active.shape[0] != lanes:
+        raise ValueError("active/counts must be int32 (lanes, cap)/(lanes,)")
+    if w_csr.dtype != torch.float32 or x2d.dtype != torch.float32:
+        raise TypeError("the event-driven product supports float32 only")
+    out = torch.empty((lanes, layout.n_dst), dtype=torch.float32, device=x2d.device)
+    rc = _lib.load().btx_spmv_event(
+        layout.row_ptr.data_ptr(), layout.col.data_ptr(), w_csr.data_ptr(), layout.n_src,
+        layout.n_dst, layout.nnz, layout.csc_ptr.data_ptr(), layout.csc_src.data_ptr(),
+        w_dst.data_ptr(), active.data_ptr(), counts.data_ptr(), lanes, active.shape[1],
+        x2d.data_ptr(), x2d.shape[1], out.data_ptr(), int(ordered), slots,
+        workspace.data_ptr(), workspace.numel(), _lib.stream_ptr())
+    _lib.check(rc, "btx_spmv_event")
+    return out
+
+
+def spmv_transpose(layout: CsrLayout, w_csr: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
+    """gx (lanes, n_src) = g @ W^T."""
+    _lib.require_hip(g, w_csr)
+    g = g.contiguous()
+    if g.shape[1] != layout.n_dst:
+        raise ValueError("gradient width does not match n_dst")
+    lanes = g.shape[0]
+    gx = torch.empty((lanes, layout.n_src), dtype=torch.float32, device=g.device)
+    _lib.check(_lib.load().btx_spmv_transpose(layout.row_ptr.data_ptr(), layout.col.data_ptr(),
+                                              w_csr.data_ptr(), layout.n_src, layout.n_dst,
+                                              g.data_ptr(), lanes, gx.data_ptr(),
+                                              _lib.stream_ptr()), "btx_spmv_transpose")
+    return gx
+
+
+def spmv_weight_grad(layout: CsrLayout, x2d: torch.Tensor | None, active: torch.Tensor,
+                     counts: torch.Tensor, g: torch.Tensor, gw_csr: torch.Tensor) -> None:
+    """gw_csr += event-driven sum over active (lane, source) of x * g[:, col]."""
+    _lib.require_hip(g, gw_csr, active, counts)
+    g = g.contiguous()
+    lanes = g.shape[0]
+    _lib.check(_lib.load().btx_spmv_weight_grad(
+        layout.row_ptr.data_ptr(), layout.col.data_ptr(), layout.n_src, layout.n_dst,
+        active.data_ptr(), counts.data_ptr(), lanes, active.shape[1],
+        None if x2d is None else x2d.data_ptr(), 0 if x2d is None else x2d.shape[1],
+        g.data_ptr(), gw_csr.data_ptr(), _lib.stream_ptr()), "btx_spmv_weight_grad")

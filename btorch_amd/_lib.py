@@ -1,0 +1,164 @@
+"""ctypes binding of libbtx.so (declared in include/btorch_amd.h).
+
+The library is the only compute backend of this package: nothing here falls
+back to another implementation.  Loading fails loudly when the shared object
+is missing, and every call that needs a GPU raises when tensors are not on a
+HIP device.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch
+
+from ._build import LIB_PATH
+
+MAX_BINS = 64
+
+NEURON_LIF = 0
+NEURON_GLIF3 = 1
+
+SURR_SIGMOID = 0
+SURR_SUPERSPIKE = 1
+SURR_ERF = 2
+
+_c_i64 = ctypes.c_int64
+_c_int = ctypes.c_int
+_c_f = ctypes.c_float
+_vp = ctypes.c_void_p
+
+
+class RsnnPlan(ctypes.Structure):
+    """Mirror of ``btx_rsnn_plan``; field order and types match the header."""
+
+    _fields_ = [
+        ("n", _c_i64),
+        ("lanes", _c_i64),
+        ("neuron_model", _c_int),
+        ("hard_reset", _c_int),
+        ("use_refractory", _c_int),
+        ("ordered", _c_int),
+        ("slots", _c_int),
+        ("dt", _c_f),
+        ("tau", _vp), ("tau_stride", _c_i64),
+        ("v_th", _vp), ("v_th_stride", _c_i64),
+        ("v_reset", _vp), ("v_reset_stride", _c_i64),
+        ("c_m", _vp), ("c_m_stride", _c_i64),
+        ("tau_ref", _vp), ("tau_ref_stride", _c_i64),
+        ("tau_syn", _vp), ("tau_syn_stride", _c_i64),
+        ("k_syn", _vp), ("k_syn_stride", _c_i64),
+        ("v", _vp),
+        ("refractory", _vp),
+        ("psc", _vp),
+        ("x", _vp), ("x_step_stride", _c_i64),
+        ("row_ptr", _vp),
+        ("col", _vp),
+        ("w", _vp),
+        ("nnz", _c_i64),
+        ("csr_d", _c_int),
+        ("ring_d", _c_int),
+        ("ring_idx", _vp),
+        ("ring_cnt", _vp),
+        ("ring_cursor", _c_int),
+        ("nbins", _c_int),
+        ("bin_delay", _c_int * MAX_BINS),
+        ("bin_sub", _c_int * MAX_BINS),
+        ("teacher", _vp),
+        ("raster", _vp),
+        ("rec_z", _vp),
+        ("rec_v", _vp),
+        ("rec_refractory", _vp),
+        ("rec_psc", _vp),
+        ("rec_v_pre", _vp),
+        ("rec_v_prev", _vp),
+        ("workspace", _vp),
+        ("workspace_bytes", ctypes.c_size_t),
+    ]
+
+
+_SIGNATURES = {
+    "btx_version": (_c_int, []),
+    "btx_last_error": (ctypes.c_char_p, []),
+    "btx_compact_dense": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "btx_compact_bits": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "btx_spmv_workspace_bytes": (ctypes.c_size_t, [_c_i64, _c_i64, _c_int, _c_int]),
+    "btx_spmv_event": (
+        _c_int,
+        [_vp, _vp, _vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i64,
+         _vp, _c_i64, _vp, _c_int, _c_int, _vp, ctypes.c_size_t, _vp],
+    ),
+    "btx_spmv_transpose": (_c_int, [_vp, _vp, _vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "btx_spmv_weight_grad": (
+        _c_int,
+        [_vp, _vp, _c_i64, _c_i64, _vp, _vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp, _vp],
+    ),
+    "btx_rsnn_workspace_bytes": (ctypes.c_size_t, [_c_i64, _c_i64, _c_int, _c_int]),
+    "btx_rsnn_run": (_c_int, [ctypes.POINTER(RsnnPlan), _c_i64, _vp]),
+    "btx_rsnn_check_overflow": (_c_int, [_vp, _vp]),
+    "btx_rsnn_run_timed": (_c_int, [ctypes.POINTER(RsnnPlan), _c_i64, _vp,
+                                    ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_int64)]),
+    "btx_lif_step": (_c_int, [ctypes.POINTER(RsnnPlan), _vp, _vp, _vp, _vp]),
+    "btx_exp_decay": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp]),
+}
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+
+
+def library_path() -> Path:
+    return LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    """Load libbtx.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"btorch_amd: {LIB_PATH} is missing. Build it with "
+                    "`python -m btorch_amd._build` (hipcc, gfx950)."
+                )
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in _SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def declared_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().btx_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (status {rc}): {msg}")
+
+
+def require_hip(*tensors: torch.Tensor) -> None:
+    """Raise unless every tensor lives on a HIP device."""
+    for t in tensors:
+        if t is not None and t.device.type != "cuda":
+            raise RuntimeError(
+                "btorch_amd runs its kernels on a HIP device only; got a tensor on "
+                f"{t.device}. Move the module and inputs to 'cuda'."
+            )
+    if not torch.cuda.is_available():
+        raise RuntimeError("btorch_amd: no HIP device is visible")
+
+
+def stream_ptr(device: torch.device | None = None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()

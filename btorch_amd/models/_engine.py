@@ -1,0 +1,289 @@
+"""Fused time-loop engine: binds ``RecurrentNN(LIF, <exp synapse>(SparseConn))``
+to ``btx_rsnn_run`` so that a whole sequence runs as device work enqueued
+from C, with no per-step Python and no (B, N*D) history matrix.
+
+Supported cells (everything else runs the generic per-step loop):
+  neuron   LIF / IF (soft or hard reset, optional refractory, scalar or
+           per-neuron parameters)
+  synapse  ExponentialPSC(linear=SparseConn(N, N))
+           DelayedPSC(ExponentialPSC(SparseConn(N, N)), max_delay_steps=m)
+           HeterSynapsePSC(base_psc=ExponentialPSC, n_receptor=1,
+                           linear=SparseConn(N*D, N), max_delay_steps=D)
+  input    x (T, *batch, N), optionally through neuron_inp_module, applied to
+           the whole sequence at once (a SparseConn becomes one batched
+           event-driven product over T*B lanes)
+Delay semantics follow rnn.py:517-528 + synapse.py:751-761: the spike of
+step t enters the ring before the synapse reads it, so a synapse of delay d
+delivers at the end of step t+d.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import torch
+
+from .. import _lib
+from . import _plan
+from .history import SpikeHistory
+from .linear import BaseSparseConn
+from .neurons.lif import LIF
+from .synapse import DelayedPSC, ExponentialPSC, HeterSynapsePSC
+
+
+@dataclass
+class CellSpec:
+    neuron: LIF
+    exp_psc: ExponentialPSC          # module whose psc decays and receives W z
+    conn: BaseSparseConn
+    history: SpikeHistory | None
+    csr_d: int
+    ring_d: int
+    bins: list[tuple[int, int]]      # (ring age, virtual-row offset)
+    psc_keys: list[str]              # dotted names whose state equals psc
+    psc_modules: list = field(default_factory=list)
+
+
+def _bin_nonempty(conn: BaseSparseConn, csr_d: int) -> list[bool]:
+    rp = conn._csr_row_ptr
+    counts = (rp[1:] - rp[:-1]).reshape(-1, csr_d).sum(0)
+    return [bool(c) for c in counts.tolist()]
+
+
+def match_cell(rnn) -> CellSpec | None:
+    """CellSpec when ``rnn`` is a cell the engine runs, else None."""
+    neuron, syn = getattr(rnn, "neuron", None), getattr(rnn, "synapse", None)
+    if type(neuron) not in (LIF,) and not isinstance(neuron, LIF):
+        return None
+    if getattr(rnn, "syn_inp_module", None) is not None:
+        return None
+    n = neuron.size
+    if isinstance(syn, ExponentialPSC):
+        conn = syn.linear
+        spec = CellSpec(neuron, syn, conn, None, 1, 1, [(0, 0)], ["synapse.psc"], [syn])
+    elif isinstance(syn, DelayedPSC) and isinstance(syn.psc_module, ExponentialPSC):
+        conn = syn.psc_module.linear
+        if syn.history is None:
+            spec = CellSpec(neuron, syn.psc_module, conn, None, 1, 1, [(0, 0)],
+                            ["synapse.psc_module.psc"], [syn.psc_module])
+        else:
+            m = syn.max_delay_steps
+            spec = CellSpec(neuron, syn.psc_module, conn, syn.history, 1, m + 1, [(m, 0)],
+                            ["synapse.psc_module.psc"], [syn.psc_module])
+    elif (isinstance(syn, HeterSynapsePSC) and isinstance(syn.base_psc, ExponentialPSC)
+          and syn.n_receptor == 1):
+        conn = syn.linear
+        keys = ["synapse.psc", "synapse.base_psc.psc"]
+        if syn.history is None:
+            spec = CellSpec(neuron, syn.base_psc, conn, None, 1, 1, [(0, 0)], keys,
+                            [syn, syn.base_psc])
+        else:
+            d = syn.max_delay_steps
+            spec = CellSpec(neuron, syn.base_psc, conn, syn.history, d, d, [], keys,
+                            [syn, syn.base_psc])
+    else:
+        return None
+    if not isinstance(conn, BaseSparseConn) or conn.bias is not None:
+        return None
+    if conn.in_features != n * spec.csr_d or conn.out_features != n:
+        return None
+    if spec.exp_psc.size != n or tuple(spec.exp_psc.n_neuron) != (n,) and \
+            spec.exp_psc.size != n:
+        return None
+    if spec.csr_d > 1:
+        ne = _bin_nonempty(conn, spec.csr_d)
+        spec.bins = [(d, d) for d in range(spec.csr_d) if ne[d]] or [(0, 0)]
+    if len(spec.bins) > _lib.MAX_BINS:
+        return None
+    return spec
+
+
+def pack_bits(spikes: torch.Tensor) -> torch.Tensor:
+    """(…, n) 0/1 -> (…, ceil(n/64)) int64 words (bit j of word w is 64w+j)."""
+    n = spikes.shape[-1]
+    nw = (n + 63) // 64
+    b = torch.zeros(*spikes.shape[:-1], nw * 64, dtype=torch.int64, device=spikes.device)
+    b[..., :n] = (spikes != 0).to(torch.int64)
+    b = b.reshape(*spikes.shape[:-1], nw, 64)
+    shifts = torch.arange(64, device=spikes.device, dtype=torch.int64)
+    return (b << shifts).sum(-1)
+
+
+def unpack_bits(words: torch.Tensor, n: int) -> torch.Tensor:
+    """(…, W) int64 spike words -> (…, n) float32 0/1 (bit j of word w is 64w+j)."""
+    shifts = torch.arange(64, device=words.device, dtype=torch.int64)
+    bits = (words.unsqueeze(-1) >> shifts) & 1
+    return bits.reshape(*words.shape[:-1], words.shape[-1] * 64)[..., :n].to(torch.float32)
+
+
+class Engine:
+    """Runs a matched cell for T steps; owns the ring and the workspace."""
+
+    def __init__(self, spec: CellSpec):
+        self.spec = spec
+        self._ws: dict = {}
+
+    def _workspace(self, lanes, n, slots, ordered, device):
+        key = (lanes, n, slots, ordered, str(device))
+        ws = self._ws.get(key)
+        if ws is None:
+            nbytes = _lib.load().btx_rsnn_workspace_bytes(lanes, n, slots, int(ordered))
+            ws = torch.zeros(nbytes, dtype=torch.uint8, device=device)
+            self._ws = {key: ws}
+        return ws
+
+    def run(self, x: torch.Tensor | None, T: int, lanes: int, record: set[str],
+            check_overflow: bool = True, teacher: torch.Tensor | None = None,
+            timings: dict | None = None):
+        """x: (T, lanes, n) fp32 contiguous or None.  Returns (raster, records)."""
+        sp = self.spec
+        neuron, syn, conn = sp.neuron, sp.exp_psc, sp.conn
+        n = neuron.size
+        dev = neuron.v.device
+        _lib.require_hip(neuron.v)
+        cfg = conn.sparse_config
+        keep: list = []
+        p = _plan.lif_plan(neuron, lanes, dev, keep)
+        p.ordered = int(cfg.ordered)
+        p.slots = int(cfg.slots)
+        tau, ts, k, ks = syn.decay_coefficients(dev)
+        keep += [tau, k]
+        p.tau_syn, p.tau_syn_stride = tau.data_ptr(), ts
+        p.k_syn, p.k_syn_stride = k.data_ptr(), ks
+        # state
+        for mod, name in ((neuron, "v"), (syn, "psc")):
+            t = getattr(mod, name)
+            if t is None or t.numel() != lanes * n or t.dtype != torch.float32:
+                raise RuntimeError(f"{type(mod).__name__}.{name} is not initialised for "
+                                   f"{lanes} lanes x {n} neurons (call init_net_state)")
+            if not t.is_contiguous():
+                setattr(mod, name, t.contiguous())
+        p.v = neuron.v.data_ptr()
+        p.psc = syn.psc.data_ptr()
+        if neuron._use_refractory:
+            if not neuron.refractory.is_contiguous():
+                neuron.refractory = neuron.refractory.contiguous()
+            p.refractory = neuron.refractory.data_ptr()
+        if x is not None:
+            if x.shape != (T, lanes, n) or not x.is_contiguous() or x.dtype != torch.float32:
+                raise ValueError("engine input must be contiguous float32 (T, lanes, n)")
+            p.x = x.data_ptr()
+            p.x_step_stride = lanes * n
+        # connectivity
+        w = conn.packed_weight().detach()
+        if w.dtype != torch.float32:
+            raise TypeError("the fused engine supports float32 weights only")
+        keep.append(w)
+        p.row_ptr = conn._csr_row_ptr.data_ptr()
+        p.col = conn._csr_col.data_ptr()
+        p.w = w.data_ptr()
+        p.nnz = conn.nnz
+        p.csr_d = sp.csr_d
+        # ring of compacted spike lists
+        ring_d = sp.ring_d
+        ring_idx = torch.empty((ring_d, lanes, n), dtype=torch.int32, device=dev)
+        ring_cnt = torch.zeros((ring_d, lanes), dtype=torch.int32, device=dev)
+        cursor = 0
+        hist = sp.history
+        if hist is not None:
+            h = hist.history
+            if h is None or h.numel() != lanes * ring_d * n:
+                raise RuntimeError("SpikeHistory is not initialised for this batch")
+            h = h.reshape(lanes, ring_d, n).to(torch.float32).contiguous()
+            keep.append(h)
+            cursor = hist.cursor
+            lib = _lib.load()
+            for j in range(ring_d):
+                src = j if hist.use_circular_buffer else ring_d - 1 - j
+                _lib.check(lib.btx_compact_dense(h.data_ptr() + 4 * src * n, lanes, n, ring_d * n,
+                                                 ring_idx[j].data_ptr(), n,
+                                                 ring_cnt[j].data_ptr(), _lib.stream_ptr()),
+                           "btx_compact_dense")
+        p.ring_d = ring_d
+        p.ring_idx = ring_idx.data_ptr()
+        p.ring_cnt = ring_cnt.data_ptr()
+        p.ring_cursor = cursor
+        p.nbins = len(sp.bins)
+        for j, (age, sub) in enumerate(sp.bins):
+            p.bin_delay[j] = age
+            p.bin_sub[j] = sub
+        # outputs
+        nwords = (n + 63) // 64
+        raster = torch.empty((T, lanes, nwords), dtype=torch.int64, device=dev)
+        p.raster = raster.data_ptr()
+        if teacher is not None:
+            if teacher.shape != (T, lanes, nwords) or teacher.dtype != torch.int64 \
+                    or not teacher.is_contiguous():
+                raise ValueError("teacher must be contiguous int64 (T, lanes, ceil(n/64))")
+            p.teacher = teacher.data_ptr()
+        recs = {}
+        for key in ("z", "v", "refractory", "psc", "v_pre", "v_prev"):
+            if key in record:
+                recs[key] = torch.empty((T, lanes, n), dtype=torch.float32, device=dev)
+        p.rec_z = recs["z"].data_ptr() if "z" in recs else None
+        p.rec_v = recs["v"].data_ptr() if "v" in recs else None
+        p.rec_refractory = recs["refractory"].data_ptr() if "refractory" in recs else None
+        p.rec_psc = recs["psc"].data_ptr() if "psc" in recs else None
+        p.rec_v_pre = recs["v_pre"].data_ptr() if "v_pre" in recs else None
+        p.rec_v_prev = recs["v_prev"].data_ptr() if "v_prev" in recs else None
+        ws = self._workspace(lanes, n, p.slots, p.ordered, dev)
+        p.workspace = ws.data_ptr()
+        p.workspace_bytes = ws.numel()
+        stream = _lib.stream_ptr()
+        if timings is None:
+            _lib.check(_lib.load().btx_rsnn_run(p, T, stream), "btx_rsnn_run")
+        else:
+            import ctypes
+            ms = (ctypes.c_double * 4)()
+            calls = (ctypes.c_int64 * 4)()
+            _lib.check(_lib.load().btx_rsnn_run_timed(p, T, stream, ms, calls),
+                       "btx_rsnn_run_timed")
+            for k, name in enumerate(("step", "compact", "push", "finish")):
+                timings[name] = (float(ms[k]), int(calls[k]))
+        if check_overflow and p.ordered:
+            if _lib.load().btx_rsnn_check_overflow(ws.data_ptr(), stream):
+                raise RuntimeError("ordered accumulation overflow list filled up; raise "
+                                   "sparse_config slots")
+        self._writeback_history(raster, T, lanes, n, cursor)
+        return raster, recs
+
+    def _writeback_history(self, raster, T, lanes, n, cursor):
+        hist = self.spec.history
+        if hist is None:
+            return
+        D = self.spec.ring_d
+        h = hist.history
+        hv = h.reshape(lanes, D, n)
+        take = min(T, D)
+        recent = unpack_bits(raster[T - take:], n)  # (take, lanes, n), oldest first
+        if hist.use_circular_buffer:
+            new = hv.clone()
+            for i in range(take):
+                t = T - take + i
+                new[:, (cursor + t) % D] = recent[i]
+            hist.history = new.reshape(h.shape)
+            hist.set_cursor(cursor + T)
+        else:
+            newest_first = recent.flip(0).transpose(0, 1)  # (lanes, take, n)
+            parts = [newest_first]
+            if take < D:
+                parts.append(hv[:, : D - take])
+            hist.history = torch.cat(parts, dim=1).reshape(h.shape)
+
+    def history_sequence(self, raster, hist0, T, lanes, n, cursor):
+        """(T, lanes, D, n) history memory after every step (for recording)."""
+        D = self.spec.ring_d
+        hist = self.spec.history
+        z = unpack_bits(raster, n)                   # (T, lanes, n)
+        init = hist0.reshape(lanes, D, n)
+        out = torch.empty((T, lanes, D, n), dtype=torch.float32, device=z.device)
+        for t in range(T):
+            for j in range(D):
+                if hist.use_circular_buffer:
+                    back = (t - (j - cursor)) % D     # steps since slot j was written
+                    ts = t - back
+                    out[t, :, j] = z[ts] if ts >= 0 else init[:, j]
+                else:
+                    ts = t - j
+                    out[t, :, j] = z[ts] if ts >= 0 else init[:, j - t - 1]
+        return out

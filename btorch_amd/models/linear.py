@@ -1,0 +1,247 @@
+"""Sparse connectivity linear: ``y = x @ conn`` for a fixed sparse ``conn``.
+
+Mirrors btorch/models/linear.py:222-591 (``BaseSparseConn``/``SparseConn``):
+constructor ``SparseConn(conn, bias=None, enforce_dale=True,
+sparse_backend=None, sparse_config=None, device=None, dtype=None)``; buffer
+``indices`` (2, E) int64 holding the coalesced (dst, src) pairs and
+Parameter ``magnitude`` (E,) in that order; ``prepare_sparse``/
+``finish_sparse``; ``get_sparse_matrix``; ``constrain``.
+
+The product runs in the HIP library as an event-driven push over a
+source-major CSR (``_csr_*`` non-persistent buffers, rebuilt on load): the
+input is compacted to its nonzero sources per lane, and each active row is
+streamed to its destinations.  The default ordered mode sums every
+destination in ascending source order with fmaf, which equals the CPU
+``torch.sparse.mm`` of the reference's native branch bit for bit.
+"""
+
+from __future__ import annotations
+
+from typing import Any
+
+import scipy.sparse
+import torch
+import torch.nn as nn
+
+from .. import _kernels, _lib
+from .. import config as btorch_config
+from .._sparse_config import HipSparseConfig, SparseBackendConfig
+
+
+def _resolve_sparse_backend(name: str | None) -> str:
+    if name is None:
+        name = btorch_config.sparse.selected or "hip"
+    name = name.lower()
+    if name == "torch_sparse":
+        name = "triton"
+    if name not in btorch_config.sparse.names():
+        raise ValueError(
+            f"sparse_backend must be one of {btorch_config.sparse.names()}, got '{name}'."
+        )
+    return name
+
+
+def available_sparse_backends(*, include_experimental: bool = False) -> list[str]:
+    """Backend names accepted by ``SparseConn`` (all run the HIP product)."""
+    return btorch_config.sparse.names()
+
+
+class _SparseProduct(torch.autograd.Function):
+    """y = x @ W with W packed in CSR order; grads for x and the packed weights
+    (autograd carries the latter back to ``magnitude`` through the gather)."""
+
+    @staticmethod
+    def forward(ctx, x2d, w_csr, conn: "BaseSparseConn"):
+        active, counts = _kernels.compact_dense(x2d)
+        ws = conn._workspace_for(x2d.shape[0])
+        cfg: HipSparseConfig = conn.sparse_config
+        w_dst = conn._get_effective_weight().detach().to(torch.float32).contiguous()
+        out = _kernels.spmv_event(conn._layout, w_csr, x2d, active, counts,
+                                  ordered=cfg.ordered, slots=cfg.slots, workspace=ws,
+                                  w_dst=w_dst)
+        ctx.conn = conn
+        ctx.save_for_backward(x2d, w_csr, active, counts)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        x2d, w_csr, active, counts = ctx.saved_tensors
+        layout = ctx.conn._layout
+        gx = gw = None
+        if ctx.needs_input_grad[0]:
+            gx = _kernels.spmv_transpose(layout, w_csr, g)
+        if ctx.needs_input_grad[1]:
+            gw = torch.zeros(layout.nnz, dtype=torch.float32, device=g.device)
+            _kernels.spmv_weight_grad(layout, x2d, active, counts, g, gw)
+        return gx, gw, None
+
+
+class BaseSparseConn(nn.Module):
+    """Fixed sparse connection ``conn`` of shape (n_src, n_dst); see module doc."""
+
+    def __init__(self, conn, bias=None, sparse_backend: str | None = None,
+                 sparse_config: SparseBackendConfig | dict[str, Any] | None = None,
+                 device=None, dtype=None):
+        super().__init__()
+        self.sparse_backend = _resolve_sparse_backend(sparse_backend)
+        self.sparse_config = btorch_config.sparse.resolve(self.sparse_backend, sparse_config)
+        self._prepared_sparse_weight: torch.Tensor | None = None
+        self._sparse_prepare_depth = 0
+        self._workspaces: dict[int, torch.Tensor] = {}
+        if isinstance(conn, torch.Tensor) and conn.is_sparse:
+            # torch COO (n_src, n_dst): transpose and coalesce on its own device
+            # (sorted by (dst, src), duplicates summed), e.g. on the GPU
+            self.in_features, self.out_features = conn.shape
+            t = conn.t().coalesce()
+            indices = t.indices().to(device=device, dtype=torch.long)
+            value = t.values().to(device=device, dtype=dtype or torch.float32)
+        else:
+            if not isinstance(conn, scipy.sparse.coo_array):
+                conn = scipy.sparse.coo_array(conn)
+            self.in_features, self.out_features = conn.shape
+            conn_t = conn.T.tocoo()
+            conn_t.sum_duplicates()  # sorted by (dst, src)
+            indices = torch.stack([
+                torch.as_tensor(conn_t.row, dtype=torch.long),
+                torch.as_tensor(conn_t.col, dtype=torch.long),
+            ]).to(device)
+            value = torch.as_tensor(conn_t.data, dtype=dtype or torch.float32).to(device)
+        self.shape = (self.in_features, self.out_features)
+        self.register_buffer("indices", indices)
+        self._rebuild_layout()
+        self.bias = nn.Parameter(bias) if bias is not None else None
+        self._init_weights(value)
+
+    # -- layout --------------------------------------------------------------
+    def _rebuild_layout(self) -> None:
+        lay = _kernels.build_csr(self.indices[0], self.indices[1], self.in_features,
+                                 self.out_features)
+        for name, t in (("_csr_row_ptr", lay.row_ptr), ("_csr_col", lay.col),
+                        ("_csr_perm", lay.perm), ("_csc_ptr", lay.csc_ptr),
+                        ("_csc_src", lay.csc_src)):
+            if name in self._buffers:
+                setattr(self, name, t)
+            else:
+                self.register_buffer(name, t, persistent=False)
+        self._workspaces = {}
+
+    @property
+    def _layout(self) -> _kernels.CsrLayout:
+        return _kernels.CsrLayout(self._csr_row_ptr, self._csr_col, self._csr_perm,
+                                  self.in_features, self.out_features, self._csc_ptr,
+                                  self._csc_src)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.indices.shape[1])
+
+    def _workspace_for(self, lanes: int) -> torch.Tensor:
+        cfg: HipSparseConfig = self.sparse_config
+        dev = self._csr_col.device
+        need = _lib.load().btx_spmv_workspace_bytes(lanes, self.out_features, cfg.slots,
+                                                    int(cfg.ordered))
+        key = (int(cfg.ordered), cfg.slots, str(dev))
+        ws = self._workspaces.get(key)
+        if ws is None or ws.numel() < need:
+            ws = torch.zeros(need, dtype=torch.uint8, device=dev)
+            self._workspaces[key] = ws
+        return ws
+
+    def _apply(self, fn, recurse=True):
+        self._prepared_sparse_weight = None
+        self._sparse_prepare_depth = 0
+        self._workspaces = {}
+        return super()._apply(fn, recurse=recurse)
+
+    def _load_from_state_dict(self, state_dict, prefix, local_metadata, strict, missing_keys,
+                              unexpected_keys, error_msgs):
+        super()._load_from_state_dict(state_dict, prefix, local_metadata, strict, missing_keys,
+                                      unexpected_keys, error_msgs)
+        self._prepared_sparse_weight = None
+        self._sparse_prepare_depth = 0
+        self._rebuild_layout()
+
+    # -- weights -------------------------------------------------------------
+    def _init_weights(self, value: torch.Tensor):
+        raise NotImplementedError
+
+    def _get_effective_weight(self) -> torch.Tensor:
+        raise NotImplementedError
+
+    def packed_weight(self) -> torch.Tensor:
+        """Effective weights in CSR order (``magnitude[perm]``)."""
+        if self._prepared_sparse_weight is not None:
+            return self._prepared_sparse_weight
+        w = self._get_effective_weight()
+        return w.index_select(0, self._csr_perm).contiguous()
+
+    def prepare_sparse(self, batch_size: int | None = None) -> None:
+        """Pack weights once for a multi-step run (reference-counted)."""
+        if self._prepared_sparse_weight is None:
+            w = self._get_effective_weight()
+            packed = w.index_select(0, self._csr_perm).contiguous()
+            self._prepared_sparse_weight = packed if torch.is_grad_enabled() else packed.detach()
+        if batch_size is not None and self._csr_col.is_cuda:
+            self._workspace_for(int(batch_size))
+        self._sparse_prepare_depth += 1
+
+    def finish_sparse(self) -> None:
+        if self._sparse_prepare_depth <= 0:
+            raise RuntimeError("finish_sparse called without prepare_sparse.")
+        self._sparse_prepare_depth -= 1
+        if self._sparse_prepare_depth == 0:
+            self._prepared_sparse_weight = None
+
+    def get_sparse_matrix(self) -> torch.Tensor:
+        idx = torch.stack([self.indices[1], self.indices[0]], dim=0)
+        return torch.sparse_coo_tensor(idx, self._get_effective_weight(),
+                                       size=(self.in_features, self.out_features),
+                                       is_coalesced=True)
+
+    # -- forward -------------------------------------------------------------
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        _lib.require_hip(x, self._csr_col)
+        no_batch = x.ndim == 1
+        if no_batch:
+            x = x[None, :]
+        lead = x.shape[:-1]
+        x2d = x.reshape(-1, x.shape[-1])
+        if x2d.dtype != torch.float32:
+            raise TypeError("The HIP sparse backend currently supports float32 only.")
+        x2d = x2d.contiguous()
+        out = _SparseProduct.apply(x2d, self.packed_weight(), self)
+        out = out.reshape(*lead, self.out_features)
+        if no_batch:
+            out = out[0]
+        if self.bias is not None:
+            out = out + self.bias
+        return out
+
+
+class SparseConn(BaseSparseConn):
+    """Sparse linear with one learnable magnitude per connection; optional
+    Dale's law via a fixed sign and ReLU projection in ``constrain``."""
+
+    def __init__(self, conn, bias=None, enforce_dale: bool = True,
+                 sparse_backend: str | None = None,
+                 sparse_config: SparseBackendConfig | dict[str, Any] | None = None,
+                 device=None, dtype=None):
+        self.enforce_dale = enforce_dale
+        super().__init__(conn, bias=bias, sparse_backend=sparse_backend,
+                         sparse_config=sparse_config, device=device, dtype=dtype)
+
+    def _init_weights(self, value: torch.Tensor):
+        if self.enforce_dale:
+            self.register_buffer("initial_sign", torch.sign(value), persistent=False)
+        self.magnitude = nn.Parameter(value)
+
+    def _get_effective_weight(self) -> torch.Tensor:
+        return self.magnitude
+
+    @torch.no_grad()
+    def constrain(self, *args, **kwargs):
+        if self.enforce_dale:
+            self.magnitude.data = (self.magnitude * self.initial_sign).relu() * self.initial_sign
+
+
+__all__ = ["BaseSparseConn", "SparseConn", "available_sparse_backends"]

@@ -1,0 +1,253 @@
+"""Time-unrolled recurrent loop (btorch/models/rnn.py:27-528).
+
+``RecurrentNN(neuron, synapse, syn_inp_module=None, neuron_inp_module=None,
+*, update_state_names=None, unroll=8, chunk_size=None, cpu_offload=False,
+grad_checkpoint=False, allow_buffer=False)``.  ``model(x_seq)`` in step mode
+"m" returns ``(z_seq, states)`` with ``z_seq`` (T, *batch, N) and ``states``
+a dict of stacked per-step hidden states (all of them when
+``update_state_names`` is None), exactly as the reference.
+
+Cells the fused engine supports (see ``_engine``) run their whole sequence
+in the HIP library; with gradients enabled they run chunk by chunk through
+the hand-written adjoint with recompute checkpointing every ``chunk_size``
+steps.  Other cells run the per-step loop of ``single_step_forward``.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Sequence
+
+import torch
+import torch.nn as nn
+
+from . import base
+from ._engine import Engine, match_cell
+from .functional import filter_hidden_states, named_hidden_states, prepare_sparse_modules
+from .linear import BaseSparseConn
+
+
+class RecurrentNNAbstract(base.MemoryModule):
+    """Loop driver; subclasses define ``single_step_forward -> (z, states)``."""
+
+    def __init__(self, update_state_names: Sequence[str] | None = None, step_mode="m",
+                 unroll: int | bool = 8, chunk_size: int | None = None,
+                 cpu_offload: bool = False, grad_checkpoint: bool = False,
+                 save_grad_history: bool = False, grad_state_names=None,
+                 cudagraph: bool = False, cudagraph_warmup: int = 3):
+        super().__init__()
+        self.step_mode = step_mode
+        self.update_state_names = update_state_names
+        self.unroll = unroll
+        self.chunk_size = chunk_size
+        self.cpu_offload = cpu_offload
+        self.grad_checkpoint = grad_checkpoint
+        self.save_grad_history = save_grad_history
+        self.grad_state_names = grad_state_names
+        self.cudagraph = cudagraph
+        self.cudagraph_warmup = cudagraph_warmup
+        self.check_overflow = True
+
+    def single_step_forward(self, *args, **kwargs):
+        raise NotImplementedError
+
+    def _detect_loop_args(self, *args):
+        if len(args) == 1:
+            return args[0].shape[0], (0,)
+        shapes = [a.shape[0] if torch.is_tensor(a) and a.ndim > 0 else None for a in args]
+        T = shapes[0]
+        assert T is not None
+        return T, tuple(i for i, s in enumerate(shapes) if s == T)
+
+    def _chunk_plan(self, *args, loop_args=None):
+        if loop_args is None:
+            T, loop_args = self._detect_loop_args(*args)
+        else:
+            T = args[loop_args[0]].shape[0]
+        unroll = T if self.unroll is False else int(self.unroll)
+        if self.chunk_size is None:
+            large = unroll if self.grad_checkpoint else T
+        else:
+            large = self.chunk_size
+            if self.unroll is not False and large % unroll != 0:
+                raise ValueError(f"chunk_size ({large}) must be a multiple of unroll ({unroll})")
+        return T, loop_args, unroll, large, (T + large - 1) // large
+
+    def multi_step_forward(self, *args, loop_args=None, **kwargs):
+        first = args[0]
+        batch = (first[0].numel() // first.shape[-1]
+                 if torch.is_tensor(first) and first.ndim >= 2 else None)
+        with prepare_sparse_modules(self, batch_size=batch):
+            return self._multi_step_loop(*args, loop_args=loop_args, **kwargs)
+
+    def _multi_step_loop(self, *args, loop_args=None, **kwargs):
+        T, loop_args, _, large, nchunks = self._chunk_plan(*args, loop_args=loop_args)
+        zs, states = [], {}
+        for c in range(nchunks):
+            lo, hi = c * large, min(T, (c + 1) * large)
+            static = list(args)
+            for t in range(lo, hi):
+                for i in loop_args:
+                    static[i] = args[i][t]
+                z, st = self.single_step_forward(*static, **kwargs)
+                if self.cpu_offload:
+                    z = z.cpu()
+                    st = {k: v.cpu() for k, v in st.items()}
+                zs.append(z)
+                for k, v in st.items():
+                    states.setdefault(k, []).append(v)
+        return torch.stack(zs, 0), {k: torch.stack(v, 0) for k, v in states.items()}
+
+
+def make_rnn(obj=None, allow_buffer=False, **rnn_kwargs):
+    """Wrap a MemoryModule (class, instance or decorator use) in the loop."""
+
+    def build(cell):
+        class RNNWrapped(RecurrentNNAbstract):
+            def __init__(self, *args, **kwargs):
+                super().__init__(**rnn_kwargs)
+                self.rnn_cell = cell(*args, **kwargs) if isinstance(cell, type) else cell
+
+            def single_step_forward(self, *args, **kwargs):
+                out = self.rnn_cell(*args, **kwargs)
+                return out, filter_hidden_states(self.rnn_cell, self.update_state_names,
+                                                 allow_buffer=allow_buffer)
+
+        RNNWrapped.__name__ = f"RNN_{getattr(cell, '__name__', type(cell).__name__)}"
+        return RNNWrapped
+
+    if isinstance(obj, type):
+        return build(obj)
+    if isinstance(obj, base.MemoryModule):
+        return build(obj)()
+    if obj is None:
+        return build
+    raise TypeError("`make_rnn` expects a MemoryModule class, a MemoryModule instance, or None.")
+
+
+class RecurrentNN(RecurrentNNAbstract):
+    """z_t = neuron(synapse.psc + x_t); synapse(z_t) -- btorch/models/rnn.py:517-528."""
+
+    def __init__(self, neuron: nn.Module, synapse: nn.Module, syn_inp_module=None,
+                 neuron_inp_module=None, *, update_state_names=None, unroll=8,
+                 chunk_size=None, cpu_offload=False, grad_checkpoint=False, allow_buffer=False,
+                 **kwargs):
+        super().__init__(update_state_names=update_state_names, unroll=unroll,
+                         chunk_size=chunk_size, cpu_offload=cpu_offload,
+                         grad_checkpoint=grad_checkpoint, **kwargs)
+        self.neuron = neuron
+        self.synapse = synapse
+        self.neuron_inp_module = neuron_inp_module
+        self.syn_inp_module = syn_inp_module
+        self.allow_buffer = allow_buffer
+        self._engine: Engine | None = None
+        self._engine_key = None
+
+    def single_step_forward(self, x, x_syn=None):
+        if self.neuron_inp_module is not None:
+            x = self.neuron_inp_module(x)
+        if self.syn_inp_module is not None:
+            x_syn = self.syn_inp_module(x_syn)
+        z = self.neuron(self.synapse.psc + x)
+        self.synapse(z if x_syn is None else z + x_syn)
+        return z, filter_hidden_states(self, self.update_state_names,
+                                       allow_buffer=self.allow_buffer)
+
+    # -- fused path ----------------------------------------------------------
+    def fused_engine(self) -> Engine | None:
+        key = (id(self.neuron), id(self.synapse), id(self.syn_inp_module))
+        if self._engine is None or self._engine_key != key:
+            spec = match_cell(self)
+            self._engine = Engine(spec) if spec is not None else None
+            self._engine_key = key
+        return self._engine
+
+    def _record_names(self) -> list[str]:
+        names = list(named_hidden_states(self, self.update_state_names,
+                                         allow_buffer=self.allow_buffer).keys())
+        return names
+
+    def _multi_step_loop(self, *args, loop_args=None, **kwargs):
+        eng = self.fused_engine()
+        x = args[0] if args else None
+        if (eng is None or len(args) != 1 or kwargs or not torch.is_tensor(x)
+                or x.device.type != "cuda"):
+            return super()._multi_step_loop(*args, loop_args=loop_args, **kwargs)
+        from ._adjoint import needs_grad, run_with_grad
+
+        if needs_grad(self, x):
+            return run_with_grad(self, eng, x)
+        with torch.no_grad():
+            return self._fused_forward(eng, x)
+
+    def _input_sequence(self, x: torch.Tensor, n: int):
+        T = x.shape[0]
+        if self.neuron_inp_module is not None:
+            mod = self.neuron_inp_module
+            flat = x.reshape(-1, x.shape[-1])
+            x = mod(flat) if isinstance(mod, BaseSparseConn) else mod(flat)
+        lead = x.shape[: -1] if x.ndim > 1 else ()
+        x = x.reshape(T, -1, n) if x.numel() else x
+        return x.to(torch.float32).contiguous(), lead
+
+    def _fused_forward(self, eng: Engine, x: torch.Tensor):
+        sp = eng.spec
+        n = sp.neuron.size
+        T = x.shape[0]
+        xs, _ = self._input_sequence(x, n)
+        lanes = xs.shape[1]
+        names = self._record_names()
+        want = self._wanted_records(names, sp)
+        hist0 = sp.history.history.clone() if (sp.history is not None and
+                                               "history" in want) else None
+        cursor0 = sp.history.cursor if sp.history is not None else 0
+        raster, recs = eng.run(xs, T, lanes, want, check_overflow=self.check_overflow)
+        return self._assemble(eng, names, recs, raster, hist0, cursor0, T, lanes, n)
+
+    @staticmethod
+    def _wanted_records(names, sp) -> set[str]:
+        want = {"z"}
+        for k in names:
+            if k == "neuron.v":
+                want.add("v")
+            elif k == "neuron.refractory":
+                want.add("refractory")
+            elif k == "neuron.v_pre_spike":
+                want.add("v_pre")
+            elif k in sp.psc_keys:
+                want.add("psc")
+            elif k.endswith("history.history"):
+                want.add("history")
+            else:
+                raise NotImplementedError(f"recording '{k}' is not supported by the fused engine")
+        return want
+
+    def _assemble(self, eng, names, recs, raster, hist0, cursor0, T, lanes, n):
+        sp = eng.spec
+        shape = tuple(sp.neuron.v.shape)
+        z = recs["z"].reshape(T, *shape)
+        states = {}
+        for k in names:
+            if k == "neuron.v":
+                states[k] = recs["v"].reshape(T, *shape)
+            elif k == "neuron.refractory":
+                states[k] = recs["refractory"].reshape(T, *shape)
+            elif k == "neuron.v_pre_spike":
+                states[k] = recs["v_pre"].reshape(T, *shape)
+            elif k in sp.psc_keys:
+                states[k] = recs["psc"].reshape(T, *shape)
+            elif k.endswith("history.history"):
+                seq = eng.history_sequence(raster, hist0, T, lanes, n, cursor0)
+                states[k] = seq.reshape(T, *sp.history.history.shape)
+        if sp.neuron.pre_spike_v and "v_pre" in recs:
+            sp.neuron.v_pre_spike = recs["v_pre"][-1].reshape(shape).clone()
+        # HeterSynapsePSC keeps its own receptor-summed copy of psc
+        for m in sp.psc_modules:
+            if m is not sp.exp_psc:
+                m.psc = sp.exp_psc.psc.reshape(shape).clone()
+        if self.cpu_offload:
+            z = z.cpu()
+            states = {k: v.cpu() for k, v in states.items()}
+        return z, states
+
+
+__all__ = ["RecurrentNNAbstract", "RecurrentNN", "make_rnn"]

@@ -1,0 +1,218 @@
+/*
+ * btorch_amd.h -- C ABI of libbtx.so, the gfx950 HIP kernels behind the
+ * recurrent spiking-network time step (spike x sparse connectome, fused
+ * neuron/synapse state update, delay ring, surrogate-gradient adjoint).
+ *
+ * Conventions
+ *  - Every pointer is a device pointer unless stated otherwise.
+ *  - No entry point allocates device memory: callers pass outputs and a
+ *    workspace sized by btx_*_workspace_bytes().
+ *  - Work is enqueued on `stream` (a hipStream_t passed as void*); nothing
+ *    synchronises unless stated.
+ *  - Return 0 on success, otherwise a nonzero status; btx_last_error() returns
+ *    a thread-local message describing the last failure.
+ *  - All floating-point state is IEEE binary32.  Kernels are compiled with
+ *    -ffp-contract=off so each expression rounds exactly where the reference's
+ *    eager op sequence rounds.
+ */
+#ifndef BTORCH_AMD_H
+#define BTORCH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BTX_MAX_BINS 64
+
+/* Library version (major*10000 + minor*100 + patch). */
+int btx_version(void);
+
+/* Thread-local description of the last nonzero status. */
+const char* btx_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Spike compaction                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Ascending indices of the nonzero entries of each lane of a dense fp32
+ * matrix x[lanes, n] (row stride x_lane_stride).  idx[lane*idx_lane_stride + k]
+ * receives the k-th nonzero column, counts[lane] the number of them.
+ * Replaces the "any source of this task spiked?" scan of the Triton backend
+ * (btorch/models/_sparse_triton.py:20-57) and equals numpy.nonzero per lane. */
+int btx_compact_dense(const float* x, int64_t lanes, int64_t n, int64_t x_lane_stride,
+                      int32_t* idx, int64_t idx_lane_stride, int32_t* counts,
+                      void* stream);
+
+/* Same, from a spike bitmask words[lanes, ceil(n/64)] (bit j of word w is
+ * column 64*w + j).  Used by the time-step engine on the raster it writes. */
+int btx_compact_bits(const uint64_t* words, int64_t lanes, int64_t n,
+                     int64_t words_lane_stride, int32_t* idx, int64_t idx_lane_stride,
+                     int32_t* counts, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Event-driven sparse product (SparseConn forward)                          */
+/* ------------------------------------------------------------------------ */
+
+/* Bytes of workspace an event-driven product with `lanes` lanes over
+ * `n_dst` destinations needs.  `slots` is the per-destination slot count of
+ * the ordered mode (ignored when ordered == 0).  The workspace must be
+ * zero-filled once after allocation; the kernels leave it zeroed. */
+size_t btx_spmv_workspace_bytes(int64_t lanes, int64_t n_dst, int slots, int ordered);
+
+/* out[lane, :] = sum over active sources s of x[lane, s] * W[s, :], with W
+ * stored source-major (CSR: row_ptr[n_src+1] int64, col[E] int32 destination,
+ * w[E] fp32, E = nnz).  active/counts are the compaction of x (btx_compact_dense).
+ * x may be NULL, meaning every active entry is 1.0 (spikes).
+ * csc_ptr[n_dst+1] / csc_src[E] / csc_w[E] are the same matrix destination-major
+ * (the reference's coalesced (dst, src) order: indices/magnitude); the ordered
+ * mode reduces a destination with more than `slots` terms by pulling over
+ * them, so its result never depends on a bounded overflow list.
+ * ordered != 0: each destination sums its terms in ascending source order
+ * with a fresh fp32 accumulator (fmaf(w, x, acc)), which is bit-identical to
+ * torch.sparse.mm on the coalesced (dst, src) COO on CPU -- the reference's
+ * native branch, btorch/models/linear.py:474-483.  ordered == 0 accumulates
+ * with float atomics (the reference's Triton order-free semantics,
+ * btorch/models/_sparse_triton.py:59-96).
+ * Replaces triton_sparse_mm (btorch/models/_sparse_triton.py:418-444). */
+int btx_spmv_event(const int64_t* row_ptr, const int32_t* col, const float* w,
+                   int64_t n_src, int64_t n_dst, int64_t nnz,
+                   const int64_t* csc_ptr, const int32_t* csc_src, const float* csc_w,
+                   const int32_t* active, const int32_t* counts, int64_t lanes,
+                   int64_t active_lane_stride,
+                   const float* x, int64_t x_lane_stride,
+                   float* out, int ordered, int slots,
+                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* Transposed product for the backward of SparseConn:
+ * gx[lane, s] = sum_e w[e] * g[lane, col[e]] over the CSR row s
+ * (every row; g is dense).  Replaces the grad_x index_add of
+ * btorch/models/_sparse_triton.py:404-410. */
+int btx_spmv_transpose(const int64_t* row_ptr, const int32_t* col, const float* w,
+                       int64_t n_src, int64_t n_dst, const float* g, int64_t lanes,
+                       float* gx, void* stream);
+
+/* Event-driven weight gradient: gw[e] += sum over lanes with x[lane, s] != 0
+ * of x[lane, s] * g[lane, col[e]] for every edge e of every active row s.
+ * gw is indexed in CSR order.  Replaces the dense grad_weight of
+ * btorch/models/_sparse_triton.py:411-414. */
+int btx_spmv_weight_grad(const int64_t* row_ptr, const int32_t* col, int64_t n_src,
+                         int64_t n_dst, const int32_t* active, const int32_t* counts,
+                         int64_t lanes, int64_t active_lane_stride, const float* x,
+                         int64_t x_lane_stride, const float* g, float* gw,
+                         void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused time-step engine                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Neuron models of the fused step. */
+#define BTX_NEURON_LIF 0      /* btorch/models/neurons/lif.py:144-187 */
+#define BTX_NEURON_GLIF3 1    /* btorch/models/neurons/glif.py:264-342 */
+
+/* Surrogate functions (forward Heaviside, backward derivative). */
+#define BTX_SURR_SIGMOID 0    /* btorch/models/surrogate/sigmoid.py:16-29 */
+#define BTX_SURR_SUPERSPIKE 1 /* btorch/models/surrogate/superspike.py:16-31 */
+#define BTX_SURR_ERF 2        /* btorch/models/surrogate/erf.py:22-28 */
+
+/* One recurrent population: neuron (LIF) + exponential current synapse +
+ * sparse recurrent connectivity, optionally through an integer-delay ring.
+ * Follows RecurrentNN.single_step_forward (btorch/models/rnn.py:517-528):
+ *   I_t   = psc_{t-1} + x_t
+ *   z_t   = neuron(I_t)                         (charge -> fire -> reset)
+ *   psc_t = psc_{t-1} + k*((-psc_{t-1})/tau_s)  (exp-Euler decay)
+ *           + W_exp^T * flatten(history after pushing z_t)
+ * Per-neuron parameters are fp32 arrays with a stride (0 = one shared value).
+ * Every pointer refers to device memory that outlives the enqueued work. */
+typedef struct btx_rsnn_plan {
+    int64_t n;            /* neurons */
+    int64_t lanes;        /* batch lanes */
+    int neuron_model;     /* BTX_NEURON_* */
+    int hard_reset;       /* 0 = soft (v - (vth - vr) s), 1 = hard (v - (v - vr) s) */
+    int use_refractory;   /* 1 = refractory counter enabled */
+    int ordered;          /* 1 = ascending-source bit-exact accumulation */
+    int slots;            /* per-destination slots for the ordered mode */
+    float dt;             /* environ dt as fp32 */
+    /* neuron parameters, (n,) or stride 0 */
+    const float* tau;      int64_t tau_stride;
+    const float* v_th;     int64_t v_th_stride;
+    const float* v_reset;  int64_t v_reset_stride;
+    const float* c_m;      int64_t c_m_stride;
+    const float* tau_ref;  int64_t tau_ref_stride;
+    /* synapse: psc_dec = psc + k*((-psc)/tau_s) ; k = expm1(dt*l)/l, l=-1/tau_s */
+    const float* tau_syn;  int64_t tau_syn_stride;
+    const float* k_syn;    int64_t k_syn_stride;
+    /* state, (lanes, n) contiguous, updated in place */
+    float* v;
+    float* refractory;    /* may be NULL when use_refractory == 0 */
+    float* psc;
+    /* external input x[t, lane, n], NULL = none; x_step_stride elements per step */
+    const float* x;       int64_t x_step_stride;
+    /* connectivity: CSR over virtual rows n*csr_d + bin (int64 row_ptr) */
+    const int64_t* row_ptr;
+    const int32_t* col;
+    const float* w;
+    int64_t nnz;          /* edges (sizes the push group width) */
+    int csr_d;            /* virtual rows per neuron (delay-expanded bins) */
+    /* delay ring of compacted spike lists: ring_d slots x lanes x n */
+    int ring_d;
+    int32_t* ring_idx;    /* [ring_d][lanes][n] */
+    int32_t* ring_cnt;    /* [ring_d][lanes] */
+    int ring_cursor;      /* ring slot that receives z of the first step */
+    int nbins;            /* delay bins read each step */
+    int bin_delay[BTX_MAX_BINS];  /* ring age of bin j (0 = this step) */
+    int bin_sub[BTX_MAX_BINS];    /* virtual-row offset of bin j */
+    /* teacher forcing: spike words [T][lanes][ceil(n/64)] pushed into the ring
+     * instead of the produced spikes (NULL = free running) */
+    const uint64_t* teacher;
+    /* outputs */
+    uint64_t* raster;     /* [T][lanes][ceil(n/64)] spike bits, required */
+    float* rec_z;         /* [T][lanes][n] or NULL */
+    float* rec_v;         /* [T][lanes][n] or NULL */
+    float* rec_refractory;/* [T][lanes][n] or NULL */
+    float* rec_psc;       /* [T][lanes][n] or NULL */
+    float* rec_v_pre;     /* [T][lanes][n] pre-reset membrane or NULL */
+    float* rec_v_prev;    /* [T][lanes][n] membrane entering the step or NULL */
+    /* workspace from btx_rsnn_workspace_bytes(), zero-filled once */
+    void* workspace;      size_t workspace_bytes;
+} btx_rsnn_plan;
+
+size_t btx_rsnn_workspace_bytes(int64_t lanes, int64_t n, int slots, int ordered);
+
+/* Run steps [0, n_steps) of the plan.  Records index t in [0, n_steps).
+ * State (v, refractory, psc, ring) is final on return (stream order). */
+int btx_rsnn_run(const btx_rsnn_plan* plan, int64_t n_steps, void* stream);
+
+/* One neuron step without synapse (LIF.single_step_forward, lif.py:144-187,
+ * base.py:1108-1116): reads the plan's neuron parameters and state v /
+ * refractory, input x[lanes, n]; writes spikes z[lanes, n] (fp32 0/1) and,
+ * if v_pre is not NULL, the membrane before reset.  Synapse, ring and
+ * connectivity fields of the plan are ignored. */
+int btx_lif_step(const btx_rsnn_plan* plan, const float* x, float* z, float* v_pre,
+                 void* stream);
+
+/* psc[lanes, n] += k*((-psc)/tau_s) in place: ExponentialPSC.conductance_charge
+ * (btorch/models/synapse.py:171-178 with exp_euler_step, btorch/models/ode.py:20-50).
+ * k = expm1(dt*l)/l with l = -1/tau_s, evaluated by the caller. */
+int btx_exp_decay(float* psc, int64_t lanes, int64_t n, const float* tau_syn,
+                  int64_t tau_syn_stride, const float* k_syn, int64_t k_syn_stride,
+                  void* stream);
+
+/* btx_rsnn_run with a HIP event pair around every kernel launch of the
+ * step sequence, recorded on `stream`.  On return (after a stream
+ * synchronisation) ms_out[k] holds the summed duration of kernel kind k and
+ * calls_out[k] its launch count, k = 0 fused step, 1 compaction, 2 push,
+ * 3 final psc.  For measurement only: the events serialise the launches. */
+int btx_rsnn_run_timed(const btx_rsnn_plan* plan, int64_t n_steps, void* stream,
+                       double* ms_out, int64_t* calls_out);
+
+/* Nonzero when an ordered-mode overflow list filled up since the last call
+ * (results of such a run are invalid).  Reads 4 bytes synchronously. */
+int btx_rsnn_check_overflow(void* workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BTORCH_AMD_H */

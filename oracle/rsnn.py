@@ -1,0 +1,239 @@
+"""Torch-CPU restatement of btorch's recurrent SNN step -- TEST INFRASTRUCTURE ONLY.
+
+Every function performs the reference's eager torch ops in the reference's
+order, on CPU tensors, so in float32 it reproduces the reference's native
+path bit for bit; with ``dtype=torch.float64`` the same code is the float64
+error oracle.  Citations are to the reference (jiaxinou2022/btorch).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+import scipy.sparse
+import torch
+
+_HERE = Path(__file__).resolve().parent
+
+
+# ---------------------------------------------------------------------------
+# Connectivity
+# ---------------------------------------------------------------------------
+
+def coalesce(conn) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(dst, src, w) of ``conn`` (n_src, n_dst) transposed and coalesced,
+    sorted by (dst, src): BaseSparseConn.__init__, linear.py:274-291."""
+    coo = scipy.sparse.coo_array(conn)
+    t = coo.T.tocoo()
+    t.sum_duplicates()
+    return t.row.astype(np.int64), t.col.astype(np.int64), t.data.astype(np.float32)
+
+
+def expand_conn_for_delays(conn, delays, n_delay_bins: int):
+    """connection.py:973-1027: row pre*D + clip(delay, 0, D-1)."""
+    delays = np.asarray(delays)
+    if np.any(delays < 0):
+        raise ValueError("delays must be non-negative")
+    coo = scipy.sparse.coo_array(conn)
+    bins = np.clip(delays, 0, n_delay_bins - 1).astype(int)
+    rows = coo.row * n_delay_bins + bins
+    return scipy.sparse.coo_array((coo.data, (rows, coo.col)),
+                                  shape=(coo.shape[0] * n_delay_bins, coo.shape[1]))
+
+
+class NativeSparse:
+    """linear.py:474-483: out = (A_{dst,src} @ x^T)^T with torch.sparse.mm."""
+
+    def __init__(self, conn, dtype=torch.float32):
+        dst, src, w = coalesce(conn)
+        self.n_src, self.n_dst = scipy.sparse.coo_array(conn).shape
+        idx = torch.stack([torch.as_tensor(dst), torch.as_tensor(src)])
+        self.dst, self.src, self.w = dst, src, w
+        self.A = torch.sparse_coo_tensor(idx, torch.as_tensor(w).to(dtype),
+                                         size=(self.n_dst, self.n_src), is_coalesced=True)
+
+    @classmethod
+    def from_coalesced(cls, dst, src, w, n_src, n_dst, dtype=torch.float32):
+        """Same operator from (dst, src)-sorted coalesced arrays."""
+        obj = cls.__new__(cls)
+        obj.n_src, obj.n_dst = n_src, n_dst
+        obj.dst, obj.src, obj.w = dst, src, w
+        idx = torch.stack([torch.as_tensor(dst), torch.as_tensor(src)])
+        obj.A = torch.sparse_coo_tensor(idx, torch.as_tensor(w).to(dtype), size=(n_dst, n_src),
+                                        is_coalesced=True)
+        return obj
+
+    def __call__(self, x2d: torch.Tensor) -> torch.Tensor:
+        return torch.sparse.mm(self.A, x2d.T).T
+
+
+def seq_spmm_c(dst, src, w, x2d: np.ndarray, n_dst: int) -> np.ndarray:
+    """The C restatement of the per-destination sequential fmaf order."""
+    lib = ctypes.CDLL(str(_HERE / "liboracle.so"))
+    x2d = np.ascontiguousarray(x2d, dtype=np.float32)
+    lanes, n_src = x2d.shape
+    out = np.empty((lanes, n_dst), np.float32)
+    dst = np.ascontiguousarray(dst, np.int64)
+    src = np.ascontiguousarray(src, np.int64)
+    w = np.ascontiguousarray(w, np.float32)
+    P = ctypes.c_void_p
+    lib.seq_spmm.argtypes = [P, P, P, ctypes.c_int64, P, ctypes.c_int64, ctypes.c_int64,
+                             ctypes.c_int64, P]
+    lib.seq_spmm(dst.ctypes.data, src.ctypes.data, w.ctypes.data, len(w), x2d.ctypes.data,
+                 lanes, n_src, n_dst, out.ctypes.data)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Neuron and synapse
+# ---------------------------------------------------------------------------
+
+@dataclass
+class LIFParams:
+    """LIF parameters as 0-d or (N,) tensors (lif.py:94-128, base.py:1032-1045)."""
+
+    tau: torch.Tensor
+    v_threshold: torch.Tensor
+    v_reset: torch.Tensor
+    c_m: torch.Tensor
+    tau_ref: torch.Tensor | None
+    hard_reset: bool = False
+
+    @staticmethod
+    def make(n, tau=20.0, v_threshold=1.0, v_reset=0.0, c_m=1.0, tau_ref=None,
+             hard_reset=False, dtype=torch.float32):
+        def t(v):
+            return None if v is None else torch.as_tensor(v, dtype=dtype)
+        return LIFParams(t(tau), t(v_threshold), t(v_reset), t(c_m), t(tau_ref), hard_reset)
+
+
+def heaviside(x: torch.Tensor) -> torch.Tensor:
+    """surrogate/base.py:6-8."""
+    return (x >= 0).to(x)
+
+
+def lif_step(v, ref, x, p: LIFParams, dt: float):
+    """One LIF step (base.py:1108-1116 with lif.py:144-187).
+
+    Returns (spike, v_new, ref_new, v_pre)."""
+    derivative = -(v - p.v_reset) / p.tau + x / p.c_m
+    v = v + dt * derivative
+    v_pre = v
+    spike = heaviside((v - p.v_threshold) / (p.v_threshold - p.v_reset))
+    if p.tau_ref is not None:
+        spike = spike * (ref == 0).to(v.dtype)
+    if p.hard_reset:
+        v = v - (v - p.v_reset) * spike
+    else:
+        v = v - (p.v_threshold - p.v_reset) * spike
+    if p.tau_ref is not None:
+        ref = torch.relu(ref + spike * p.tau_ref - dt)
+    return spike, v, ref, v_pre
+
+
+def exp_psc_decay(psc, tau_syn, dt: float):
+    """ExponentialPSC.conductance_charge (synapse.py:171-178) through
+    exp_euler_step (ode.py:20-50): x + expm1(dt*A)/A * f(x)."""
+    derivative = -psc / tau_syn
+    linear = -1.0 / tau_syn
+    return psc + torch.expm1(dt * linear) / linear * derivative
+
+
+# ---------------------------------------------------------------------------
+# Delay ring (history.py:28-358)
+# ---------------------------------------------------------------------------
+
+class Ring:
+    """SpikeHistory buffer (B, D, N) in circular or cat mode."""
+
+    def __init__(self, lanes, n, D, circular=True, dtype=torch.float32, init=None, cursor=0):
+        self.D, self.n, self.circular = D, n, circular
+        self.h = torch.zeros(lanes, D, n, dtype=dtype) if init is None else init.clone()
+        self.cursor = cursor
+
+    def update(self, z):
+        if self.circular:
+            self.h[:, self.cursor] = z
+            self.cursor = (self.cursor + 1) % self.D
+        else:
+            self.h = torch.cat([z.unsqueeze(1), self.h[:, :-1]], dim=1)
+
+    def get_delay(self, d):
+        return self.h[:, (self.cursor - 1 - d) % self.D] if self.circular else self.h[:, d]
+
+    def get_flattened(self, n_delays):
+        stacked = torch.stack([self.get_delay(d) for d in range(n_delays)], dim=-1)
+        return stacked.reshape(stacked.shape[0], -1)  # index n*D + d
+
+
+# ---------------------------------------------------------------------------
+# RecurrentNN(LIF, ExponentialPSC-like(SparseConn)) time loop (rnn.py:517-528)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Net:
+    n: int
+    conn: object               # scipy (n_src, n_dst); n_src = n*csr_d
+    lif: LIFParams
+    tau_syn: torch.Tensor
+    dt: float
+    delay_mode: str = "none"   # "none" | "hetero" (HeterSynapsePSC) | "fixed" (DelayedPSC)
+    max_delay_steps: int = 1
+    circular: bool = False
+
+
+def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, teacher=None):
+    """Run T steps; x_seq (T, B, N).  ``teacher`` (T, B, N) forces the spikes
+    fed to the synapse (teacher forcing) while still reporting the produced
+    raster.  Returns dict of stacked z, v, refractory, psc, v_pre (T, B, N)
+    and the final state."""
+    T, B, n = x_seq.shape
+    p = LIFParams(*(None if t is None else t.to(dtype) for t in
+                    (net.lif.tau, net.lif.v_threshold, net.lif.v_reset, net.lif.c_m,
+                     net.lif.tau_ref)), net.lif.hard_reset)
+    tau_syn = torch.as_tensor(net.tau_syn).to(dtype)
+    mm = net.conn if isinstance(net.conn, NativeSparse) else NativeSparse(net.conn, dtype)
+    st = state or {}
+    # memory v starts at v_reset (register_memory("v", v_reset, ...), base.py:1022)
+    v0 = torch.zeros(B, n, dtype=dtype) + p.v_reset
+    v = st.get("v", v0).to(dtype).clone()
+    ref = st.get("refractory", torch.zeros(B, n, dtype=dtype)).to(dtype).clone()
+    psc = st.get("psc", torch.zeros(B, n, dtype=dtype)).to(dtype).clone()
+    ring = None
+    if net.delay_mode == "hetero" and net.max_delay_steps > 1:
+        ring = Ring(B, n, net.max_delay_steps, net.circular, dtype, st.get("history"),
+                    st.get("cursor", 0))
+    elif net.delay_mode == "fixed" and net.max_delay_steps > 1:
+        ring = Ring(B, n, net.max_delay_steps + 1, net.circular, dtype, st.get("history"),
+                    st.get("cursor", 0))
+    rec = {k: [] for k in ("z", "v", "refractory", "psc", "v_pre")}
+    x_seq = x_seq.to(dtype)
+    for t in range(T):
+        z, v, ref, v_pre = lif_step(v, ref, psc + x_seq[t], p, net.dt)
+        zin = teacher[t].to(dtype) if teacher is not None else z
+        psc = exp_psc_decay(psc, tau_syn, net.dt)
+        if ring is None:
+            zf = zin
+        elif net.delay_mode == "hetero":
+            ring.update(zin)
+            zf = ring.get_flattened(net.max_delay_steps)
+        else:
+            ring.update(zin)
+            zf = ring.get_delay(net.max_delay_steps)
+        psc = psc + mm(zf)
+        for k, val in (("z", z), ("v", v), ("refractory", ref), ("psc", psc), ("v_pre", v_pre)):
+            rec[k].append(val)
+    out = {k: torch.stack(vals) for k, vals in rec.items()}
+    out["final"] = {"v": v, "refractory": ref, "psc": psc}
+    if ring is not None:
+        out["final"]["history"] = ring.h
+        out["final"]["cursor"] = ring.cursor
+    return out
+
+
+def active_lists(spikes: np.ndarray) -> list[np.ndarray]:
+    """Ascending active indices per lane (the compaction's closed form)."""
+    return [np.flatnonzero(row) .astype(np.int32) for row in np.asarray(spikes)]

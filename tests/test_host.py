@@ -1,0 +1,183 @@
+"""CPU tests of the host side: config registry, module state management,
+connection layout, delay expansion, and the C ABI library's exports."""
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.sparse
+import torch
+
+import btorch_amd
+from btorch_amd import _lib
+from btorch_amd.config import sparse as sparse_registry
+from btorch_amd.connectome import expand_conn_for_delays
+from btorch_amd.models import (
+    LIF, ExponentialPSC, HeterSynapsePSC, RecurrentNN, SparseConn, SpikeHistory, environ,
+    functional,
+)
+from btorch_amd._sparse_config import HipSparseConfig, SparseConfigRegistry
+from oracle import rsnn
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "btorch_amd.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(btx_\w+)\(", header, re.M))
+    assert declared, "no declarations parsed"
+    lib = _lib.load()
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert declared == set(_lib.declared_symbols())
+    assert lib.btx_version() >= 100
+
+
+def test_sparse_registry_and_errors():
+    reg = SparseConfigRegistry()
+    with pytest.raises(RuntimeError):
+        reg.backend()
+    assert isinstance(reg.backend("HIP"), HipSparseConfig)
+    assert reg.resolve("triton").ordered is False
+    assert reg.resolve("native").ordered is True
+    assert reg.resolve("hip", {"slots": 4}).slots == 4
+    with pytest.raises(ValueError):
+        reg.resolve("hip", {"hash": True})
+    with pytest.raises(ValueError):
+        reg.resolve("hip", {"slots": 1})
+    with pytest.raises(ValueError):
+        reg.backend("cusparse")
+
+
+def test_sparse_conn_layout_matches_coalesced_matrix():
+    rng = np.random.default_rng(0)
+    dense = (rng.random((70, 50)) < 0.2) * rng.normal(size=(70, 50))
+    conn = SparseConn(scipy.sparse.coo_array(dense.astype(np.float32)))
+    dst, src, w = rsnn.coalesce(dense.astype(np.float32))
+    assert torch.equal(conn.indices, torch.stack([torch.from_numpy(dst), torch.from_numpy(src)]))
+    assert np.array_equal(conn.magnitude.detach().numpy(), w)
+    rp, col, perm = conn._csr_row_ptr.numpy(), conn._csr_col.numpy(), conn._csr_perm.numpy()
+    rebuilt = np.zeros_like(dense)
+    for s in range(70):
+        cols = col[rp[s]:rp[s + 1]]
+        assert np.all(np.diff(cols) > 0)
+        rebuilt[s, cols] = w[perm[rp[s]:rp[s + 1]]]
+    assert np.array_equal(rebuilt.astype(np.float32), dense.astype(np.float32))
+    sd = conn.state_dict()
+    assert set(sd) == {"indices", "magnitude"}
+    conn2 = SparseConn(scipy.sparse.coo_array(dense.astype(np.float32)))
+    conn2.load_state_dict(sd)
+    assert torch.equal(conn2._csr_perm, conn._csr_perm)
+
+
+def test_sparse_conn_cpu_forward_fails_loudly():
+    conn = SparseConn(scipy.sparse.random(10, 10, density=0.3, format="coo", random_state=1))
+    with pytest.raises(RuntimeError, match="HIP device"):
+        conn(torch.ones(2, 10))
+
+
+def test_prepare_finish_refcount():
+    conn = SparseConn(scipy.sparse.random(8, 8, density=0.3, format="coo", random_state=1))
+    with pytest.raises(RuntimeError):
+        conn.finish_sparse()
+    conn.prepare_sparse()
+    conn.prepare_sparse()
+    conn.finish_sparse()
+    assert conn._prepared_sparse_weight is not None
+    conn.finish_sparse()
+    assert conn._prepared_sparse_weight is None
+
+
+def test_expand_conn_matches_oracle():
+    rng = np.random.default_rng(4)
+    m = scipy.sparse.random(30, 30, density=0.2, format="coo", random_state=4)
+    d = rng.integers(0, 9, size=m.nnz)
+    a = expand_conn_for_delays(m, d, 6).toarray()
+    b = rsnn.expand_conn_for_delays(m, d, 6).toarray()
+    assert np.array_equal(a, b)
+    with pytest.raises(ValueError):
+        expand_conn_for_delays(m, -np.ones(m.nnz), 6)
+
+
+def test_memory_module_init_reset_inplace():
+    lif = LIF(16, tau_ref=2.0, v_reset=-0.5)
+    lif.init_state(batch_size=3)
+    assert lif.v.shape == (3, 16) and torch.all(lif.v == -0.5)
+    lif.v.fill_(7.0)
+    ptr = lif.v.data_ptr()
+    lif.reset(inplace=True)
+    assert lif.v.data_ptr() == ptr and torch.all(lif.v == -0.5)
+    with pytest.raises(ValueError):
+        lif.reset(batch_size=4, inplace=True)
+    lif.reset(batch_size=4)
+    assert lif.v.shape == (4, 16)
+
+
+def test_param_compaction_round_trip():
+    tau = torch.full((16,), 20.0)
+    lif = LIF(16, tau=tau, trainable_param={"tau"})
+    sd = lif.state_dict()
+    assert sd["tau"].shape == ()
+    lif2 = LIF(16, tau=torch.linspace(10, 30, 16), trainable_param={"tau"})
+    lif2.load_state_dict(lif.state_dict())
+    assert torch.all(lif2.tau == 20.0)
+    with pytest.raises(ValueError):
+        LIF(16, tau=torch.ones(5))
+
+
+def test_hidden_state_keys_and_set():
+    n, D = 20, 4
+    m = scipy.sparse.random(n, n, density=0.2, format="coo", random_state=0)
+    conn = SparseConn(expand_conn_for_delays(m, np.ones(m.nnz, int), D))
+    syn = HeterSynapsePSC(n, 1, None, conn, base_psc=ExponentialPSC, max_delay_steps=D,
+                          tau_syn=5.0)
+    rnn = RecurrentNN(LIF(n, tau_ref=2.0), syn)
+    functional.init_net_state(rnn, batch_size=2)
+    keys = set(functional.named_hidden_states(rnn))
+    assert keys == {"neuron.v", "neuron.refractory", "synapse.psc", "synapse.history.history",
+                    "synapse.base_psc.psc"}
+    functional.set_hidden_states(rnn, {"neuron.v": torch.ones(2, n)})
+    assert torch.all(rnn.neuron.v == 1)
+    functional.set_hidden_states(rnn, {"neuron": {"v": torch.zeros(2, n)}})
+    assert torch.all(rnn.neuron.v == 0)
+
+
+def test_environ_context_stack():
+    with pytest.raises(KeyError):
+        environ.get("zz_missing")
+    with environ.context(dt=0.5):
+        with environ.context(dt=0.1):
+            assert environ.get("dt") == 0.1
+        assert environ.get("dt") == 0.5
+
+
+def test_spike_history_modes_match_oracle():
+    for circular in (True, False):
+        h = SpikeHistory(5, max_delay_steps=3, use_circular_buffer=circular)
+        h.init_state(batch_size=2)
+        o = rsnn.Ring(2, 5, 3, circular)
+        rng = np.random.default_rng(0)
+        for _ in range(7):
+            z = torch.from_numpy((rng.random((2, 5)) < 0.5).astype(np.float32))
+            h.update(z)
+            o.update(z)
+            assert torch.equal(h.get_flattened(3), o.get_flattened(3))
+            assert torch.equal(h.get_delay(2), o.get_delay(2))
+
+
+def test_engine_matches_supported_cells():
+    from btorch_amd.models._engine import match_cell
+    n = 12
+    m = scipy.sparse.random(n, n, density=0.3, format="coo", random_state=2)
+    rnn = RecurrentNN(LIF(n), ExponentialPSC(n, 5.0, SparseConn(m)))
+    spec = match_cell(rnn)
+    assert spec is not None and spec.bins == [(0, 0)]
+    delays = np.array([1, 3] * (m.nnz // 2) + [1] * (m.nnz % 2))
+    conn = SparseConn(expand_conn_for_delays(m, delays, 5))
+    syn = HeterSynapsePSC(n, 1, None, conn, base_psc=ExponentialPSC, max_delay_steps=5,
+                          tau_syn=5.0)
+    spec = match_cell(RecurrentNN(LIF(n), syn))
+    assert spec.bins == [(1, 1), (3, 3)] and spec.csr_d == 5 and spec.ring_d == 5
+    # an unsupported cell is not matched
+    assert match_cell(RecurrentNN(LIF(n), ExponentialPSC(n, 5.0, torch.nn.Identity()))) is None

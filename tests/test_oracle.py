@@ -1,0 +1,165 @@
+"""Pins the CPU oracle against the reference's own call (torch.sparse.mm),
+closed forms, invariants and the reference tests' known answers."""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse
+import torch
+
+from oracle import rsnn
+
+
+def _rand_conn(n_src, n_dst, p, seed, signed=True):
+    rng = np.random.default_rng(seed)
+    dense = (rng.random((n_src, n_dst)) < p) * rng.lognormal(-1, 1.0, (n_src, n_dst))
+    if signed:
+        dense = dense * rng.choice([-1.0, 1.0], size=dense.shape)
+    return scipy.sparse.coo_array(dense.astype(np.float32))
+
+
+@pytest.mark.parametrize("values", ["spikes", "real"])
+@pytest.mark.parametrize("lanes", [1, 8, 64])
+def test_sparse_mm_equals_sequential_fma(values, lanes):
+    # linear.py:474-483 sums each destination in ascending source order with
+    # one fmaf per term; the C restatement must agree bit for bit.
+    conn = _rand_conn(300, 200, 0.3, 0)
+    rng = np.random.default_rng(lanes)
+    if values == "spikes":
+        x = (rng.random((lanes, 300)) < 0.3).astype(np.float32)
+    else:
+        x = (rng.random((lanes, 300)) * 3 - 1).astype(np.float32)
+    mm = rsnn.NativeSparse(conn)
+    out = mm(torch.from_numpy(x)).numpy()
+    ref = rsnn.seq_spmm_c(mm.dst, mm.src, mm.w, x, 200)
+    assert np.array_equal(out.view(np.int32), ref.view(np.int32))
+
+
+def test_sequential_fma_catches_reordering():
+    # A descending order differs somewhere: the check above is sensitive to order.
+    conn = _rand_conn(400, 50, 0.5, 3)
+    x = np.random.default_rng(1).random((4, 400)).astype(np.float32)
+    mm = rsnn.NativeSparse(conn)
+    out = mm(torch.from_numpy(x)).numpy()
+    rev = rsnn.seq_spmm_c(mm.dst[::-1], mm.src[::-1], mm.w[::-1], x, 50)
+    assert not np.array_equal(out, rev)
+
+
+def test_sparse_mm_matches_dense_float64():
+    # test_linear.py:24-96: sparse product equals x @ W.
+    conn = _rand_conn(120, 90, 0.1, 5)
+    x = np.random.default_rng(2).random((7, 120)).astype(np.float32)
+    out = rsnn.NativeSparse(conn)(torch.from_numpy(x)).numpy()
+    ref = x.astype(np.float64) @ conn.toarray().astype(np.float64)
+    np.testing.assert_allclose(out, ref, rtol=1e-5, atol=1e-6)
+
+
+def test_delay_expansion_known_answers():
+    # tests/connectome/test_delay_expansion.py:34-107
+    conn = scipy.sparse.coo_array(([5.0], ([0], [1])), shape=(3, 3))
+    d = rsnn.expand_conn_for_delays(conn, np.array([2]), 5).toarray()
+    assert d.shape == (15, 3) and d[2, 1] == 5.0 and d[:, 1].sum() == 5.0
+    conn = scipy.sparse.coo_array(([1.0, 2.0, 3.0, 4.0], ([0, 0, 1, 2], [1, 2, 0, 1])),
+                                  shape=(3, 3))
+    d = rsnn.expand_conn_for_delays(conn, np.array([0, 1, 2, 3]), 5).toarray()
+    assert d[0, 1] == 1.0 and d[1, 2] == 2.0 and d[7, 0] == 3.0 and d[13, 1] == 4.0
+    conn = scipy.sparse.coo_array(([1.0, 2.0], ([0, 1], [1, 2])), shape=(3, 3))
+    d = rsnn.expand_conn_for_delays(conn, np.array([10, 100]), 5).toarray()
+    assert d[4, 1] == 1.0 and d[9, 2] == 2.0
+    with pytest.raises(ValueError):
+        rsnn.expand_conn_for_delays(conn, np.array([-1, 0]), 5)
+
+
+def test_ring_and_expanded_matrix_round_trip():
+    # test_delay_expansion.py:342-389: spike of neuron 0 at t=0 reaches
+    # neuron 1 through a delay-2 synapse when read at t=3.
+    conn = scipy.sparse.coo_array(([2.0], ([0], [1])), shape=(2, 2))
+    cd = rsnn.expand_conn_for_delays(conn, np.array([2]), 3)
+    for circular in (True, False):
+        ring = rsnn.Ring(1, 2, 3, circular)
+        ring.update(torch.tensor([[1.0, 0.0]]))
+        ring.update(torch.tensor([[0.0, 0.0]]))
+        ring.update(torch.tensor([[0.0, 0.0]]))
+        zf = ring.get_flattened(3)
+        assert zf.tolist() == [[0.0, 0.0, 1.0, 0.0, 0.0, 0.0]]
+        out = rsnn.NativeSparse(cd)(zf)
+        assert out.tolist() == [[0.0, 2.0]]
+
+
+def test_exp_psc_closed_form_with_latency():
+    # test_synapse.py:21-83: psc_{t+1} = psc_t e^{-dt/tau} + spike_{t-l}
+    dt, tau, lat = 1.0, 2.0, 3
+    z = torch.tensor([[1.0, 1.0, 0.5], [1.0, 0.0, 0.5], [0.0, 1.0, -0.5], [1.0, 1.0, 0.0],
+                      [0.0, 0.0, 2.0]])
+    ring = rsnn.Ring(1, 3, lat + 1, circular=False)
+    psc = torch.zeros(1, 3)
+    a = math.exp(-dt / tau)
+    exp = torch.zeros(3, dtype=torch.float64)
+    for t in range(5):
+        ring.update(z[t][None])
+        psc = rsnn.exp_psc_decay(psc, torch.tensor(tau), dt) + ring.get_delay(lat)
+        exp = exp * a + (z[t - lat].double() if t >= lat else 0.0)
+        np.testing.assert_allclose(psc[0].numpy(), exp.numpy(), atol=1e-6)
+
+
+def test_delayed_single_neuron_exact_values():
+    # test_delayed_psc.py:493-531: delay 3, tau_syn 1, dt 1
+    ring = rsnn.Ring(1, 1, 4, circular=False)
+    psc = torch.zeros(1, 1)
+    seq = [1.0, 0.0, 0.0, 0.0, 2.0]
+    outs = []
+    for s in seq:
+        ring.update(torch.tensor([[s]]))
+        psc = rsnn.exp_psc_decay(psc, torch.tensor(1.0), 1.0) + ring.get_delay(3)
+        outs.append(psc.item())
+    assert outs[:3] == [0.0, 0.0, 0.0]
+    assert abs(outs[3] - 1.0) < 1e-6
+    assert abs(outs[4] - math.exp(-1.0)) < 1e-6
+
+
+def test_lif_euler_closed_form():
+    # Constant input I, no spike: v_n = I*tau*c^{-1}... Euler recursion
+    # v_{n+1} = v_n (1 - dt/tau) + dt I  =>  v_n = I tau (1 - (1 - dt/tau)^n).
+    p = rsnn.LIFParams.make(1, tau=20.0, v_threshold=1e9, dtype=torch.float64)
+    v = torch.zeros(1, 1, dtype=torch.float64)
+    ref = torch.zeros_like(v)
+    I, dt = 0.7, 0.1
+    for k in range(1, 200):
+        _, v, ref, _ = rsnn.lif_step(v, ref, torch.full_like(v, I), p, dt)
+        closed = I * 20.0 * (1 - (1 - dt / 20.0) ** k)
+        assert abs(v.item() - closed) < 1e-12
+
+
+def test_lif_reset_and_refractory_invariants():
+    # soft reset subtracts (v_th - v_reset); hard reset lands on v_reset;
+    # no spike while the refractory counter is positive, counter = tau_ref - k dt.
+    for hard in (False, True):
+        p = rsnn.LIFParams.make(1, tau=20.0, v_threshold=1.0, v_reset=-0.5, tau_ref=2.0,
+                                hard_reset=hard, dtype=torch.float64)
+        v = torch.zeros(1, 1, dtype=torch.float64)
+        ref = torch.zeros_like(v)
+        s, v2, ref2, v_pre = rsnn.lif_step(v, ref, torch.full_like(v, 30.0), p, 0.1)
+        assert s.item() == 1.0
+        expect = -0.5 if hard else v_pre.item() - 1.5
+        assert abs(v2.item() - expect) < 1e-12
+        assert abs(ref2.item() - 1.9) < 1e-12
+        spikes = []
+        for _ in range(25):
+            s, v2, ref2, _ = rsnn.lif_step(v2, ref2, torch.full_like(v, 30.0), p, 0.1)
+            spikes.append(s.item())
+        first = spikes.index(1.0)
+        assert first == 19  # 1.9 ms remaining at dt = 0.1 -> 19 silent steps
+
+
+def test_float64_oracle_tracks_float32():
+    from btorch_amd import workloads
+    net_c = workloads.plain_lif_net(n=256, p=0.05, seed=3)
+    x = torch.from_numpy(workloads.poisson_input(100, 2, 256, 200.0, 0.1, seed=3) * 12.0)
+    net = rsnn.Net(256, net_c.matrix(), rsnn.LIFParams.make(256, tau_ref=2.0),
+                   torch.tensor(5.0), 0.1)
+    a = rsnn.simulate(net, x)
+    b = rsnn.simulate(net, x, dtype=torch.float64)
+    agree = (a["z"].double() == b["z"]).double().mean().item()
+    assert agree > 0.999
+    assert a["z"].sum() > 0
