@@ -66,6 +66,10 @@ class RsnnPlan(ctypes.Structure):
         ("nbins", _c_int),
         ("bin_delay", _c_int * MAX_BINS),
         ("bin_sub", _c_int * MAX_BINS),
+        ("csc_ptr", _vp),
+        ("csc_src", _vp),
+        ("csc_w", _vp),
+        ("hist_words", _vp),
         ("teacher", _vp),
         ("raster", _vp),
         ("rec_z", _vp),
@@ -97,10 +101,10 @@ _SIGNATURES = {
     ),
     "btx_rsnn_workspace_bytes": (ctypes.c_size_t, [_c_i64, _c_i64, _c_int, _c_int]),
     "btx_rsnn_run": (_c_int, [ctypes.POINTER(RsnnPlan), _c_i64, _vp]),
-    "btx_rsnn_check_overflow": (_c_int, [_vp, _vp]),
     "btx_rsnn_run_timed": (_c_int, [ctypes.POINTER(RsnnPlan), _c_i64, _vp,
                                     ctypes.POINTER(ctypes.c_double),
-                                    ctypes.POINTER(ctypes.c_int64)]),
+                                    ctypes.POINTER(ctypes.c_int64),
+                                    ctypes.POINTER(ctypes.c_double), _c_i64]),
     "btx_lif_step": (_c_int, [ctypes.POINTER(RsnnPlan), _vp, _vp, _vp, _vp]),
     "btx_exp_decay": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp]),
 }

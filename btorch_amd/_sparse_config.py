@@ -28,16 +28,17 @@ class HipSparseConfig(SparseBackendConfig):
 
     ordered: sum each destination's terms in ascending source order (bit-exact
         against the CPU reference); False uses float atomics.
-    slots: per-destination term slots of the ordered mode; terms past this
-        count go to an overflow list (correct, slower).
+    slots: per-destination term slots of the ordered mode (2..16); a
+        destination with more terms is reduced by pulling its incoming edges
+        (exact, slower for that destination).
     """
 
     ordered: bool = True
-    slots: int = 8
+    slots: int = 16
 
     def validate(self) -> None:
-        if self.slots < 2 or self.slots > 64:
-            raise ValueError("slots must be in [2, 64].")
+        if self.slots < 2 or self.slots > 16:
+            raise ValueError("slots must be in [2, 16].")
 
 
 class SparseConfigRegistry:

@@ -52,24 +52,30 @@ __device__ __forceinline__ int block_excl_scan(int v, int& total) {
     return r;
 }
 
-// Control words at the head of every workspace.
+// Control words at the head of every workspace (reserved for counters).
 struct Ctl {
-    int ovf_count;   // entries appended to the overflow list this step
-    int ovf_error;   // sticky: overflow list ran out of capacity
-    int pad[62];
+    int pad[64];
 };
 
+// Slots per destination of the ordered mode are held in registers when the
+// terms are sorted, so their count is bounded.
+constexpr int kMaxSlots = 16;
+
 // Workspace layout shared by the standalone product and the step engine.
+// Ordered mode: cnt[m] term counts and slots[K][m] (structure of arrays, so
+// slot j of consecutive destinations is contiguous).  Fast mode: acc[m].
 struct WsLayout {
-    size_t ctl = 0, cnt = 0, slots = 0, ovf = 0, acc = 0, total = 0;
-    int64_t ovf_cap = 0;
+    size_t ctl = 0, cnt = 0, slots = 0, acc = 0, total = 0;
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-inline WsLayout ws_layout(int64_t lanes, int64_t n_dst, int slots, int ordered) {
+// `planes` accumulators of lanes*n_dst destinations each (one per future step
+// of the engine's delivery ring; 1 for the standalone product).
+inline WsLayout ws_layout(int64_t lanes, int64_t n_dst, int slots, int ordered,
+                          int64_t planes = 1) {
     WsLayout L;
-    const int64_t m = lanes * n_dst;
+    const int64_t m = lanes * n_dst * planes;
     size_t off = 0;
     L.ctl = off;
     off = align256(off + sizeof(Ctl));
@@ -78,9 +84,6 @@ inline WsLayout ws_layout(int64_t lanes, int64_t n_dst, int slots, int ordered) 
         off = align256(off + sizeof(int) * m);
         L.slots = off;
         off = align256(off + sizeof(int2) * m * (int64_t)slots);
-        L.ovf_cap = m < 4096 ? 4096 : (m > (1 << 20) ? (1 << 20) : m);
-        L.ovf = off;
-        off = align256(off + sizeof(int4) * L.ovf_cap);
     } else {
         L.acc = off;
         off = align256(off + sizeof(float) * m);
@@ -89,49 +92,37 @@ inline WsLayout ws_layout(int64_t lanes, int64_t n_dst, int slots, int ordered) 
     return L;
 }
 
-// Sum the terms a destination received this step in ascending source order,
-// starting from a fresh +0.0f accumulator with one fmaf per term.  This is
-// the per-destination order of torch.sparse.mm on a coalesced (dst, src) COO
-// on CPU (btorch/models/linear.py:474-483).  Terms live in `slots` (the first
-// min(c, K) of them) and, past K, in the overflow list keyed by `key`.
-// Sources are distinct per destination (coalesced matrix), so a selection by
-// strictly increasing source id visits every term exactly once.
+// Sum the c <= K terms a destination received, in ascending source order,
+// from a fresh +0.0f accumulator with one fmaf per term: the per-destination
+// order of torch.sparse.mm on a coalesced (dst, src) COO on CPU
+// (btorch/models/linear.py:474-483).  Term j of destination `key` is
+// slots[j*m + key] = (source row, weight bits).  Sources are distinct per
+// destination (coalesced matrix), so selecting strictly increasing source
+// ids visits every term once; the terms stay in registers.
 template <bool HASX>
 __device__ __forceinline__ float ordered_sum(int64_t key, int c, const int2* __restrict__ slots,
-                                             int K, const int4* __restrict__ ovf, int novf,
-                                             const float* __restrict__ xl) {
-    const int2* sl = slots + key * K;
+                                             int64_t m, const float* __restrict__ xl) {
     if (!HASX) {
-        if (c == 1) return 0.f + __int_as_float(sl[0].y);
-        if (c == 2) return (0.f + __int_as_float(sl[0].y)) + __int_as_float(sl[1].y);
+        const float w0 = __int_as_float(slots[key].y);
+        if (c == 1) return 0.f + w0;
+        if (c == 2) return (0.f + w0) + __int_as_float(slots[m + key].y);
     }
-    const int nk = c < K ? c : K;
-    const int klo = (int)(key & 0xffffffff);
-    const int khi = (int)(key >> 32);
+    int2 e[kMaxSlots];
+#pragma unroll
+    for (int j = 0; j < kMaxSlots; ++j)
+        e[j] = j < c ? slots[(int64_t)j * m + key] : make_int2(0x7fffffff, 0);
     float acc = 0.f;
     int last = -1;
     for (int k = 0; k < c; ++k) {
         int best = 0x7fffffff;
         float bw = 0.f;
-        for (int j = 0; j < nk; ++j) {
-            const int2 e = sl[j];
-            if (e.x > last && e.x < best) {
-                best = e.x;
-                bw = __int_as_float(e.y);
+#pragma unroll
+        for (int j = 0; j < kMaxSlots; ++j) {
+            if (e[j].x > last && e[j].x < best) {
+                best = e[j].x;
+                bw = __int_as_float(e[j].y);
             }
         }
-        if (c > K) {
-            for (int j = 0; j < novf; ++j) {
-                const int4 o = ovf[j];
-                if (o.x == klo && o.y == khi && o.z > last && o.z < best) {
-                    best = o.z;
-                    bw = __int_as_float(o.w);
-                }
-            }
-        }
-        // every term was found unless an overflow list ran out of room (the
-        // run is flagged invalid then); never index with the sentinel
-        if (best == 0x7fffffff) break;
         const float xv = HASX ? xl[best] : 1.f;
         acc = fmaf(bw, xv, acc);
         last = best;

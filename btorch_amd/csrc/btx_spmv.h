@@ -27,11 +27,9 @@ struct PushArgs {
     const float* x;  // per-source values, NULL = spikes (1.0)
     int64_t x_stride;
     int ordered;
-    int K;
-    int ovf_cap;              // 0: terms past K are only counted (caller pulls them)
+    int K;                    // terms past K are only counted; the reduction pulls them
     int* cnt;
-    int2* slots;
-    int4* ovf;
+    int2* slots;              // [K][m], m = ring_lanes * n_dst
     Ctl* ctl;
     float* acc;
 };
@@ -42,6 +40,6 @@ int push_blocks();
 
 __global__ void k_compact_bits(const uint64_t* __restrict__ words, int64_t n, int64_t nwords,
                                int64_t wstride, int32_t* __restrict__ idx, int64_t istride,
-                               int32_t* __restrict__ counts, int* __restrict__ ctl_reset);
+                               int32_t* __restrict__ counts);
 
 }  // namespace btx

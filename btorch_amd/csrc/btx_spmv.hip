@@ -55,8 +55,7 @@ __global__ __launch_bounds__(1024) void k_compact_bits(const uint64_t* __restric
                                                        int64_t n, int64_t nwords, int64_t wstride,
                                                        int32_t* __restrict__ idx,
                                                        int64_t istride,
-                                                       int32_t* __restrict__ counts,
-                                                       int* __restrict__ ctl_reset) {
+                                                       int32_t* __restrict__ counts) {
     const int lane = blockIdx.x;
     const uint64_t* wl = words + (int64_t)lane * wstride;
     const int64_t per = (nwords + blockDim.x - 1) / blockDim.x;
@@ -77,18 +76,14 @@ __global__ __launch_bounds__(1024) void k_compact_bits(const uint64_t* __restric
             m &= m - 1;
         }
     }
-    if (threadIdx.x == 0) {
-        counts[lane] = total;
-        if (ctl_reset != nullptr && lane == 0) ctl_reset[0] = 0;
-    }
+    if (threadIdx.x == 0) counts[lane] = total;
 }
 
 __global__ __launch_bounds__(1024) void k_compact_dense(const float* __restrict__ x, int64_t n,
                                                         int64_t xstride,
                                                         int32_t* __restrict__ idx,
                                                         int64_t istride,
-                                                        int32_t* __restrict__ counts,
-                                                        int* __restrict__ ctl_reset) {
+                                                        int32_t* __restrict__ counts) {
     const int lane = blockIdx.x;
     const float* xl = x + (int64_t)lane * xstride;
     const int64_t per = (n + blockDim.x - 1) / blockDim.x;
@@ -101,10 +96,7 @@ __global__ __launch_bounds__(1024) void k_compact_dense(const float* __restrict_
     int32_t* out = idx + (int64_t)lane * istride + excl;
     for (int64_t e = e0; e < e1; ++e)
         if (xl[e] != 0.f) *out++ = (int32_t)e;
-    if (threadIdx.x == 0) {
-        counts[lane] = total;
-        if (ctl_reset != nullptr && lane == 0) ctl_reset[0] = 0;
-    }
+    if (threadIdx.x == 0) counts[lane] = total;
 }
 
 // ---------------------------------------------------------------------------
@@ -172,16 +164,9 @@ __global__ __launch_bounds__(kPushThreads) void k_push(PushArgs a) {
             const int64_t key = base + dst;
             if (ORDERED) {
                 const int pos = atomicAdd(&a.cnt[key], 1);
-                if (pos < a.K) {
-                    a.slots[key * a.K + pos] = make_int2((int)r, __float_as_int(wv));
-                } else if (a.ovf_cap > 0) {
-                    const int o = atomicAdd(&a.ctl->ovf_count, 1);
-                    if (o < a.ovf_cap)
-                        a.ovf[o] = make_int4((int)(key & 0xffffffff), (int)(key >> 32), (int)r,
-                                             __float_as_int(wv));
-                    else
-                        atomicOr(&a.ctl->ovf_error, 1);
-                }
+                if (pos < a.K)
+                    a.slots[(int64_t)pos * a.ring_lanes * a.n_dst + key] =
+                        make_int2((int)r, __float_as_int(wv));
             } else {
                 atomicAdd(&a.acc[key], HASX ? wv * xv : wv);
             }
@@ -268,7 +253,7 @@ __global__ __launch_bounds__(256) void k_reduce_ordered(int64_t m, int64_t n_dst
         const int64_t b = i / n_dst;
         const float* xl = x + b * x_stride;
         if (c <= K) {
-            acc = ordered_sum<true>(i, c, slots, K, nullptr, 0, xl);
+            acc = ordered_sum<true>(i, c, slots, m, xl);
         } else {
             const int64_t d = i - b * n_dst;
             for (int64_t e = csc_ptr[d]; e < csc_ptr[d + 1]; ++e)
@@ -277,10 +262,6 @@ __global__ __launch_bounds__(256) void k_reduce_ordered(int64_t m, int64_t n_dst
         cnt[i] = 0;
     }
     out[i] = acc;
-}
-
-__global__ void k_reset_ctl(Ctl* ctl) {
-    if (threadIdx.x == 0) ctl->ovf_count = 0;
 }
 
 __global__ __launch_bounds__(256) void k_zero(float* __restrict__ p, int64_t m) {
@@ -362,8 +343,7 @@ BTX_API int btx_compact_dense(const float* x, int64_t lanes, int64_t n, int64_t 
     if (lanes <= 0) return 0;
     if (n >= (int64_t)1 << 31) return fail("btx_compact_dense: n exceeds int32 indices");
     hipLaunchKernelGGL(k_compact_dense, dim3((unsigned)lanes), dim3(1024), 0,
-                       (hipStream_t)stream, x, n, x_lane_stride, idx, idx_lane_stride, counts,
-                       nullptr);
+                       (hipStream_t)stream, x, n, x_lane_stride, idx, idx_lane_stride, counts);
     return check_launch("btx_compact_dense");
 }
 
@@ -375,7 +355,7 @@ BTX_API int btx_compact_bits(const uint64_t* words, int64_t lanes, int64_t n,
     const int64_t nwords = (n + 63) / 64;
     hipLaunchKernelGGL(k_compact_bits, dim3((unsigned)lanes), dim3(1024), 0,
                        (hipStream_t)stream, words, n, nwords, words_lane_stride, idx,
-                       idx_lane_stride, counts, nullptr);
+                       idx_lane_stride, counts);
     return check_launch("btx_compact_bits");
 }
 
@@ -391,7 +371,8 @@ BTX_API int btx_spmv_event(const int64_t* row_ptr, const int32_t* col, const flo
                            int slots, void* workspace, size_t workspace_bytes, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (lanes <= 0 || n_dst <= 0) return 0;
-    if (ordered && slots < 2) return fail("btx_spmv_event: ordered mode needs slots >= 2");
+    if (ordered && (slots < 2 || slots > kMaxSlots))
+        return fail("btx_spmv_event: ordered mode needs 2 <= slots <= 16");
     if (ordered && (x == nullptr || csc_ptr == nullptr || csc_src == nullptr || csc_w == nullptr))
         return fail("btx_spmv_event: ordered mode needs x and the dst-major arrays");
     if (n_src >= (int64_t)1 << 31) return fail("btx_spmv_event: n_src exceeds int32");
@@ -419,16 +400,13 @@ BTX_API int btx_spmv_event(const int64_t* row_ptr, const int32_t* col, const flo
     a.ordered = ordered;
     a.ctl = (Ctl*)(ws + L.ctl);
     a.K = slots;
-    a.ovf_cap = 0;  // destinations past K terms are pulled by the reduction
     const unsigned rblocks = (unsigned)((m + 255) / 256);
     if (ordered) {
         a.cnt = (int*)(ws + L.cnt);
         a.slots = (int2*)(ws + L.slots);
-        a.ovf = (int4*)(ws + L.ovf);
     } else {
         a.acc = (float*)(ws + L.acc);
     }
-    hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, s, a.ctl);
     const double avg = n_src > 0 ? (double)nnz / (double)n_src : 64.0;
     int rc = launch_push(a, pick_group(avg), push_blocks(), s);
     if (rc) return rc;

@@ -58,34 +58,125 @@ struct StepArgs {
     float* rec_psc_prev;   // step t-1 or NULL
     float* rec_v_pre;
     float* rec_v_prev;
+    int lanes;
     int* cnt;
-    const int2* slots;
-    const int4* ovf;
-    int ovf_cap;
-    const Ctl* ctl;
+    const int2* slots;     // [K][lanes*n]
     float* acc;
+    // pull of a destination that received more than K terms
+    int64_t push_step;     // step whose push F consumes (t - 1)
+    const int64_t* csc_ptr;
+    const int32_t* csc_src;
+    const float* csc_w;
+    const uint64_t* words;       // pushed spike words of this run [T][lanes][nwords]
+    const uint64_t* hist_words;  // words of the ring_d steps before the run [ring_d][lanes][nwords]
+    int ring_d;
+    int cursor;
+    int csr_d;
+    int age_of_sub[BTX_MAX_BINS];  // ring age of virtual-row offset, -1 = not read
 };
 
 __device__ __forceinline__ float prm(const float* p, int64_t stride, int64_t n) {
     return p[stride * n];
 }
 
-// Synaptic input delivered by the previous step's push, leaving the
-// accumulator zeroed for the next push.
-__device__ __forceinline__ float take_input(const StepArgs& a, int64_t i) {
-    if (a.ordered) {
-        const int c = a.cnt[i];
-        if (c == 0) return 0.f;
-        const int novf = a.ctl->ovf_count < a.ovf_cap ? a.ctl->ovf_count : a.ovf_cap;
-        const float r = ordered_sum<false>(i, c, a.slots, a.K, a.ovf, novf, nullptr);
-        a.cnt[i] = 0;
-        return r;
+// Exact input of destination `dst` of `lane` from the push of step
+// a.push_step, pulled over its incoming edges in the coalesced (dst, src)
+// order: a virtual row src*csr_d + sub is active when neuron src spiked
+// age_of_sub[sub] steps before that push.
+__device__ __forceinline__ bool pulled_active(const StepArgs& a, int64_t lane, int r) {
+    const int src = r / a.csr_d;
+    const int age = a.age_of_sub[r - src * a.csr_d];
+    if (age < 0) return false;
+    const int64_t st = a.push_step - age;
+    const uint64_t* row;
+    if (st >= 0) {
+        row = a.words + (st * a.lanes + lane) * a.nwords;
+    } else {
+        int64_t slot = (a.cursor + st) % a.ring_d;
+        if (slot < 0) slot += a.ring_d;
+        row = a.hist_words + (slot * a.lanes + lane) * a.nwords;
     }
+    return (row[src >> 6] >> (src & 63)) & 1ull;
+}
+
+__device__ float pull_input(const StepArgs& a, int64_t lane, int64_t dst) {
+    float acc = 0.f;
+    int64_t e = a.csc_ptr[dst];
+    const int64_t e1 = a.csc_ptr[dst + 1];
+    // four independent activity tests in flight, summed in order
+    for (; e + 4 <= e1; e += 4) {
+        const int r0 = a.csc_src[e], r1 = a.csc_src[e + 1];
+        const int r2 = a.csc_src[e + 2], r3 = a.csc_src[e + 3];
+        const bool b0 = pulled_active(a, lane, r0), b1 = pulled_active(a, lane, r1);
+        const bool b2 = pulled_active(a, lane, r2), b3 = pulled_active(a, lane, r3);
+        if (b0) acc = acc + a.csc_w[e];
+        if (b1) acc = acc + a.csc_w[e + 1];
+        if (b2) acc = acc + a.csc_w[e + 2];
+        if (b3) acc = acc + a.csc_w[e + 3];
+    }
+    for (; e < e1; ++e)
+        if (pulled_active(a, lane, a.csc_src[e])) acc = acc + a.csc_w[e];
+    return acc;
+}
+
+// Synaptic input delivered by the previous step's push (c terms in the
+// ordered mode), leaving the accumulator zeroed for the next push.
+__device__ __forceinline__ float take_input_c(const StepArgs& a, int64_t i, int64_t lane,
+                                              int64_t nidx, int c) {
+    if (c == 0) return 0.f;
+    const float r = c <= a.K ? ordered_sum<false>(i, c, a.slots, a.lanes * a.n, nullptr)
+                             : pull_input(a, lane, nidx);
+    a.cnt[i] = 0;
+    return r;
+}
+
+__device__ __forceinline__ float take_input(const StepArgs& a, int64_t i, int64_t lane,
+                                            int64_t nidx) {
+    if (a.ordered) return take_input_c(a, i, lane, nidx, a.cnt[i]);
     const float r = a.acc[i];
     if (r != 0.f) a.acc[i] = 0.f;
     return r;
 }
 
+// One neuron of the fused step.  `psc` enters as the decayed current of the
+// previous step (without its push), leaves as this step's decayed current.
+// Returns the spike (0/1).
+__device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
+                                                int64_t i, float& psc, float xin, float& v,
+                                                float& ref, float wz) {
+    if (!a.first) {
+        psc = psc + wz;
+        if (a.rec_psc_prev) a.rec_psc_prev[i] = psc;
+    }
+    const float I = a.psc ? psc + xin : xin;
+    const float tau = prm(a.tau, a.tau_s, nidx);
+    const float vth = prm(a.vth, a.vth_s, nidx);
+    const float vr = prm(a.vr, a.vr_s, nidx);
+    const float cm = prm(a.cm, a.cm_s, nidx);
+    if (a.rec_v_prev) a.rec_v_prev[i] = v;
+    const float dv = (-(v - vr)) / tau + I / cm;
+    const float v1 = v + a.dt * dv;
+    if (a.rec_v_pre) a.rec_v_pre[i] = v1;
+    const float u = (v1 - vth) / (vth - vr);
+    float s = (u >= 0.f) ? 1.f : 0.f;
+    if (a.use_ref) {
+        s = s * ((ref == 0.f) ? 1.f : 0.f);
+        float r2 = (ref + s * prm(a.tref, a.tref_s, nidx)) - a.dt;
+        ref = (r2 < 0.f) ? 0.f : r2;
+        if (a.rec_ref) a.rec_ref[i] = ref;
+    }
+    v = a.hard ? (v1 - (v1 - vr) * s) : (v1 - (vth - vr) * s);
+    if (a.rec_v) a.rec_v[i] = v;
+    if (a.rec_z) a.rec_z[i] = s;
+    if (a.psc) {
+        const float tsyn = prm(a.tsyn, a.tsyn_s, nidx);
+        const float k = prm(a.ksyn, a.ksyn_s, nidx);
+        psc = psc + k * ((-psc) / tsyn);
+    }
+    return s;
+}
+
+// Scalar form: one neuron per thread, spike word by wavefront ballot.
 __global__ __launch_bounds__(256) void k_lif_exp_step(StepArgs a) {
     const int64_t lane = blockIdx.y;
     const int64_t nidx = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -93,40 +184,14 @@ __global__ __launch_bounds__(256) void k_lif_exp_step(StepArgs a) {
     if (nidx < a.n) {
         const int64_t i = lane * a.n + nidx;
         float psc = a.psc ? a.psc[i] : 0.f;
-        if (!a.first) {
-            psc = psc + take_input(a, i);
-            if (a.rec_psc_prev) a.rec_psc_prev[i] = psc;
-        }
+        const float wz = a.first ? 0.f : take_input(a, i, lane, nidx);
         const float xin = a.x ? a.x[i] : 0.f;
-        const float I = a.psc ? psc + xin : xin;
-        const float tau = prm(a.tau, a.tau_s, nidx);
-        const float vth = prm(a.vth, a.vth_s, nidx);
-        const float vr = prm(a.vr, a.vr_s, nidx);
-        const float cm = prm(a.cm, a.cm_s, nidx);
-        const float v = a.v[i];
-        if (a.rec_v_prev) a.rec_v_prev[i] = v;
-        const float dv = (-(v - vr)) / tau + I / cm;
-        const float v1 = v + a.dt * dv;
-        if (a.rec_v_pre) a.rec_v_pre[i] = v1;
-        const float u = (v1 - vth) / (vth - vr);
-        float s = (u >= 0.f) ? 1.f : 0.f;
-        if (a.use_ref) {
-            const float ref = a.ref[i];
-            s = s * ((ref == 0.f) ? 1.f : 0.f);
-            float r2 = (ref + s * prm(a.tref, a.tref_s, nidx)) - a.dt;
-            r2 = (r2 < 0.f) ? 0.f : r2;
-            a.ref[i] = r2;
-            if (a.rec_ref) a.rec_ref[i] = r2;
-        }
-        const float v2 = a.hard ? (v1 - (v1 - vr) * s) : (v1 - (vth - vr) * s);
-        a.v[i] = v2;
-        if (a.rec_v) a.rec_v[i] = v2;
-        if (a.rec_z) a.rec_z[i] = s;
-        if (a.psc) {
-            const float tsyn = prm(a.tsyn, a.tsyn_s, nidx);
-            const float k = prm(a.ksyn, a.ksyn_s, nidx);
-            a.psc[i] = psc + k * ((-psc) / tsyn);
-        }
+        float v = a.v[i];
+        float ref = a.use_ref ? a.ref[i] : 0.f;
+        const float s = lif_exp_neuron(a, lane, nidx, i, psc, xin, v, ref, wz);
+        a.v[i] = v;
+        if (a.use_ref) a.ref[i] = ref;
+        if (a.psc) a.psc[i] = psc;
         spk = s != 0.f;
     }
     const uint64_t m = __ballot(spk);
@@ -136,11 +201,58 @@ __global__ __launch_bounds__(256) void k_lif_exp_step(StepArgs a) {
     }
 }
 
+// Vector form (n % 4 == 0, 16-byte aligned state): four consecutive neurons
+// per thread with 16-byte loads/stores; the 64-bit spike word of 16 threads
+// is assembled by an OR across them.
+__global__ __launch_bounds__(256) void k_lif_exp_step4(StepArgs a) {
+    const int64_t lane = blockIdx.y;
+    const int64_t n0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    uint64_t nib = 0;
+    if (n0 < a.n) {
+        const int64_t i0 = lane * a.n + n0;
+        float4 v4 = *(const float4*)(a.v + i0);
+        float4 p4 = a.psc ? *(const float4*)(a.psc + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 r4 = a.use_ref ? *(const float4*)(a.ref + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 x4 = a.x ? *(const float4*)(a.x + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+        float w[4] = {0.f, 0.f, 0.f, 0.f};
+        if (!a.first) {
+            if (a.ordered) {
+                const int4 c4 = *(const int4*)(a.cnt + i0);
+                w[0] = take_input_c(a, i0, lane, n0, c4.x);
+                w[1] = take_input_c(a, i0 + 1, lane, n0 + 1, c4.y);
+                w[2] = take_input_c(a, i0 + 2, lane, n0 + 2, c4.z);
+                w[3] = take_input_c(a, i0 + 3, lane, n0 + 3, c4.w);
+            } else {
+                const float4 q = *(const float4*)(a.acc + i0);
+                w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+                if (q.x != 0.f || q.y != 0.f || q.z != 0.f || q.w != 0.f)
+                    *(float4*)(a.acc + i0) = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        float s0 = lif_exp_neuron(a, lane, n0, i0, p4.x, x4.x, v4.x, r4.x, w[0]);
+        float s1 = lif_exp_neuron(a, lane, n0 + 1, i0 + 1, p4.y, x4.y, v4.y, r4.y, w[1]);
+        float s2 = lif_exp_neuron(a, lane, n0 + 2, i0 + 2, p4.z, x4.z, v4.z, r4.z, w[2]);
+        float s3 = lif_exp_neuron(a, lane, n0 + 3, i0 + 3, p4.w, x4.w, v4.w, r4.w, w[3]);
+        *(float4*)(a.v + i0) = v4;
+        if (a.use_ref) *(float4*)(a.ref + i0) = r4;
+        if (a.psc) *(float4*)(a.psc + i0) = p4;
+        nib = (uint64_t)(s0 != 0.f) | ((uint64_t)(s1 != 0.f) << 1) |
+              ((uint64_t)(s2 != 0.f) << 2) | ((uint64_t)(s3 != 0.f) << 3);
+    }
+    uint64_t word = nib << (4 * (threadIdx.x & 15));
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) word |= __shfl_xor(word, o, 64);
+    if (a.raster != nullptr && (threadIdx.x & 15) == 0) {
+        const int64_t wi = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+        if (wi < a.nwords) a.raster[lane * a.nwords + wi] = word;
+    }
+}
+
 // psc_{T-1} = psc_dec + input of the last push.
 __global__ __launch_bounds__(256) void k_finish_psc(StepArgs a, int64_t m) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
-    const float psc = a.psc[i] + take_input(a, i);
+    const float psc = a.psc[i] + take_input(a, i, i / a.n, i % a.n);
     a.psc[i] = psc;
     if (a.rec_psc_prev) a.rec_psc_prev[i] = psc;
 }
@@ -245,14 +357,17 @@ struct KernelTimer {
         hipEventRecord(b, s);
         ev[k].back().second = b;
     }
-    void collect(double* ms, int64_t* calls) {
+    void collect(double* ms, int64_t* calls, double* per, int64_t cap) {
         hipStreamSynchronize(s);
         for (int k = 0; k < 4; ++k) {
             double tot = 0;
+            int64_t idx = 0;
             for (auto& pr : ev[k]) {
                 float m = 0;
                 hipEventElapsedTime(&m, pr.first, pr.second);
                 tot += m;
+                if (per != nullptr && idx < cap) per[k * cap + idx] = m;
+                ++idx;
                 hipEventDestroy(pr.first);
                 hipEventDestroy(pr.second);
             }
@@ -273,12 +388,13 @@ BTX_API int btx_rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, void* stream) 
 }
 
 BTX_API int btx_rsnn_run_timed(const btx_rsnn_plan* p, int64_t n_steps, void* stream,
-                               double* ms_out, int64_t* calls_out) {
+                               double* ms_out, int64_t* calls_out, double* per_launch_ms,
+                               int64_t per_launch_cap) {
     KernelTimer kt;
     kt.on = true;
     kt.s = (hipStream_t)stream;
     const int rc = rsnn_run(p, n_steps, (hipStream_t)stream, kt);
-    kt.collect(ms_out, calls_out);
+    kt.collect(ms_out, calls_out, per_launch_ms, per_launch_cap);
     return rc;
 }
 
@@ -299,7 +415,12 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         if (p->bin_sub[j] < 0 || p->bin_sub[j] >= p->csr_d)
             return fail("btx_rsnn_run: bin offset outside csr_d");
     }
-    if (p->ordered && p->slots < 2) return fail("btx_rsnn_run: ordered mode needs slots >= 2");
+    if (p->ordered && (p->slots < 2 || p->slots > kMaxSlots))
+        return fail("btx_rsnn_run: ordered mode needs 2 <= slots <= 16");
+    if (p->ordered && (p->csc_ptr == nullptr || p->csc_src == nullptr || p->csc_w == nullptr ||
+                       p->hist_words == nullptr))
+        return fail("btx_rsnn_run: ordered mode needs the dst-major arrays and history words");
+    if (p->csr_d > BTX_MAX_BINS) return fail("btx_rsnn_run: csr_d exceeds 64");
     const WsLayout L = ws_layout(p->lanes, p->n, p->slots, p->ordered);
     if (p->workspace_bytes < L.total) return fail("btx_rsnn_run: workspace too small");
     char* ws = (char*)p->workspace;
@@ -308,16 +429,26 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     const int64_t nwords = (n + 63) / 64;
     StepArgs a = neuron_args(p);
     a.psc = p->psc;
-    a.ctl = (const Ctl*)(ws + L.ctl);
-    a.ovf_cap = (int)L.ovf_cap;
+    a.lanes = (int)lanes;
     if (p->ordered) {
         a.cnt = (int*)(ws + L.cnt);
         a.slots = (const int2*)(ws + L.slots);
-        a.ovf = (const int4*)(ws + L.ovf);
     } else {
         a.acc = (float*)(ws + L.acc);
     }
+    a.csc_ptr = p->csc_ptr;
+    a.csc_src = p->csc_src;
+    a.csc_w = p->csc_w;
+    a.words = p->teacher ? p->teacher : p->raster;
+    a.hist_words = p->hist_words;
+    a.ring_d = p->ring_d;
+    a.cursor = p->ring_cursor;
+    a.csr_d = p->csr_d;
+    for (int k = 0; k < BTX_MAX_BINS; ++k) a.age_of_sub[k] = -1;
+    for (int j = 0; j < p->nbins; ++j) a.age_of_sub[p->bin_sub[j]] = p->bin_delay[j];
 
+    // Per step the push reads, for every delay bin, the compacted spikes of
+    // the ring slot that bin's age points at, and streams the bin's rows.
     PushArgs pa{};
     pa.row_ptr = p->row_ptr;
     pa.col = p->col;
@@ -334,20 +465,26 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     pa.x = nullptr;
     pa.ordered = p->ordered;
     pa.K = p->slots;
-    pa.ovf_cap = (int)L.ovf_cap;
     pa.ctl = (Ctl*)(ws + L.ctl);
     pa.cnt = a.cnt;
     pa.slots = (int2*)a.slots;
-    pa.ovf = (int4*)a.ovf;
     pa.acc = a.acc;
 
-    // Average row length over the bins read, for the push group width.
+    // push group width from the mean row length of the bins read
     const int group = pick_group((double)p->nnz / (double)(n * p->csr_d));
     const int blocks = push_blocks();
-    const dim3 sgrid((unsigned)((n + 255) / 256), (unsigned)lanes);
+    // the vector step needs n % 4 == 0 and 16-byte aligned state arrays
+    auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
+    const bool vec4 = (n % 4 == 0) && al16(p->v) && al16(p->psc) &&
+                      (!p->use_refractory || al16(p->refractory)) && (!p->x || al16(p->x)) &&
+                      al16(ws) && (!p->rec_psc || al16(p->rec_psc));
+    const dim3 sgrid(vec4 ? (unsigned)((n + 1023) / 1024) : (unsigned)((n + 255) / 256),
+                     (unsigned)lanes);
+    hipEvent_t e0;
     const int64_t rec_step = m;
     for (int64_t t = 0; t < n_steps; ++t) {
         a.first = (t == 0);
+        a.push_step = t - 1;
         a.x = p->x ? p->x + t * p->x_step_stride : nullptr;
         a.raster = p->raster + t * lanes * nwords;
         a.rec_z = p->rec_z ? p->rec_z + t * rec_step : nullptr;
@@ -356,16 +493,16 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         a.rec_v_pre = p->rec_v_pre ? p->rec_v_pre + t * rec_step : nullptr;
         a.rec_v_prev = p->rec_v_prev ? p->rec_v_prev + t * rec_step : nullptr;
         a.rec_psc_prev = (p->rec_psc && t > 0) ? p->rec_psc + (t - 1) * rec_step : nullptr;
-        hipEvent_t e0;
         kt.begin(0, e0);
-        hipLaunchKernelGGL(k_lif_exp_step, sgrid, dim3(256), 0, s, a);
+        if (vec4) hipLaunchKernelGGL(k_lif_exp_step4, sgrid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_lif_exp_step, sgrid, dim3(256), 0, s, a);
         kt.end(0);
         const int64_t slot = mod(p->ring_cursor + t, p->ring_d);
         const uint64_t* pushed = p->teacher ? p->teacher + t * lanes * nwords : a.raster;
         kt.begin(1, e0);
         hipLaunchKernelGGL(k_compact_bits, dim3((unsigned)lanes), dim3(1024), 0, s, pushed,
                            n, nwords, nwords, p->ring_idx + slot * lanes * n, n,
-                           p->ring_cnt + slot * lanes, (int*)(ws + L.ctl));
+                           p->ring_cnt + slot * lanes);
         kt.end(1);
         for (int j = 0; j < p->nbins; ++j) {
             pa.bin_slot[j] = (int)mod(p->ring_cursor + t - p->bin_delay[j], p->ring_d);
@@ -377,20 +514,11 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         if (rc) return rc;
     }
     a.first = 0;
+    a.push_step = n_steps - 1;
     a.rec_psc_prev = p->rec_psc ? p->rec_psc + (n_steps - 1) * rec_step : nullptr;
-    hipEvent_t e1;
-    kt.begin(3, e1);
+    kt.begin(3, e0);
     hipLaunchKernelGGL(k_finish_psc, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, a, m);
     kt.end(3);
     return check_launch("btx_rsnn_run");
 }
 
-BTX_API int btx_rsnn_check_overflow(void* workspace, void* stream) {
-    Ctl c{};
-    if (hipMemcpyAsync(&c, workspace, sizeof(Ctl), hipMemcpyDeviceToHost, (hipStream_t)stream) !=
-        hipSuccess)
-        return fail("btx_rsnn_check_overflow: copy failed");
-    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
-        return fail("btx_rsnn_check_overflow: sync failed");
-    return c.ovf_error != 0;
-}

@@ -133,8 +133,7 @@ class Engine:
         return ws
 
     def run(self, x: torch.Tensor | None, T: int, lanes: int, record: set[str],
-            check_overflow: bool = True, teacher: torch.Tensor | None = None,
-            timings: dict | None = None):
+            teacher: torch.Tensor | None = None, timings: dict | None = None):
         """x: (T, lanes, n) fp32 contiguous or None.  Returns (raster, records)."""
         sp = self.spec
         neuron, syn, conn = sp.neuron, sp.exp_psc, sp.conn
@@ -179,12 +178,19 @@ class Engine:
         p.w = w.data_ptr()
         p.nnz = conn.nnz
         p.csr_d = sp.csr_d
+        w_dst = conn._get_effective_weight().detach().to(torch.float32).contiguous()
+        keep.append(w_dst)
+        p.csc_ptr = conn._csc_ptr.data_ptr()
+        p.csc_src = conn._csc_src.data_ptr()
+        p.csc_w = w_dst.data_ptr()
         # ring of compacted spike lists
         ring_d = sp.ring_d
         ring_idx = torch.empty((ring_d, lanes, n), dtype=torch.int32, device=dev)
         ring_cnt = torch.zeros((ring_d, lanes), dtype=torch.int32, device=dev)
         cursor = 0
         hist = sp.history
+        nwords = (n + 63) // 64
+        hist_words = torch.zeros((ring_d, lanes, nwords), dtype=torch.int64, device=dev)
         if hist is not None:
             h = hist.history
             if h is None or h.numel() != lanes * ring_d * n:
@@ -193,12 +199,17 @@ class Engine:
             keep.append(h)
             cursor = hist.cursor
             lib = _lib.load()
+            order = (list(range(ring_d)) if hist.use_circular_buffer
+                     else [ring_d - 1 - j for j in range(ring_d)])
+            hist_words.copy_(pack_bits(h.transpose(0, 1)[order]))
             for j in range(ring_d):
-                src = j if hist.use_circular_buffer else ring_d - 1 - j
+                src = order[j]
                 _lib.check(lib.btx_compact_dense(h.data_ptr() + 4 * src * n, lanes, n, ring_d * n,
                                                  ring_idx[j].data_ptr(), n,
                                                  ring_cnt[j].data_ptr(), _lib.stream_ptr()),
                            "btx_compact_dense")
+        keep.append(hist_words)
+        p.hist_words = hist_words.data_ptr()
         p.ring_d = ring_d
         p.ring_idx = ring_idx.data_ptr()
         p.ring_cnt = ring_cnt.data_ptr()
@@ -208,7 +219,6 @@ class Engine:
             p.bin_delay[j] = age
             p.bin_sub[j] = sub
         # outputs
-        nwords = (n + 63) // 64
         raster = torch.empty((T, lanes, nwords), dtype=torch.int64, device=dev)
         p.raster = raster.data_ptr()
         if teacher is not None:
@@ -236,14 +246,13 @@ class Engine:
             import ctypes
             ms = (ctypes.c_double * 4)()
             calls = (ctypes.c_int64 * 4)()
-            _lib.check(_lib.load().btx_rsnn_run_timed(p, T, stream, ms, calls),
+            cap = T + 1
+            per = (ctypes.c_double * (4 * cap))()
+            _lib.check(_lib.load().btx_rsnn_run_timed(p, T, stream, ms, calls, per, cap),
                        "btx_rsnn_run_timed")
             for k, name in enumerate(("step", "compact", "push", "finish")):
                 timings[name] = (float(ms[k]), int(calls[k]))
-        if check_overflow and p.ordered:
-            if _lib.load().btx_rsnn_check_overflow(ws.data_ptr(), stream):
-                raise RuntimeError("ordered accumulation overflow list filled up; raise "
-                                   "sparse_config slots")
+                timings[name + "_each"] = [per[k * cap + i] for i in range(min(calls[k], cap))]
         self._writeback_history(raster, T, lanes, n, cursor)
         return raster, recs
 

@@ -45,7 +45,6 @@ class RecurrentNNAbstract(base.MemoryModule):
         self.grad_state_names = grad_state_names
         self.cudagraph = cudagraph
         self.cudagraph_warmup = cudagraph_warmup
-        self.check_overflow = True
 
     def single_step_forward(self, *args, **kwargs):
         raise NotImplementedError
@@ -200,7 +199,7 @@ class RecurrentNN(RecurrentNNAbstract):
         hist0 = sp.history.history.clone() if (sp.history is not None and
                                                "history" in want) else None
         cursor0 = sp.history.cursor if sp.history is not None else 0
-        raster, recs = eng.run(xs, T, lanes, want, check_overflow=self.check_overflow)
+        raster, recs = eng.run(xs, T, lanes, want)
         return self._assemble(eng, names, recs, raster, hist0, cursor0, T, lanes, n)
 
     @staticmethod

@@ -100,7 +100,8 @@ def powerlaw_net(n=140_000, exponent=2.1, k_min=20, k_max=5000, exc_frac=0.7,
 
 
 def powerlaw_net_fast(n=140_000, exponent=2.1, k_min=20, k_max=5000, exc_frac=0.7,
-                      delay_lo=1, delay_hi=20, seed=1) -> Connectome:
+                      delay_lo=1, delay_hi=20, seed=1, w_mu=-1.0, w_sigma=0.5,
+                      inh_scale=4.0) -> Connectome:
     """Same distribution as ``powerlaw_net`` with vectorised target sampling
     (targets drawn with replacement then de-duplicated per row; the realised
     degree can fall short of the draw by the collision count, <1% here)."""
@@ -113,7 +114,7 @@ def powerlaw_net_fast(n=140_000, exponent=2.1, k_min=20, k_max=5000, exc_frac=0.
     key = pre * n + post
     key = np.unique(key)
     pre, post = key // n, key % n
-    w = _signed_lognormal(rng, pre, n_exc)
+    w = _signed_lognormal(rng, pre, n_exc, inh_scale, w_mu, w_sigma)
     delay = rng.integers(delay_lo, delay_hi + 1, size=pre.size, dtype=np.int64)
     return Connectome(n, pre, post, w, delay, n_exc)
 

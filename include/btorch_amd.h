@@ -124,6 +124,11 @@ int btx_spmv_weight_grad(const int64_t* row_ptr, const int32_t* col, int64_t n_s
  *   z_t   = neuron(I_t)                         (charge -> fire -> reset)
  *   psc_t = psc_{t-1} + k*((-psc_{t-1})/tau_s)  (exp-Euler decay)
  *           + W_exp^T * flatten(history after pushing z_t)
+ * Each step, the push reads the compacted spike list of the ring slot each
+ * delay bin points at and streams that bin's virtual rows; all deliveries of
+ * a step land in one accumulator that the next fused step consumes.
+ * Ordered mode keeps each term's virtual-row id so every destination sums in
+ * ascending virtual-row order.
  * Per-neuron parameters are fp32 arrays with a stride (0 = one shared value).
  * Every pointer refers to device memory that outlives the enqueued work. */
 typedef struct btx_rsnn_plan {
@@ -164,6 +169,15 @@ typedef struct btx_rsnn_plan {
     int nbins;            /* delay bins read each step */
     int bin_delay[BTX_MAX_BINS];  /* ring age of bin j (0 = this step) */
     int bin_sub[BTX_MAX_BINS];    /* virtual-row offset of bin j */
+    /* destination-major copy of the connectivity (the reference's coalesced
+     * (dst, src) order: indices/magnitude) and the spike words of the ring_d
+     * steps before the run ([ring_d][lanes][ceil(n/64)], ring-slot order);
+     * with them the ordered mode reduces a destination that received more
+     * than `slots` terms exactly, by pulling its incoming edges */
+    const int64_t* csc_ptr;   /* (n+1) */
+    const int32_t* csc_src;   /* (nnz) virtual source rows, ascending per destination */
+    const float* csc_w;       /* (nnz) */
+    const uint64_t* hist_words;
     /* teacher forcing: spike words [T][lanes][ceil(n/64)] pushed into the ring
      * instead of the produced spikes (NULL = free running) */
     const uint64_t* teacher;
@@ -204,13 +218,12 @@ int btx_exp_decay(float* psc, int64_t lanes, int64_t n, const float* tau_syn,
  * step sequence, recorded on `stream`.  On return (after a stream
  * synchronisation) ms_out[k] holds the summed duration of kernel kind k and
  * calls_out[k] its launch count, k = 0 fused step, 1 compaction, 2 push,
- * 3 final psc.  For measurement only: the events serialise the launches. */
+ * 3 final psc.  If per_launch_ms is not NULL, per_launch_ms[k*per_launch_cap + i]
+ * receives the duration of the i-th launch of kind k (i < per_launch_cap).
+ * For measurement only: the events serialise the launches. */
 int btx_rsnn_run_timed(const btx_rsnn_plan* plan, int64_t n_steps, void* stream,
-                       double* ms_out, int64_t* calls_out);
-
-/* Nonzero when an ordered-mode overflow list filled up since the last call
- * (results of such a run are invalid).  Reads 4 bytes synchronously. */
-int btx_rsnn_check_overflow(void* workspace, void* stream);
+                       double* ms_out, int64_t* calls_out, double* per_launch_ms,
+                       int64_t per_launch_cap);
 
 #ifdef __cplusplus
 }

@@ -80,7 +80,7 @@ def test_sparse_conn_ordered_bit_exact(lanes, values):
 
 
 def test_sparse_conn_many_terms_per_destination_overflow_path():
-    # dense columns force > slots terms per destination (overflow list)
+    # dense columns force > slots terms per destination (pulled reduction)
     conn = _conn(500, 16, 0.9, 4)
     x = np.ones((3, 500), np.float32)
     ref = rsnn.NativeSparse(conn)(torch.from_numpy(x))
@@ -181,9 +181,10 @@ def test_engine_plain_with_input_projection_bit_exact():
     assert bits_equal(model.synapse.psc, ref["final"]["psc"])
 
 
-@pytest.mark.parametrize("hard", [False, True])
-def test_engine_heterogeneous_params_bit_exact(hard):
-    n, B, T = 700, 3, 250
+@pytest.mark.parametrize("hard,n", [(False, 700), (True, 700), (False, 703)])
+def test_engine_heterogeneous_params_bit_exact(hard, n):
+    # n = 703 runs the scalar step kernel, n = 700 the 4-wide one
+    B, T = 3, 250
     tau = torch.from_numpy(np.random.default_rng(5).uniform(10, 30, n).astype(np.float32))
     model, conn, _ = _plain_model(n, 0.04, 5, hard=hard, tau=tau, v_reset=-0.1)
     functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
@@ -311,3 +312,29 @@ def test_engine_float64_error_statement():
     v_err = (st["neuron.v"].cpu().double() - r64["v"]).abs()[same.bool()].max().item()
     print(f"\nfloat32 vs float64 oracle (teacher forced): max |dv| = {v_err:.3e}")
     assert v_err < 1e-4
+
+
+@pytest.mark.parametrize("circular", [True, False])
+def test_engine_pull_path_bursts_bit_exact(circular):
+    # slots=2 with a dense, bursty delayed net: most destinations receive > 2
+    # terms per step and are reduced by the exact pull over incoming edges,
+    # including pulls that read the spikes of steps before the run.
+    n, B, T, D = 300, 2, 80, 5
+    c = workloads.powerlaw_net_fast(n=n, k_min=40, k_max=150, delay_lo=1, delay_hi=D - 1,
+                                    seed=31)
+    cexp = c.expanded(D)
+    syn = HeterSynapsePSC(n, 1, None, SparseConn(cexp, sparse_config={"slots": 2}),
+                          base_psc=ExponentialPSC, max_delay_steps=D,
+                          use_circular_buffer=circular, tau_syn=5.0)
+    model = RecurrentNN(LIF(n, tau_ref=0.5), syn, step_mode="m",
+                        update_state_names=["synapse.psc"]).to(DEV)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    x = torch.from_numpy(workloads.poisson_input(T, B, n, 2000.0, 0.1, seed=31) * 25.0)
+    net = _net(n, cexp, tau_ref=0.5, delay_mode="hetero", max_delay_steps=D, circular=circular)
+    ref = rsnn.simulate(net, x)
+    with environ.context(dt=0.1), torch.no_grad():
+        z1, s1 = model(x[:37].to(DEV))
+        z2, s2 = model(x[37:].to(DEV))
+    assert ref["z"].sum() > 500
+    assert bits_equal(torch.cat([z1, z2]), ref["z"])
+    assert bits_equal(torch.cat([s1["synapse.psc"], s2["synapse.psc"]]), ref["psc"])
