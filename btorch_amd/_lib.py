@@ -83,6 +83,37 @@ class RsnnPlan(ctypes.Structure):
     ]
 
 
+class AdjLifArgs(ctypes.Structure):
+    """Mirror of ``btx_adj_lif_args``."""
+
+    _fields_ = [
+        ("n", _c_i64),
+        ("lanes", _c_i64),
+        ("dt", _c_f),
+        ("hard_reset", _c_int),
+        ("detach_reset", _c_int),
+        ("surrogate", _c_int),
+        ("alpha", _c_f),
+        ("damping", _c_f),
+        ("tau", _vp), ("tau_stride", _c_i64),
+        ("v_th", _vp), ("v_th_stride", _c_i64),
+        ("v_reset", _vp), ("v_reset_stride", _c_i64),
+        ("c_m", _vp), ("c_m_stride", _c_i64),
+        ("tau_syn", _vp), ("tau_syn_stride", _c_i64),
+        ("k_syn", _vp), ("k_syn_stride", _c_i64),
+        ("v_prev", _vp),
+        ("v_pre", _vp),
+        ("z", _vp),
+        ("ref_prev", _vp),
+        ("gz", _vp),
+        ("gsyn", _vp),
+        ("gv", _vp),
+        ("gpsc", _vp),
+        ("gx", _vp),
+        ("gtau", _vp),
+    ]
+
+
 _SIGNATURES = {
     "btx_version": (_c_int, []),
     "btx_last_error": (ctypes.c_char_p, []),
@@ -107,6 +138,9 @@ _SIGNATURES = {
                                     ctypes.POINTER(ctypes.c_double), _c_i64]),
     "btx_lif_step": (_c_int, [ctypes.POINTER(RsnnPlan), _vp, _vp, _vp, _vp]),
     "btx_exp_decay": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "btx_adj_lif_step": (_c_int, [ctypes.POINTER(AdjLifArgs), _vp]),
+    "btx_adj_transpose": (_c_int, [_vp, _vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp]),
+    "btx_adj_wgrad": (_c_int, [_vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp, _vp]),
 }
 
 _lock = threading.Lock()

@@ -1,8 +1,26 @@
-"""Training path of the fused engine (placeholder until the adjoint lands)."""
+"""Training path of the fused engine: BPTT through the hand-written adjoint.
+
+The forward of a chunk of K steps runs the fused engine; its backward walks
+the chunk in reverse with ``btx_adj_transpose`` (W dL/dpsc),
+``btx_adj_wgrad`` (event-driven dL/dW over the step's spiking neurons) and
+``btx_adj_lif_step`` (surrogate derivative, reset and synapse-decay adjoint,
+dL/dtau).  With ``grad_checkpoint`` a chunk keeps only its entry state and
+recomputes its forward in the backward (checkpoint every ``chunk_size``
+steps, rnn.py:177-240); otherwise the forward records are kept.
+
+Gradients flow to the packed weights (and from there to ``magnitude``), to
+a trainable ``tau``, to the input sequence and to the entry state v / psc.
+The refractory counter only enters through a detached comparison
+(lif.py:165-167), so it carries no gradient.
+"""
 
 from __future__ import annotations
 
 import torch
+
+from .. import _lib
+from . import _plan
+from .surrogate import Erf, Sigmoid, SuperSpike
 
 
 def needs_grad(rnn, x: torch.Tensor) -> bool:
@@ -10,8 +28,253 @@ def needs_grad(rnn, x: torch.Tensor) -> bool:
         return False
     if x.requires_grad:
         return True
-    return any(p.requires_grad for p in rnn.parameters())
+    if any(p.requires_grad for p in rnn.parameters()):
+        return True
+    n = rnn.neuron
+    return bool(n.v is not None and n.v.requires_grad)
+
+
+def _surrogate_code(fn, require_spiking: bool = True) -> tuple[int, float, float]:
+    if isinstance(fn, SuperSpike):
+        kind = _lib.SURR_SUPERSPIKE
+    elif isinstance(fn, Erf):
+        kind = _lib.SURR_ERF
+    elif isinstance(fn, Sigmoid):
+        kind = _lib.SURR_SIGMOID
+    else:
+        raise NotImplementedError(f"surrogate {type(fn).__name__} has no adjoint kernel")
+    if require_spiking and not fn.spiking:
+        raise NotImplementedError("the fused engine runs the spiking (Heaviside) forward")
+    return kind, float(fn.alpha), float(fn.damping_factor)
+
+
+def _check_supported(spec, lanes):
+    neuron = spec.neuron
+    if spec.history is not None or spec.csr_d != 1 or spec.bins != [(0, 0)]:
+        raise NotImplementedError("the fused adjoint supports synapses without delays")
+    for name in ("v_threshold", "v_reset", "c_m", "tau_ref"):
+        t = getattr(neuron, name, None)
+        if isinstance(t, torch.nn.Parameter) and t.requires_grad:
+            raise NotImplementedError(f"gradient w.r.t. LIF.{name} is not provided")
+    if lanes > 64:
+        raise NotImplementedError("the fused adjoint supports at most 64 lanes")
+
+
+def adjoint_chunk(spec, traj: dict, grads: dict, w_csr: torch.Tensor, dt: float,
+                  valued_weights: bool = False) -> dict:
+    """Reverse a chunk.
+
+    traj: (K, B, N) ``v_prev``, ``v_pre``, ``z`` and ``ref_prev`` (or None),
+        the forward values of every step.
+    grads: ``gz`` (K, B, N) or None, ``gv``/``gpsc`` (B, N) or None -- the
+        gradients of the outputs and of the state after the chunk.
+    valued_weights: z holds real values (relaxed forward) -- the weight
+        gradient then uses the valued product instead of spike masks.
+    Returns ``gx`` (K,B,N), ``gw`` (E,) in CSR order, ``gtau`` ((N,) or (1,)),
+    ``gv``/``gpsc`` (B, N) of the state entering the chunk.
+    """
+    neuron, syn, conn = spec.neuron, spec.exp_psc, spec.conn
+    z = traj["z"]
+    K, B, n = z.shape
+    dev = z.device
+    lib = _lib.load()
+    stream = _lib.stream_ptr()
+
+    def nb(t):  # (B, N) -> (N, B)
+        return t.transpose(-1, -2).contiguous()
+
+    v_prev, v_pre, zz = nb(traj["v_prev"]), nb(traj["v_pre"]), nb(z)
+    ref_prev = nb(traj["ref_prev"]) if traj.get("ref_prev") is not None else None
+    gz = nb(grads["gz"]) if grads.get("gz") is not None else torch.zeros_like(zz)
+    gv = nb(grads["gv"]) if grads.get("gv") is not None else torch.zeros(n, B, device=dev)
+    gpsc = nb(grads["gpsc"]) if grads.get("gpsc") is not None else torch.zeros(n, B,
+                                                                              device=dev)
+    gsyn = torch.empty(n, B, device=dev)
+    gx = torch.empty(K, n, B, device=dev)
+    gw = torch.zeros(conn.nnz, device=dev)
+    keep: list = []
+    plan = _plan.lif_plan(neuron, B, dev, keep)
+    tau_vec, tau_s = keep[0], plan.tau_stride
+    gtau = torch.zeros(tau_vec.numel(), device=dev)
+    ts, tss, k, ks = syn.decay_coefficients(dev)
+    kind, alpha, damping = _surrogate_code(neuron.surrogate_function, require_spiking=False)
+    a = _lib.AdjLifArgs()
+    a.n, a.lanes, a.dt = n, B, float(dt)
+    a.hard_reset = int(bool(neuron.hard_reset))
+    a.detach_reset = int(bool(neuron.detach_reset))
+    a.surrogate, a.alpha, a.damping = kind, alpha, damping
+    a.tau, a.tau_stride = plan.tau, plan.tau_stride
+    a.v_th, a.v_th_stride = plan.v_th, plan.v_th_stride
+    a.v_reset, a.v_reset_stride = plan.v_reset, plan.v_reset_stride
+    a.c_m, a.c_m_stride = plan.c_m, plan.c_m_stride
+    a.tau_syn, a.tau_syn_stride = ts.data_ptr(), tss
+    a.k_syn, a.k_syn_stride = k.data_ptr(), ks
+    a.gsyn = gsyn.data_ptr()
+    a.gv = gv.data_ptr()
+    a.gpsc = gpsc.data_ptr()
+    a.gtau = gtau.data_ptr()
+    layout = conn._layout
+    w = w_csr.detach().to(torch.float32).contiguous()
+    if not valued_weights:
+        shifts = torch.arange(B, device=dev, dtype=torch.int64)
+        masks = ((zz != 0).to(torch.int64) << shifts).sum(-1).contiguous()  # (K, N)
+    for t in range(K - 1, -1, -1):
+        _lib.check(lib.btx_adj_transpose(layout.row_ptr.data_ptr(), layout.col.data_ptr(),
+                                         w.data_ptr(), n, B, gpsc.data_ptr(),
+                                         gsyn.data_ptr(), stream), "btx_adj_transpose")
+        if valued_weights:
+            zt = traj["z"][t].contiguous()
+            from .._kernels import compact_dense, spmv_weight_grad
+            act, cnt = compact_dense(zt)
+            spmv_weight_grad(layout, zt, act, cnt, gpsc.t().contiguous(), gw)
+        else:
+            _lib.check(lib.btx_adj_wgrad(layout.row_ptr.data_ptr(), layout.col.data_ptr(), n,
+                                         B, masks[t].data_ptr(), gpsc.data_ptr(),
+                                         gw.data_ptr(), stream), "btx_adj_wgrad")
+        a.v_prev = v_prev[t].data_ptr()
+        a.v_pre = v_pre[t].data_ptr()
+        a.z = zz[t].data_ptr()
+        a.ref_prev = ref_prev[t].data_ptr() if ref_prev is not None else None
+        a.gz = gz[t].data_ptr()
+        a.gx = gx[t].data_ptr()
+        _lib.check(lib.btx_adj_lif_step(a, stream), "btx_adj_lif_step")
+    return {"gx": gx.permute(0, 2, 1).contiguous(), "gw": gw, "gtau": gtau,
+            "gv": gv.t().contiguous(), "gpsc": gpsc.t().contiguous()}
+
+
+def _set_state(spec, v, ref, psc):
+    neuron, syn = spec.neuron, spec.exp_psc
+    neuron.v = v.detach().clone().contiguous()
+    if neuron._use_refractory:
+        neuron.refractory = ref.detach().clone().contiguous()
+    syn.psc = psc.detach().clone().contiguous()
+
+
+def _forward_chunk(eng, x, v0, ref0, psc0, record):
+    spec = eng.spec
+    _set_state(spec, v0, ref0, psc0)
+    K, B = x.shape[0], x.shape[1]
+    raster, recs = eng.run(x.detach().contiguous(), K, B, record)
+    n = spec.neuron
+    ref_end = n.refractory if n._use_refractory else torch.zeros_like(n.v)
+    return recs, n.v, ref_end, spec.exp_psc.psc
+
+
+class _FusedChunk(torch.autograd.Function):
+    """K fused steps; differentiable w.r.t. x, packed weights, tau, v0, psc0."""
+
+    @staticmethod
+    def forward(ctx, eng, cfg, x, w_csr, tau, v0, ref0, psc0):
+        checkpoint, extra, sink = cfg
+        record = {"z"} | set(extra)
+        if not checkpoint:
+            record |= {"v_prev", "v_pre", "refractory"}
+        recs, v1, ref1, psc1 = _forward_chunk(eng, x, v0, ref0, psc0, record)
+        ctx.eng, ctx.checkpoint, ctx.tau_shape = eng, checkpoint, tau.shape
+        ctx.dt = eng.spec_dt
+        if checkpoint:
+            ctx.save_for_backward(x, w_csr, v0, ref0, psc0)
+        else:
+            ctx.save_for_backward(x, w_csr, v0, ref0, psc0, recs["v_prev"], recs["v_pre"],
+                                  recs["z"], recs["refractory"])
+        for k in extra:
+            sink.setdefault(k, []).append(recs[k])
+        return recs["z"], v1.clone(), ref1.clone(), psc1.clone()
+
+    @staticmethod
+    def backward(ctx, gz, gv, gref, gpsc):
+        from . import environ
+
+        # the backward runs outside the caller's environment: restore its dt
+        with environ.context(dt=ctx.dt):
+            return _FusedChunk._backward(ctx, gz, gv, gpsc)
+
+    @staticmethod
+    def _backward(ctx, gz, gv, gpsc):
+        eng = ctx.eng
+        saved = ctx.saved_tensors
+        x, w_csr, v0, ref0, psc0 = saved[:5]
+        if ctx.checkpoint:
+            # recompute the chunk from its entry state, with records
+            with torch.no_grad():
+                snapshot = _snapshot(eng.spec)
+                recs, _, _, _ = _forward_chunk(eng, x, v0, ref0, psc0,
+                                               {"z", "v_prev", "v_pre", "refractory"})
+                _restore(eng.spec, snapshot)
+            v_prev, v_pre, z, ref = recs["v_prev"], recs["v_pre"], recs["z"], recs["refractory"]
+        else:
+            v_prev, v_pre, z, ref = saved[5:]
+        ref_prev = (torch.cat([ref0.detach().unsqueeze(0), ref[:-1]])
+                    if eng.spec.neuron._use_refractory else None)
+        traj = {"v_prev": v_prev, "v_pre": v_pre, "z": z, "ref_prev": ref_prev}
+        out = adjoint_chunk(eng.spec, traj, {"gz": gz, "gv": gv, "gpsc": gpsc}, w_csr, ctx.dt)
+        gtau = out["gtau"]
+        if len(ctx.tau_shape) == 0 or int(torch.Size(ctx.tau_shape).numel()) == 1:
+            gtau = gtau.sum().reshape(ctx.tau_shape)
+        else:
+            gtau = gtau.reshape(ctx.tau_shape)
+        return (None, None, out["gx"], out["gw"], gtau, out["gv"], None, out["gpsc"])
+
+
+def _snapshot(spec):
+    n = spec.neuron
+    return (n.v, n.refractory if n._use_refractory else None, spec.exp_psc.psc)
+
+
+def _restore(spec, snap):
+    n = spec.neuron
+    n.v = snap[0]
+    if n._use_refractory:
+        n.refractory = snap[1]
+    spec.exp_psc.psc = snap[2]
 
 
 def run_with_grad(rnn, eng, x):
-    raise NotImplementedError("gradients through the fused engine are not available yet")
+    """RecurrentNN.multi_step_forward under autograd through the adjoint."""
+    from . import environ
+
+    spec = eng.spec
+    neuron, syn, conn = spec.neuron, spec.exp_psc, spec.conn
+    n = neuron.size
+    xs, _ = rnn._input_sequence(x, n)
+    T, lanes = xs.shape[0], xs.shape[1]
+    _check_supported(spec, lanes)
+    _surrogate_code(neuron.surrogate_function)
+    eng.spec_dt = float(environ.get("dt"))
+    names = rnn._record_names()
+    want = rnn._wanted_records(names, spec) - {"z", "history"}
+    chunk = T
+    if rnn.grad_checkpoint:
+        chunk = rnn.chunk_size or (T if rnn.unroll is False else int(rnn.unroll))
+    w_csr = conn.packed_weight()
+    tau = neuron.tau
+    v = neuron.v
+    ref = neuron.refractory if neuron._use_refractory else torch.zeros_like(v)
+    psc = syn.psc
+    zs, sink = [], {}
+    cfg = (bool(rnn.grad_checkpoint), tuple(sorted(want)), sink)
+    for c0 in range(0, T, chunk):
+        z, v, ref, psc = _FusedChunk.apply(eng, cfg, xs[c0:c0 + chunk], w_csr, tau, v, ref, psc)
+        zs.append(z)
+    neuron.v = v
+    if neuron._use_refractory:
+        neuron.refractory = ref.detach()
+    syn.psc = psc
+    for m in spec.psc_modules:
+        if m is not syn:
+            m.psc = psc.reshape(m.psc.shape)
+    shape = tuple(v.shape)
+    z = torch.cat(zs).reshape(T, *shape)
+    # recorded hidden states come back without gradient history
+    recs = {k: torch.cat(vals) for k, vals in sink.items()}
+    states = {}
+    for key in names:
+        if key == "neuron.v":
+            states[key] = recs["v"].reshape(T, *shape)
+        elif key == "neuron.refractory":
+            states[key] = recs["refractory"].reshape(T, *shape)
+        elif key == "neuron.v_pre_spike":
+            states[key] = recs["v_pre"].reshape(T, *shape)
+        elif key in spec.psc_keys:
+            states[key] = recs["psc"].reshape(T, *shape)
+    return z, states

@@ -225,6 +225,54 @@ int btx_rsnn_run_timed(const btx_rsnn_plan* plan, int64_t n_steps, void* stream,
                        double* ms_out, int64_t* calls_out, double* per_launch_ms,
                        int64_t per_launch_cap);
 
+/* ------------------------------------------------------------------------ */
+/* Surrogate-gradient adjoint (BPTT), one step backward per call             */
+/* ------------------------------------------------------------------------ */
+/* All arrays of one step are neuron-major (n, lanes): element (j, b) at
+ * j*lanes + b.  Replaces the autograd chain of the reference's eager step
+ * (lif.py:144-187, surrogate/base.py:21-25, synapse.py:171-178). */
+typedef struct btx_adj_lif_args {
+    int64_t n;
+    int64_t lanes;
+    float dt;
+    int hard_reset;
+    int detach_reset;
+    int surrogate;        /* BTX_SURR_* */
+    float alpha;
+    float damping;
+    const float* tau;      int64_t tau_stride;
+    const float* v_th;     int64_t v_th_stride;
+    const float* v_reset;  int64_t v_reset_stride;
+    const float* c_m;      int64_t c_m_stride;
+    const float* tau_syn;  int64_t tau_syn_stride;
+    const float* k_syn;    int64_t k_syn_stride;
+    const float* v_prev;   /* membrane entering the step */
+    const float* v_pre;    /* membrane before reset */
+    const float* z;        /* spikes of the step */
+    const float* ref_prev; /* refractory entering the step, NULL = none */
+    const float* gz;       /* dL/dz of the step */
+    const float* gsyn;     /* (W dL/dpsc')[j]: from btx_adj_transpose */
+    float* gv;             /* in dL/dv after the step, out dL/dv before it */
+    float* gpsc;           /* in dL/dpsc after the step, out before it */
+    float* gx;             /* out dL/dx of the step */
+    float* gtau;           /* accumulated dL/dtau ((n,) or 1 when tau_stride = 0), or NULL */
+} btx_adj_lif_args;
+
+int btx_adj_lif_step(const btx_adj_lif_args* args, void* stream);
+
+/* g_src[j, :] = sum over CSR row j of w[e] * g_dst[col[e], :] (every row;
+ * (n, lanes) layout).  The transposed product of the backward pass. */
+int btx_adj_transpose(const int64_t* row_ptr, const int32_t* col, const float* w,
+                      int64_t n_rows, int64_t lanes, const float* g_dst, float* g_src,
+                      void* stream);
+
+/* gw[e] += sum over lanes b with bit b of masks[j] set of g_dst[col[e], b] for
+ * every edge e of row j (lanes <= 64; masks[j] = spike bits of neuron j over
+ * lanes).  Event-driven weight gradient: rows with an empty mask cost one
+ * load.  gw is in CSR order. */
+int btx_adj_wgrad(const int64_t* row_ptr, const int32_t* col, int64_t n_rows, int64_t lanes,
+                  const uint64_t* masks, const float* g_dst, float* gw, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -237,3 +237,87 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
 def active_lists(spikes: np.ndarray) -> list[np.ndarray]:
     """Ascending active indices per lane (the compaction's closed form)."""
     return [np.flatnonzero(row) .astype(np.int32) for row in np.asarray(spikes)]
+
+
+# ---------------------------------------------------------------------------
+# Autograd restatement (reference gradients; surrogate/base.py:11-62)
+# ---------------------------------------------------------------------------
+
+SIGMOID_K = 2.0 * np.log(np.sqrt(2.0) + 1.0)
+SUPERSPIKE_K = np.sqrt(2.0) - 1.0
+
+
+def surrogate_derivative(kind: str, u, alpha: float, damping: float):
+    """sigmoid.py:22-29, superspike.py:25-31, erf.py:22-28."""
+    if kind == "sigmoid":
+        s = torch.sigmoid(SIGMOID_K * alpha * u)
+        return damping * 4.0 * s * (1.0 - s)
+    if kind == "superspike":
+        return damping / (1.0 + SUPERSPIKE_K * alpha * u.abs()) ** 2
+    return damping * torch.exp(-np.log(2.0) * (alpha * u) ** 2)
+
+
+def surrogate_primitive(kind: str, u, alpha: float):
+    if kind == "sigmoid":
+        return torch.sigmoid(SIGMOID_K * alpha * u)
+    if kind == "superspike":
+        ka = SUPERSPIKE_K * alpha
+        return torch.where(u >= 0.0, 1.0 - 0.5 / (1.0 + ka * u), 0.5 / (1.0 - ka * u))
+    return torch.special.erfc(-np.sqrt(np.log(2.0)) * alpha * u) / 2.0
+
+
+class _Spike(torch.autograd.Function):
+    """Heaviside (or primitive) forward, surrogate-derivative backward."""
+
+    @staticmethod
+    def forward(ctx, u, kind, alpha, damping, spiking):
+        ctx.save_for_backward(u)
+        ctx.cfg = (kind, alpha, damping)
+        return heaviside(u) if spiking else surrogate_primitive(kind, u, alpha)
+
+    @staticmethod
+    def backward(ctx, g):
+        (u,) = ctx.saved_tensors
+        kind, alpha, damping = ctx.cfg
+        return g * surrogate_derivative(kind, u, alpha, damping), None, None, None, None
+
+
+def simulate_grad(n, dst, src, magnitude, x_seq, tau, dt, tau_syn, v_th=1.0, v_reset=0.0,
+                  c_m=1.0, tau_ref=None, hard_reset=False, detach_reset=False,
+                  surrogate=("superspike", 2.0, 1.0, True), state=None, traj=None):
+    """Differentiable RecurrentNN(LIF, ExponentialPSC(SparseConn)) without
+    delays, op for op; ``magnitude`` in coalesced (dst, src) order.  Returns
+    z (T,B,N) and the final (v, refractory, psc); every op records autograd."""
+    T, B, _ = x_seq.shape
+    dtype = x_seq.dtype
+    kind, alpha, damping, spiking = surrogate
+    idx = torch.stack([torch.as_tensor(dst), torch.as_tensor(src)])
+    A = torch.sparse_coo_tensor(idx, magnitude, size=(n, n), is_coalesced=True)
+    vth = torch.as_tensor(v_th, dtype=dtype)
+    vr = torch.as_tensor(v_reset, dtype=dtype)
+    cm = torch.as_tensor(c_m, dtype=dtype)
+    ts = torch.as_tensor(tau_syn, dtype=dtype)
+    st = state or {}
+    v = st.get("v", torch.zeros(B, n, dtype=dtype) + vr)
+    ref = st.get("refractory", torch.zeros(B, n, dtype=dtype))
+    psc = st.get("psc", torch.zeros(B, n, dtype=dtype))
+    zs = []
+    for t in range(T):
+        xin = psc + x_seq[t]
+        if traj is not None:
+            traj.setdefault("v_prev", []).append(v.detach().clone())
+            traj.setdefault("ref_prev", []).append(ref.detach().clone())
+        v = v + dt * (-(v - vr) / tau + xin / cm)
+        if traj is not None:
+            traj.setdefault("v_pre", []).append(v.detach().clone())
+        s = _Spike.apply((v - vth) / (vth - vr), kind, alpha, damping, spiking)
+        if tau_ref is not None:
+            s = s * (ref == 0).detach().to(dtype)
+        sd = s.detach() if detach_reset else s
+        v = v - (v - vr) * sd if hard_reset else v - (vth - vr) * sd
+        if tau_ref is not None:
+            ref = torch.relu(ref + sd * tau_ref - dt)
+        psc = psc + torch.expm1(dt * (-1.0 / ts)) / (-1.0 / ts) * (-psc / ts)
+        psc = psc + torch.sparse.mm(A, s.T).T
+        zs.append(s)
+    return torch.stack(zs), v, ref, psc

@@ -1,0 +1,127 @@
+"""GPU checks of the hand-written adjoint: against the reference's autograd
+(torch-CPU restatement with the surrogate backward) and against float64
+finite differences on a sigmoid-relaxed forward."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from btorch_amd import workloads
+from btorch_amd.models import (
+    LIF, ExponentialPSC, RecurrentNN, Sigmoid, SparseConn, SuperSpike, environ, functional,
+)
+from btorch_amd.models._adjoint import adjoint_chunk
+from oracle import rsnn
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def rel_err(a, b):
+    a, b = a.detach().double().cpu(), b.detach().double().cpu()
+    return float((a - b).norm() / b.norm().clamp_min(1e-30))
+
+
+def _model(n, seed, tau, surrogate, hard=False, tau_ref=2.0, checkpoint=False, chunk=None):
+    c = workloads.plain_lif_net(n=n, p=0.08, seed=seed)
+    conn = c.matrix()
+    lif = LIF(n, tau=tau, tau_ref=tau_ref, hard_reset=hard, trainable_param={"tau"},
+              surrogate_function=surrogate)
+    model = RecurrentNN(lif, ExponentialPSC(n, 5.0, SparseConn(conn)), step_mode="m",
+                        unroll=10, grad_checkpoint=checkpoint, chunk_size=chunk,
+                        update_state_names=[]).to(DEV)
+    return model, conn
+
+
+@pytest.mark.parametrize("hard,checkpoint", [(False, False), (False, True), (True, True)])
+def test_adjoint_matches_reference_autograd(hard, checkpoint):
+    n, B, T = 256, 4, 60
+    rng = np.random.default_rng(3)
+    tau = torch.from_numpy(rng.uniform(10, 30, n).astype(np.float32))
+    sur = SuperSpike(alpha=25.0 / (math.sqrt(2) - 1))  # fast sigmoid, slope 25
+    model, conn = _model(n, 41, tau, sur, hard=hard, checkpoint=checkpoint, chunk=20)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    x = torch.from_numpy(workloads.poisson_input(T, B, n, 400.0, 0.1, seed=41) * 25.0)
+    R = torch.randn(T, B, n, generator=torch.Generator().manual_seed(1))
+    q = torch.randn(B, n, generator=torch.Generator().manual_seed(2))
+    xg = x.to(DEV).requires_grad_(True)
+    with environ.context(dt=0.1):
+        z, _ = model(xg)
+    loss = (z * R.to(DEV)).sum() + (model.neuron.v * q.to(DEV)).sum() + \
+        (model.synapse.psc * q.to(DEV)).sum()
+    loss.backward()
+    # reference autograd on CPU, float32, same ops as the reference
+    conn_m = model.synapse.linear
+    dst, src = conn_m.indices[0].cpu(), conn_m.indices[1].cpu()
+    mag = conn_m.magnitude.detach().cpu().clone().requires_grad_(True)
+    tau_r = tau.clone().requires_grad_(True)
+    xr = x.clone().requires_grad_(True)
+    zr, vr, _, pr = rsnn.simulate_grad(n, dst, src, mag, xr, tau_r, 0.1, 5.0, tau_ref=2.0,
+                                       hard_reset=hard,
+                                       surrogate=("superspike", sur.alpha, 1.0, True))
+    lr = (zr * R).sum() + (vr * q).sum() + (pr * q).sum()
+    lr.backward()
+    assert torch.equal(z.detach().cpu(), zr.detach()), "forward rasters differ"
+    assert zr.sum() > 20
+    e_w = rel_err(conn_m.magnitude.grad, mag.grad)
+    e_t = rel_err(model.neuron.tau.grad, tau_r.grad)
+    e_x = rel_err(xg.grad, xr.grad)
+    print(f"\nrelative L2 error vs reference autograd: dW {e_w:.2e} dtau {e_t:.2e} dx {e_x:.2e}")
+    assert e_w < 1e-4 and e_t < 1e-4 and e_x < 1e-4
+
+
+def test_adjoint_matches_finite_differences_relaxed():
+    # Sigmoid(alpha, damping = k*alpha/4, spiking=False): forward sigmoid(k alpha u),
+    # backward exactly its derivative -> a smooth model finite differences can check.
+    n, B, T, dt = 48, 2, 25, 0.5
+    alpha = 2.0
+    k = 2.0 * math.log(math.sqrt(2.0) + 1.0)
+    rng = np.random.default_rng(7)
+    tau = torch.from_numpy(rng.uniform(10, 30, n))
+    c = workloads.plain_lif_net(n=n, p=0.15, seed=7)
+    dst, src, w = rsnn.coalesce(c.matrix())
+    x = torch.from_numpy(rng.uniform(0, 3, (T, B, n)))
+    R = torch.from_numpy(rng.normal(size=(T, B, n)))
+    q = torch.from_numpy(rng.normal(size=(B, n)))
+    sur = ("sigmoid", alpha, k * alpha / 4.0, False)
+
+    def loss64(mag, tau_):
+        z, v, _, _ = rsnn.simulate_grad(n, dst, src, mag, x, tau_, dt, 5.0, surrogate=sur)
+        return (z * R).sum() + (v * q).sum()
+
+    mag64 = torch.from_numpy(w.astype(np.float64))
+    traj = {}
+    z, v, _, _ = rsnn.simulate_grad(n, dst, src, mag64.float(), x.float(), tau.float(), dt, 5.0,
+                                    surrogate=sur, traj=traj)
+    # the HIP adjoint on that trajectory
+    lif = LIF(n, tau=tau.float(), trainable_param={"tau"},
+              surrogate_function=Sigmoid(alpha, damping_factor=k * alpha / 4.0, spiking=False))
+    conn = SparseConn(c.matrix())
+    model = RecurrentNN(lif, ExponentialPSC(n, 5.0, conn), step_mode="m").to(DEV)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    spec = model.fused_engine().spec
+    tr = {key: torch.stack(vals).float().to(DEV) for key, vals in traj.items()}
+    tr["z"] = z.detach().float().to(DEV)
+    tr["ref_prev"] = None
+    with environ.context(dt=dt):
+        out = adjoint_chunk(spec, tr, {"gz": R.float().to(DEV), "gv": q.float().to(DEV)},
+                            conn.packed_weight().detach(), dt, valued_weights=True)
+    gmag = torch.zeros(conn.nnz, dtype=torch.float64)
+    gmag[conn._csr_perm.cpu()] = out["gw"].double().cpu()
+    eps = 1e-6
+    picks = rng.choice(len(w), size=8, replace=False)
+    for e in picks:
+        mp, mm = mag64.clone(), mag64.clone()
+        mp[e] += eps
+        mm[e] -= eps
+        fd = float((loss64(mp, tau) - loss64(mm, tau)) / (2 * eps))
+        assert abs(gmag[e] - fd) <= 2e-3 * max(1.0, abs(fd)), (e, float(gmag[e]), fd)
+    gt = out["gtau"].double().cpu()
+    for j in rng.choice(n, size=6, replace=False):
+        tp, tm = tau.clone(), tau.clone()
+        tp[j] += 1e-4
+        tm[j] -= 1e-4
+        fd = float((loss64(mag64, tp) - loss64(mag64, tm)) / 2e-4)
+        assert abs(gt[j] - fd) <= 2e-3 * max(1.0, abs(fd)), (j, float(gt[j]), fd)
