@@ -98,6 +98,33 @@ def compact_bits(words: torch.Tensor, n: int) -> tuple[torch.Tensor, torch.Tenso
     return idx, cnt
 
 
+def pack_rows(x: torch.Tensor) -> torch.Tensor:
+    """(rows, n) float32 -> (rows, ceil(n/64)) int64 spike words (x != 0)."""
+    _lib.require_hip(x)
+    rows, n = x.shape
+    if x.stride(1) != 1 or x.dtype != torch.float32:
+        raise ValueError("rows must be float32 with unit inner stride")
+    words = torch.empty((rows, (n + 63) // 64), dtype=torch.int64, device=x.device)
+    _lib.check(_lib.load().btx_pack_bits(x.data_ptr(), rows, n, x.stride(0), words.data_ptr(),
+                                         _lib.stream_ptr()), "btx_pack_bits")
+    return words
+
+
+def unpack_rows(words: torch.Tensor, n: int, out: torch.Tensor | None = None) -> torch.Tensor:
+    """(rows, ceil(n/64)) int64 words -> (rows, n) float32 0/1 (into ``out`` rows if given)."""
+    _lib.require_hip(words)
+    rows = words.shape[0]
+    if words.shape[1] != (n + 63) // 64 or not words.is_contiguous():
+        raise ValueError("words must be contiguous (rows, ceil(n/64))")
+    if out is None:
+        out = torch.empty((rows, n), dtype=torch.float32, device=words.device)
+    if out.shape != (rows, n) or out.stride(1) != 1 or out.dtype != torch.float32:
+        raise ValueError("out must be float32 (rows, n) with unit inner stride")
+    _lib.check(_lib.load().btx_unpack_bits(words.data_ptr(), rows, n, out.data_ptr(),
+                                           out.stride(0), _lib.stream_ptr()), "btx_unpack_bits")
+    return out
+
+
 def spmv_workspace(lanes: int, n_dst: int, slots: int, ordered: bool, device) -> torch.Tensor:
     nbytes = _lib.load().btx_spmv_workspace_bytes(lanes, n_dst, slots, int(ordered))
     return torch.zeros(nbytes, dtype=torch.uint8, device=device)

@@ -54,6 +54,10 @@ class RsnnPlan(ctypes.Structure):
         ("refractory", _vp),
         ("psc", _vp),
         ("x", _vp), ("x_step_stride", _c_i64),
+        ("in_seed", _vp),
+        ("in_prob", _vp),
+        ("in_t0", _vp),
+        ("in_kick", _c_f),
         ("row_ptr", _vp),
         ("col", _vp),
         ("w", _vp),
@@ -70,6 +74,7 @@ class RsnnPlan(ctypes.Structure):
         ("csc_src", _vp),
         ("csc_w", _vp),
         ("hist_words", _vp),
+        ("hist_steps", _c_i64),
         ("teacher", _vp),
         ("raster", _vp),
         ("rec_z", _vp),
@@ -78,6 +83,10 @@ class RsnnPlan(ctypes.Structure):
         ("rec_psc", _vp),
         ("rec_v_pre", _vp),
         ("rec_v_prev", _vp),
+        ("counts", _vp),
+        ("rec_sample", _vp),
+        ("n_sample", _c_i64),
+        ("lane_t_end", _vp),
         ("workspace", _vp),
         ("workspace_bytes", ctypes.c_size_t),
     ]
@@ -120,6 +129,8 @@ _SIGNATURES = {
     "btx_compact_dense": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
     "btx_compact_bits": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
     "btx_spmv_workspace_bytes": (ctypes.c_size_t, [_c_i64, _c_i64, _c_int, _c_int]),
+    "btx_pack_bits": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _vp]),
+    "btx_unpack_bits": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _vp]),
     "btx_spmv_event": (
         _c_int,
         [_vp, _vp, _vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i64,

@@ -329,11 +329,58 @@ __global__ __launch_bounds__(256) void k_weight_grad(const int64_t* __restrict__
     }
 }
 
+// Spike words of rows of a dense fp32 matrix: bit j of word w of a row is
+// (x[row, 64w + j] != 0).  One thread per element, one ballot per word.
+__global__ __launch_bounds__(256) void k_pack_bits(const float* __restrict__ x, int64_t n,
+                                                   int64_t x_stride, uint64_t* __restrict__ words,
+                                                   int64_t nwords) {
+    const int64_t row = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int on = j < n ? (x[row * x_stride + j] != 0.f) : 0;
+    const uint64_t m = __ballot(on);
+    if ((threadIdx.x & 63) == 0) {
+        const int64_t w = j >> 6;
+        if (w < nwords) words[row * nwords + w] = m;
+    }
+}
+
+// out[row, j] = bit j of the row's words, as 0.0f / 1.0f.
+__global__ __launch_bounds__(256) void k_unpack_bits(const uint64_t* __restrict__ words,
+                                                     int64_t n, int64_t nwords,
+                                                     float* __restrict__ out,
+                                                     int64_t out_stride) {
+    const int64_t row = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t w = words[row * nwords + (j >> 6)];
+    out[row * out_stride + j] = ((w >> (j & 63)) & 1ull) ? 1.f : 0.f;
+}
+
 }  // namespace btx
 
 using namespace btx;
 
 BTX_API int btx_version(void) { return 100; }
+
+BTX_API int btx_pack_bits(const float* x, int64_t rows, int64_t n, int64_t x_row_stride,
+                          uint64_t* words, void* stream) {
+    if (rows <= 0 || n <= 0) return 0;
+    if (rows > 65535) return fail("btx_pack_bits: more than 65535 rows");
+    const int64_t nwords = (n + 63) / 64;
+    hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((n + 255) / 256), (unsigned)rows), dim3(256),
+                       0, (hipStream_t)stream, x, n, x_row_stride, words, nwords);
+    return check_launch("btx_pack_bits");
+}
+
+BTX_API int btx_unpack_bits(const uint64_t* words, int64_t rows, int64_t n, float* out,
+                            int64_t out_row_stride, void* stream) {
+    if (rows <= 0 || n <= 0) return 0;
+    if (rows > 65535) return fail("btx_unpack_bits: more than 65535 rows");
+    const int64_t nwords = (n + 63) / 64;
+    hipLaunchKernelGGL(k_unpack_bits, dim3((unsigned)((n + 255) / 256), (unsigned)rows),
+                       dim3(256), 0, (hipStream_t)stream, words, n, nwords, out, out_row_stride);
+    return check_launch("btx_unpack_bits");
+}
 
 BTX_API const char* btx_last_error(void) { return g_last_error.c_str(); }
 

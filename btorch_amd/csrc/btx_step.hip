@@ -69,6 +69,17 @@ struct StepArgs {
     const float* csc_w;
     const uint64_t* words;       // pushed spike words of this run [T][lanes][nwords]
     const uint64_t* hist_words;  // words of the ring_d steps before the run [ring_d][lanes][nwords]
+    int64_t hist_steps;          // > 0: hist_words is the raster of that many preceding steps
+    // in-kernel Poisson drive
+    const uint64_t* in_seed;
+    const float* in_prob;
+    const int64_t* in_t0;
+    float in_kick;
+    int64_t t;                   // step of this launch within the run
+    int* counts;
+    float* rec_sample;           // step t, [lanes][n_sample]
+    int64_t n_sample;
+    const int64_t* lane_t_end;
     int ring_d;
     int cursor;
     int csr_d;
@@ -91,6 +102,8 @@ __device__ __forceinline__ bool pulled_active(const StepArgs& a, int64_t lane, i
     const uint64_t* row;
     if (st >= 0) {
         row = a.words + (st * a.lanes + lane) * a.nwords;
+    } else if (a.hist_steps > 0) {
+        row = a.hist_words + ((a.hist_steps + st) * a.lanes + lane) * a.nwords;
     } else {
         int64_t slot = (a.cursor + st) % a.ring_d;
         if (slot < 0) slot += a.ring_d;
@@ -138,16 +151,34 @@ __device__ __forceinline__ float take_input(const StepArgs& a, int64_t i, int64_
     return r;
 }
 
+// SplitMix64 finaliser (Steele, Lea, Flood 2014): a bijective 64-bit mix.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// In-kernel Poisson drive of neuron nidx of `lane` at this step.
+__device__ __forceinline__ float poisson_drive(const StepArgs& a, int64_t lane, int64_t nidx) {
+    if (a.in_prob == nullptr) return 0.f;
+    const uint64_t s = (uint64_t)(a.in_t0[lane] + a.t);
+    const uint64_t h = mix64(a.in_seed[lane] ^ mix64(s * 0x100000001b3ull + (uint64_t)nidx));
+    const float u = (float)(h >> 40) * (1.0f / 16777216.0f);
+    return u < a.in_prob[lane] ? a.in_kick : 0.f;
+}
+
 // One neuron of the fused step.  `psc` enters as the decayed current of the
 // previous step (without its push), leaves as this step's decayed current.
 // Returns the spike (0/1).
 __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
                                                 int64_t i, float& psc, float xin, float& v,
                                                 float& ref, float wz) {
+    // xin: external input (tensor) of this neuron-lane, before the Poisson drive
     if (!a.first) {
         psc = psc + wz;
         if (a.rec_psc_prev) a.rec_psc_prev[i] = psc;
     }
+    xin = xin + poisson_drive(a, lane, nidx);
     const float I = a.psc ? psc + xin : xin;
     const float tau = prm(a.tau, a.tau_s, nidx);
     const float vth = prm(a.vth, a.vth_s, nidx);
@@ -167,6 +198,9 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
     }
     v = a.hard ? (v1 - (v1 - vr) * s) : (v1 - (vth - vr) * s);
     if (a.rec_v) a.rec_v[i] = v;
+    const bool live = a.lane_t_end == nullptr || a.in_t0[lane] + a.t < a.lane_t_end[lane];
+    if (a.counts && s != 0.f && live) a.counts[i] += 1;
+    if (a.rec_sample && nidx < a.n_sample && live) a.rec_sample[lane * a.n_sample + nidx] = v;
     if (a.rec_z) a.rec_z[i] = s;
     if (a.psc) {
         const float tsyn = prm(a.tsyn, a.tsyn_s, nidx);
@@ -420,6 +454,14 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     if (p->ordered && (p->csc_ptr == nullptr || p->csc_src == nullptr || p->csc_w == nullptr ||
                        p->hist_words == nullptr))
         return fail("btx_rsnn_run: ordered mode needs the dst-major arrays and history words");
+    if (p->hist_steps > 0 && p->hist_steps < p->ring_d)
+        return fail("btx_rsnn_run: hist_steps must cover the ring");
+    if (p->in_prob != nullptr && (p->in_seed == nullptr || p->in_t0 == nullptr))
+        return fail("btx_rsnn_run: Poisson drive needs per-lane seeds and clocks");
+    if (p->lane_t_end != nullptr && p->in_t0 == nullptr)
+        return fail("btx_rsnn_run: lane_t_end needs per-lane clocks");
+    if (p->rec_sample != nullptr && (p->n_sample < 1 || p->n_sample > p->n))
+        return fail("btx_rsnn_run: n_sample must be in [1, n]");
     if (p->csr_d > BTX_MAX_BINS) return fail("btx_rsnn_run: csr_d exceeds 64");
     const WsLayout L = ws_layout(p->lanes, p->n, p->slots, p->ordered);
     if (p->workspace_bytes < L.total) return fail("btx_rsnn_run: workspace too small");
@@ -441,6 +483,14 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     a.csc_w = p->csc_w;
     a.words = p->teacher ? p->teacher : p->raster;
     a.hist_words = p->hist_words;
+    a.hist_steps = p->hist_steps;
+    a.in_seed = p->in_seed;
+    a.in_prob = p->in_prob;
+    a.in_t0 = p->in_t0;
+    a.in_kick = p->in_kick;
+    a.counts = p->counts;
+    a.n_sample = p->n_sample;
+    a.lane_t_end = p->lane_t_end;
     a.ring_d = p->ring_d;
     a.cursor = p->ring_cursor;
     a.csr_d = p->csr_d;
@@ -486,6 +536,8 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         a.first = (t == 0);
         a.push_step = t - 1;
         a.x = p->x ? p->x + t * p->x_step_stride : nullptr;
+        a.t = t;
+        a.rec_sample = p->rec_sample ? p->rec_sample + t * lanes * p->n_sample : nullptr;
         a.raster = p->raster + t * lanes * nwords;
         a.rec_z = p->rec_z ? p->rec_z + t * rec_step : nullptr;
         a.rec_v = p->rec_v ? p->rec_v + t * rec_step : nullptr;

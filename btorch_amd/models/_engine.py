@@ -133,7 +133,13 @@ class Engine:
         return ws
 
     def run(self, x: torch.Tensor | None, T: int, lanes: int, record: set[str],
-            teacher: torch.Tensor | None = None, timings: dict | None = None):
+            teacher: torch.Tensor | None = None, timings: dict | None = None,
+            drive: dict | None = None, counts: torch.Tensor | None = None,
+            sample: torch.Tensor | None = None):
+        """drive: {"seed": (lanes,) int64, "prob": (lanes,) f32, "t0": (lanes,) int64,
+        "kick": float, optional "t_end": (lanes,) int64} in-kernel Poisson input;
+        counts: (lanes, n) int32 accumulated spike counts; sample: (T, lanes, S)
+        f32 membrane of neurons 0..S-1 each step."""
         """x: (T, lanes, n) fp32 contiguous or None.  Returns (raster, records)."""
         sp = self.spec
         neuron, syn, conn = sp.neuron, sp.exp_psc, sp.conn
@@ -201,7 +207,9 @@ class Engine:
             lib = _lib.load()
             order = (list(range(ring_d)) if hist.use_circular_buffer
                      else [ring_d - 1 - j for j in range(ring_d)])
-            hist_words.copy_(pack_bits(h.transpose(0, 1)[order]))
+            from .._kernels import pack_rows
+            for j in range(ring_d):
+                hist_words[j] = pack_rows(h[:, order[j]])
             for j in range(ring_d):
                 src = order[j]
                 _lib.check(lib.btx_compact_dense(h.data_ptr() + 4 * src * n, lanes, n, ring_d * n,
@@ -218,6 +226,31 @@ class Engine:
         for j, (age, sub) in enumerate(sp.bins):
             p.bin_delay[j] = age
             p.bin_sub[j] = sub
+        if drive is not None:
+            for key, dt_ in (("seed", torch.int64), ("prob", torch.float32), ("t0", torch.int64)):
+                t_ = drive[key]
+                if t_.shape != (lanes,) or t_.dtype != dt_ or not t_.is_contiguous():
+                    raise ValueError(f"drive[{key!r}] must be contiguous {dt_} of shape ({lanes},)")
+            p.in_seed = drive["seed"].data_ptr()
+            p.in_prob = drive["prob"].data_ptr()
+            p.in_t0 = drive["t0"].data_ptr()
+            p.in_kick = float(drive["kick"])
+            if drive.get("t_end") is not None:
+                te = drive["t_end"]
+                if te.shape != (lanes,) or te.dtype != torch.int64 or not te.is_contiguous():
+                    raise ValueError("drive['t_end'] must be contiguous int64 (lanes,)")
+                p.lane_t_end = te.data_ptr()
+        if counts is not None:
+            if counts.shape != (lanes, n) or counts.dtype != torch.int32 \
+                    or not counts.is_contiguous():
+                raise ValueError("counts must be contiguous int32 (lanes, n)")
+            p.counts = counts.data_ptr()
+        if sample is not None:
+            if sample.dim() != 3 or sample.shape[:2] != (T, lanes) or not sample.is_contiguous() \
+                    or sample.dtype != torch.float32 or not 1 <= sample.shape[2] <= n:
+                raise ValueError("sample must be contiguous float32 (T, lanes, S), 1 <= S <= n")
+            p.rec_sample = sample.data_ptr()
+            p.n_sample = sample.shape[2]
         # outputs
         raster = torch.empty((T, lanes, nwords), dtype=torch.int64, device=dev)
         p.raster = raster.data_ptr()
@@ -260,24 +293,27 @@ class Engine:
         hist = self.spec.history
         if hist is None:
             return
+        from .._kernels import unpack_rows
         D = self.spec.ring_d
         h = hist.history
         hv = h.reshape(lanes, D, n)
         take = min(T, D)
-        recent = unpack_bits(raster[T - take:], n)  # (take, lanes, n), oldest first
         if hist.use_circular_buffer:
-            new = hv.clone()
+            if not hv.is_contiguous():
+                hv = hv.contiguous()
             for i in range(take):
                 t = T - take + i
-                new[:, (cursor + t) % D] = recent[i]
-            hist.history = new.reshape(h.shape)
+                unpack_rows(raster[t], n, out=hv[:, (cursor + t) % D])
+            hist.history = hv.reshape(h.shape)
             hist.set_cursor(cursor + T)
-        else:
-            newest_first = recent.flip(0).transpose(0, 1)  # (lanes, take, n)
-            parts = [newest_first]
-            if take < D:
-                parts.append(hv[:, : D - take])
-            hist.history = torch.cat(parts, dim=1).reshape(h.shape)
+            return
+        # cat mode: index d holds delay d, i.e. step T-1-d
+        new = torch.empty_like(hv)
+        for d in range(take):
+            unpack_rows(raster[T - 1 - d], n, out=new[:, d])
+        if take < D:
+            new[:, take:] = hv[:, : D - take]
+        hist.history = new.reshape(h.shape)
 
     def history_sequence(self, raster, hist0, T, lanes, n, cursor):
         """(T, lanes, D, n) history memory after every step (for recording)."""

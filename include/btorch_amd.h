@@ -52,6 +52,16 @@ int btx_compact_bits(const uint64_t* words, int64_t lanes, int64_t n,
                      int64_t words_lane_stride, int32_t* idx, int64_t idx_lane_stride,
                      int32_t* counts, void* stream);
 
+/* Spike words of each row of a dense fp32 matrix (bit j of word w of a row
+ * = x[row, 64w + j] != 0): words[rows][ceil(n/64)].  The delay ring's
+ * dense history (btorch/models/history.py:28-358) to bit form. */
+int btx_pack_bits(const float* x, int64_t rows, int64_t n, int64_t x_row_stride,
+                  uint64_t* words, void* stream);
+
+/* Inverse of btx_pack_bits: out[row*out_row_stride + j] = bit j, as 0.0f/1.0f. */
+int btx_unpack_bits(const uint64_t* words, int64_t rows, int64_t n, float* out,
+                    int64_t out_row_stride, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Event-driven sparse product (SparseConn forward)                          */
 /* ------------------------------------------------------------------------ */
@@ -155,6 +165,15 @@ typedef struct btx_rsnn_plan {
     float* psc;
     /* external input x[t, lane, n], NULL = none; x_step_stride elements per step */
     const float* x;       int64_t x_step_stride;
+    /* in-kernel Poisson drive, added to x: neuron n of lane b receives in_kick
+     * at its local step s = in_t0[b] + t when the counter-based hash of
+     * (in_seed[b], s, n) falls below in_prob[b]; NULL in_prob = none.  The
+     * drive of a lane depends only on its own seed and clock, so a job gives
+     * the same result alone or packed with others. */
+    const uint64_t* in_seed;
+    const float* in_prob;
+    const int64_t* in_t0;
+    float in_kick;
     /* connectivity: CSR over virtual rows n*csr_d + bin (int64 row_ptr) */
     const int64_t* row_ptr;
     const int32_t* col;
@@ -178,6 +197,9 @@ typedef struct btx_rsnn_plan {
     const int32_t* csc_src;   /* (nnz) virtual source rows, ascending per destination */
     const float* csc_w;       /* (nnz) */
     const uint64_t* hist_words;
+    /* hist_steps > 0: hist_words is instead the raster of the hist_steps
+     * (>= ring_d) steps that precede the run, [hist_steps][lanes][ceil(n/64)] */
+    int64_t hist_steps;
     /* teacher forcing: spike words [T][lanes][ceil(n/64)] pushed into the ring
      * instead of the produced spikes (NULL = free running) */
     const uint64_t* teacher;
@@ -189,6 +211,12 @@ typedef struct btx_rsnn_plan {
     float* rec_psc;       /* [T][lanes][n] or NULL */
     float* rec_v_pre;     /* [T][lanes][n] pre-reset membrane or NULL */
     float* rec_v_prev;    /* [T][lanes][n] membrane entering the step or NULL */
+    int32_t* counts;      /* [lanes][n] spike counts, accumulated, or NULL */
+    float* rec_sample;    /* [T][lanes][n_sample] membrane of neurons 0..n_sample-1, or NULL */
+    int64_t n_sample;
+    /* per-lane end of trial (local step, with in_t0): counts and samples stop
+     * there; NULL = no end */
+    const int64_t* lane_t_end;
     /* workspace from btx_rsnn_workspace_bytes(), zero-filled once */
     void* workspace;      size_t workspace_bytes;
 } btx_rsnn_plan;

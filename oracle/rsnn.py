@@ -321,3 +321,43 @@ def simulate_grad(n, dst, src, magnitude, x_seq, tau, dt, tau_syn, v_th=1.0, v_r
         psc = psc + torch.sparse.mm(A, s.T).T
         zs.append(s)
     return torch.stack(zs), v, ref, psc
+
+
+# ---------------------------------------------------------------------------
+# Counter-based Poisson drive (the serving input; no reference counterpart:
+# the reference feeds a precomputed input tensor, which this reproduces)
+# ---------------------------------------------------------------------------
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def mix64(z: np.ndarray) -> np.ndarray:
+    """SplitMix64 output finaliser (Steele, Lea, Flood, OOPSLA 2014) on uint64."""
+    z = np.asarray(z, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def poisson_drive(seed, prob, t0, T, n, kick) -> torch.Tensor:
+    """(T, B, n) float32 drive: ``kick`` where the hash of (lane seed, local
+    step t0+t, neuron) falls below the lane's probability, else 0.
+
+    u = top 24 bits of mix64(seed ^ mix64(step * 0x100000001b3 + neuron)) / 2^24.
+    """
+    seed = np.asarray(seed, dtype=np.int64).astype(np.uint64)
+    prob = np.asarray(prob, dtype=np.float32)
+    t0 = np.asarray(t0, dtype=np.int64)
+    B = seed.size
+    out = np.zeros((T, B, n), dtype=np.float32)
+    nid = np.arange(n, dtype=np.uint64)
+    for t in range(T):
+        for b in range(B):
+            s = np.uint64(int(t0[b]) + t)
+            with np.errstate(over="ignore"):
+                inner = mix64(s * np.uint64(0x100000001B3) + nid)
+            h = mix64(seed[b] ^ inner)
+            u = (h >> np.uint64(40)).astype(np.float32) * np.float32(1.0 / 16777216.0)
+            out[t, b] = np.where(u < prob[b], np.float32(kick), np.float32(0.0))
+    return torch.from_numpy(out)

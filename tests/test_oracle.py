@@ -163,3 +163,29 @@ def test_float64_oracle_tracks_float32():
     agree = (a["z"].double() == b["z"]).double().mean().item()
     assert agree > 0.999
     assert a["z"].sum() > 0
+
+
+def test_mix64_splitmix64_known_answers():
+    # SplitMix64 with seed 0: outputs mix64(k * golden gamma), k = 1, 2, 3
+    # (Steele, Lea, Flood 2014; Vigna's reference splitmix64.c)
+    g = 0x9E3779B97F4A7C15
+    want = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    got = [int(rsnn.mix64(np.uint64((g * k) % 2**64))) for k in (1, 2, 3)]
+    assert got == want
+
+
+def test_poisson_drive_invariants():
+    n, T = 4096, 40
+    p = np.float32(0.01)
+    d = rsnn.poisson_drive([7, 8, 7], [p, p, p], [0, 0, 0], T, n, 12.0)
+    assert set(np.unique(d.numpy())) <= {0.0, 12.0}
+    # a lane's drive depends on its own seed and step only
+    assert torch.equal(d[:, 0], d[:, 2]) and not torch.equal(d[:, 0], d[:, 1])
+    # the step counter is the lane's own: starting at t0 skips t0 steps
+    d5 = rsnn.poisson_drive([7], [p], [5], T - 5, n, 12.0)
+    assert torch.equal(d5[:, 0], d[5:, 0])
+    # Bernoulli(p) rate within 6 sigma; no draws at p = 0
+    k = float((d[:, :2] > 0).sum())
+    m = 2 * T * n
+    assert abs(k / m - p) < 6 * math.sqrt(p * (1 - p) / m)
+    assert float(rsnn.poisson_drive([1], [0.0], [0], 5, n, 12.0).sum()) == 0.0
