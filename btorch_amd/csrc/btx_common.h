@@ -52,9 +52,12 @@ __device__ __forceinline__ int block_excl_scan(int v, int& total) {
     return r;
 }
 
-// Control words at the head of every workspace (reserved for counters).
+// Control block at the head of every workspace: per-run tables that kernels
+// index with a run-time value.  (Indexing an array held in the kernel
+// argument block with a run-time index makes the compiler copy the whole
+// argument block to per-lane scratch memory; device-memory tables avoid it.)
 struct Ctl {
-    int pad[64];
+    int age_of_sub[64];  // ring age read by virtual-row offset, -1 = not read
 };
 
 // Slots per destination of the ordered mode are held in registers when the

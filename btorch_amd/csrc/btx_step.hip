@@ -83,8 +83,19 @@ struct StepArgs {
     int ring_d;
     int cursor;
     int csr_d;
-    int age_of_sub[BTX_MAX_BINS];  // ring age of virtual-row offset, -1 = not read
+    const int* age_of_sub;  // [csr_d] ring age of virtual-row offset, -1 = not read
 };
+
+// Per-run table of ages, written from the launch arguments by one block.
+struct AgeTable {
+    int v[BTX_MAX_BINS];
+};
+
+__global__ __launch_bounds__(64) void k_set_ages(AgeTable t, int* __restrict__ out) {
+#pragma unroll
+    for (int k = 0; k < BTX_MAX_BINS; ++k)
+        if ((int)threadIdx.x == k) out[k] = t.v[k];
+}
 
 __device__ __forceinline__ float prm(const float* p, int64_t stride, int64_t n) {
     return p[stride * n];
@@ -494,8 +505,12 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     a.ring_d = p->ring_d;
     a.cursor = p->ring_cursor;
     a.csr_d = p->csr_d;
-    for (int k = 0; k < BTX_MAX_BINS; ++k) a.age_of_sub[k] = -1;
-    for (int j = 0; j < p->nbins; ++j) a.age_of_sub[p->bin_sub[j]] = p->bin_delay[j];
+    AgeTable ages;
+    for (int k = 0; k < BTX_MAX_BINS; ++k) ages.v[k] = -1;
+    for (int j = 0; j < p->nbins; ++j) ages.v[p->bin_sub[j]] = p->bin_delay[j];
+    int* age_dev = ((Ctl*)(ws + L.ctl))->age_of_sub;
+    hipLaunchKernelGGL(k_set_ages, dim3(1), dim3(64), 0, s, ages, age_dev);
+    a.age_of_sub = age_dev;
 
     // Per step the push reads, for every delay bin, the compacted spikes of
     // the ring slot that bin's age points at, and streams the bin's rows.

@@ -210,12 +210,10 @@ class Engine:
             from .._kernels import pack_rows
             for j in range(ring_d):
                 hist_words[j] = pack_rows(h[:, order[j]])
-            for j in range(ring_d):
-                src = order[j]
-                _lib.check(lib.btx_compact_dense(h.data_ptr() + 4 * src * n, lanes, n, ring_d * n,
-                                                 ring_idx[j].data_ptr(), n,
-                                                 ring_cnt[j].data_ptr(), _lib.stream_ptr()),
-                           "btx_compact_dense")
+                _lib.check(lib.btx_compact_bits(hist_words[j].data_ptr(), lanes, n, nwords,
+                                                ring_idx[j].data_ptr(), n,
+                                                ring_cnt[j].data_ptr(), _lib.stream_ptr()),
+                           "btx_compact_bits")
         keep.append(hist_words)
         p.hist_words = hist_words.data_ptr()
         p.ring_d = ring_d
