@@ -57,6 +57,11 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests of the serving measurement (0 = skip it)")
+    ap.add_argument("--serve-lanes", type=int, default=256)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="arrivals per second")
+    ap.add_argument("--serve-tmax", type=int, default=5000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 512 MiB device copy first (HBM counter calibration)")
     return ap.parse_args()
@@ -198,30 +203,34 @@ def main():
            if not k.endswith("_each")}
     pct = {k[:-5]: [round(float(np.percentile(v, q)) * 1e3, 2) for q in (50, 90, 99)]
            for k, v in timings.items() if k.endswith("_each") and v}
-    dom = max(("step", "push", "compact"), key=lambda k: per[k][2])
+    kinds = [k for k in ("step", "push", "reduce") if per.get(k, (0, 0, 0))[1] > 0]
+    dom = max(kinds, key=lambda k: per[k][2])
     nwords = (n + 63) // 64
     a_rows_per_launch = rows_k / KS
     ev_per_launch = ev_k / KS
+    spikes_per_launch = float(unpack_count(r_k)) / KS
     bytes_model = {
-        # v, refractory, psc read+write; x and slot count read: 32 B per
-        # neuron-lane; per-neuron tau; raster word; slot records of the
-        # previous step's terms (8 B each)
-        "step": 32.0 * n * lanes + 4.0 * n + 8.0 * nwords * lanes + 8.0 * ev_per_launch,
-        # active index + int64 row_ptr pair per active row; col + weight per
-        # term; slot record + count per term
-        "push": 20.0 * a_rows_per_launch + 8.0 * ev_per_launch + 12.0 * ev_per_launch,
-        # spike words + active indices
-        "compact": 8.0 * nwords * lanes + 4.0 * a_rows_per_launch / max(D - 1, 1),
+        # v, refractory, psc read+write and the input x read: 28 B per
+        # neuron-lane; per-neuron tau; spike words; ring indices of the spikes
+        "step": 28.0 * n * lanes + 4.0 * n + 8.0 * nwords * lanes + 4.0 * spikes_per_launch,
+        # per active row: index + int64 row_ptr pair; per term: col + weight
+        # read, term record (row, weight, destination) written
+        "push": 20.0 * a_rows_per_launch + 18.0 * ev_per_launch,
+        # per term: record read; per term (one destination each, nearly all):
+        # psc read + write
+        "reduce": 18.0 * ev_per_launch,
     }
     avg_ms = per[dom][0]
     achieved = bytes_model[dom] / (avg_ms * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                "kernel": {"step": "k_lif_exp_step", "push": "k_push",
-                           "compact": "k_compact_bits"}[dom],
+                "kernel": {"step": "k_lif_exp_step4", "push": "k_push_step",
+                           "reduce": "k_reduce_bins"}[dom],
                 "bytes_per_launch": bytes_model[dom], "avg_us": avg_ms * 1e3,
                 "kernel_avg_us": {k: round(v[0] * 1e3, 3) for k, v in per.items()},
                 "kernel_p50_p90_p99_us": pct}
+
+    serving = serve_bench(args, model, dev) if args.serve_requests > 0 else None
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
@@ -252,8 +261,78 @@ def main():
                    "accumulation": args.mode, "build_s": round(t_build, 2)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
+
+
+def _private_net(n, seed, dev):
+    from btorch_amd import workloads
+    from btorch_amd.models import LIF, ExponentialPSC, RecurrentNN, SparseConn
+
+    c = workloads.er_net_fast(n=n, p=0.02, seed=seed, w_mu=-3.7)
+    idx = torch.from_numpy(np.stack([c.pre, c.post])).to(dev)
+    coo = torch.sparse_coo_tensor(idx, torch.from_numpy(c.w).to(dev), size=(n, n))
+    return RecurrentNN(LIF(n, tau_ref=2.0), ExponentialPSC(n, 5.0, SparseConn(coo, device=dev)),
+                       step_mode="m", update_state_names=[]).to(dev)
+
+
+def serve_bench(args, model, dev):
+    """Continuous batching vs one job at a time on the shared connectome plus
+    private 4k-32k nets (every fifth request).  Returns the JSON object."""
+    from btorch_amd.serving import LanePool, Server, SimRequest, run_plain
+
+    rng = np.random.default_rng(args.seed + 3)
+    R = args.serve_requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, R))
+    nets = {"a": 4096, "b": 16384, "c": 32768}
+    reqs = []
+    for i in range(R):
+        private = rng.random() < 0.2
+        reqs.append(SimRequest(i, T=int(rng.integers(200, args.serve_tmax + 1)),
+                               batch=int(rng.integers(1, 33)),
+                               rate_scale=float(rng.uniform(0.5, 2.0)),
+                               seed=int(rng.integers(1 << 40)), arrival=float(arrivals[i]),
+                               net=str(rng.choice(list(nets))) if private else "shared"))
+    t_build = time.perf_counter()
+    models = {k: _private_net(v, 100 + j, dev) for j, (k, v) in enumerate(nets.items())}
+    models["shared"] = model
+    D = args.delay_max + 1
+    chunk = {"shared": 10 * D}
+    kw = dict(n_sample=16, base_rate_hz=5.0, kick=args.kick)
+    shared = LanePool(model, lanes=args.serve_lanes, chunk=chunk["shared"], graph=True, **kw)
+
+    def private(key):
+        return LanePool(models[key], lanes=64, chunk=200, stream=torch.cuda.Stream(),
+                        graph=True, **kw)
+
+    # plain path: one job at a time on a pool sized to the job (built up front)
+    plain_pools = {}
+    for r in reqs:
+        key = (r.net, r.batch)
+        if key not in plain_pools:
+            plain_pools[key] = LanePool(models[r.net], lanes=r.batch,
+                                        chunk=chunk.get(r.net, 200), graph=True, **kw)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+
+    mix = Server(shared, private).serve(reqs, realtime=True)
+    plain = run_plain(lambda r: plain_pools[(r.net, r.batch)], reqs)
+    lane_steps = sum(r.T * r.batch for r in reqs)
+
+    def summ(st):
+        return {"wall_s": round(st.wall, 3), "lane_steps_per_s": lane_steps / st.wall,
+                "p50_latency_s": round(st.pct(50), 4), "p99_latency_s": round(st.pct(99), 4),
+                "completed": len(st.results), "rejected": len(st.rejected)}
+
+    m, pl = summ(mix), summ(plain)
+    return {"requests": R, "arrival_rate_per_s": args.serve_rate,
+            "trial_steps": f"U(200, {args.serve_tmax})", "batch": "U{1..32}",
+            "private_nets": "20% of requests on ER p=0.02 nets of 4096/16384/32768 neurons",
+            "shared_pool_lanes": args.serve_lanes, "lane_steps": lane_steps,
+            "continuous": m, "plain": pl,
+            "throughput_vs_plain": m["lane_steps_per_s"] / pl["lane_steps_per_s"],
+            "build_s": round(t_build, 2)}
 
 
 def cpu_baseline(args, conn, snap, x_cpu, r_t, deg, tables, W):

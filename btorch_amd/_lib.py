@@ -17,6 +17,7 @@ import torch
 from ._build import LIB_PATH
 
 MAX_BINS = 64
+RING_CHUNK = 1024
 
 NEURON_LIF = 0
 NEURON_GLIF3 = 1
@@ -73,8 +74,7 @@ class RsnnPlan(ctypes.Structure):
         ("csc_ptr", _vp),
         ("csc_src", _vp),
         ("csc_w", _vp),
-        ("hist_words", _vp),
-        ("hist_steps", _c_i64),
+        ("ring_words", _vp),
         ("teacher", _vp),
         ("raster", _vp),
         ("rec_z", _vp),
@@ -87,6 +87,8 @@ class RsnnPlan(ctypes.Structure):
         ("rec_sample", _vp),
         ("n_sample", _c_i64),
         ("lane_t_end", _vp),
+        ("ring_rebuild", _c_int),
+        ("bin_cap", _c_int),
         ("workspace", _vp),
         ("workspace_bytes", ctypes.c_size_t),
     ]

@@ -28,17 +28,23 @@ class HipSparseConfig(SparseBackendConfig):
 
     ordered: sum each destination's terms in ascending source order (bit-exact
         against the CPU reference); False uses float atomics.
-    slots: per-destination term slots of the ordered mode (2..16); a
-        destination with more terms is reduced by pulling its incoming edges
-        (exact, slower for that destination).
+    slots: per-destination terms the ordered mode sorts (2..16); a
+        destination with more terms in one step is reduced by pulling its
+        incoming edges (exact, slower for that destination).
+    bin_cap: terms per (lane, 4096-destination tile) bin of the recurrent
+        engine's ordered mode, 0 = the maximum (6144); a bin that receives
+        more is reduced by pulling every destination of the tile.
     """
 
     ordered: bool = True
     slots: int = 16
+    bin_cap: int = 0
 
     def validate(self) -> None:
         if self.slots < 2 or self.slots > 16:
             raise ValueError("slots must be in [2, 16].")
+        if self.bin_cap < 0 or self.bin_cap > 6144:
+            raise ValueError("bin_cap must be in [0, 6144].")
 
 
 class SparseConfigRegistry:

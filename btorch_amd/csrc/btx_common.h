@@ -58,6 +58,8 @@ __device__ __forceinline__ int block_excl_scan(int v, int& total) {
 // argument block to per-lane scratch memory; device-memory tables avoid it.)
 struct Ctl {
     int age_of_sub[64];  // ring age read by virtual-row offset, -1 = not read
+    int bin_delay[64];   // ring age of delay bin j
+    int bin_sub[64];     // virtual-row offset of delay bin j
 };
 
 // Slots per destination of the ordered mode are held in registers when the

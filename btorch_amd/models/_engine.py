@@ -132,25 +132,18 @@ class Engine:
             self._ws = {key: ws}
         return ws
 
-    def run(self, x: torch.Tensor | None, T: int, lanes: int, record: set[str],
-            teacher: torch.Tensor | None = None, timings: dict | None = None,
-            drive: dict | None = None, counts: torch.Tensor | None = None,
-            sample: torch.Tensor | None = None):
-        """drive: {"seed": (lanes,) int64, "prob": (lanes,) f32, "t0": (lanes,) int64,
-        "kick": float, optional "t_end": (lanes,) int64} in-kernel Poisson input;
-        counts: (lanes, n) int32 accumulated spike counts; sample: (T, lanes, S)
-        f32 membrane of neurons 0..S-1 each step."""
-        """x: (T, lanes, n) fp32 contiguous or None.  Returns (raster, records)."""
+    def _base_plan(self, lanes: int, keep: list):
+        """Plan fields shared by every run: parameters, state, connectivity."""
         sp = self.spec
         neuron, syn, conn = sp.neuron, sp.exp_psc, sp.conn
         n = neuron.size
         dev = neuron.v.device
         _lib.require_hip(neuron.v)
         cfg = conn.sparse_config
-        keep: list = []
         p = _plan.lif_plan(neuron, lanes, dev, keep)
         p.ordered = int(cfg.ordered)
         p.slots = int(cfg.slots)
+        p.bin_cap = int(getattr(cfg, "bin_cap", 0))
         tau, ts, k, ks = syn.decay_coefficients(dev)
         keep += [tau, k]
         p.tau_syn, p.tau_syn_stride = tau.data_ptr(), ts
@@ -165,15 +158,12 @@ class Engine:
                 setattr(mod, name, t.contiguous())
         p.v = neuron.v.data_ptr()
         p.psc = syn.psc.data_ptr()
+        keep += [neuron.v, syn.psc]
         if neuron._use_refractory:
             if not neuron.refractory.is_contiguous():
                 neuron.refractory = neuron.refractory.contiguous()
             p.refractory = neuron.refractory.data_ptr()
-        if x is not None:
-            if x.shape != (T, lanes, n) or not x.is_contiguous() or x.dtype != torch.float32:
-                raise ValueError("engine input must be contiguous float32 (T, lanes, n)")
-            p.x = x.data_ptr()
-            p.x_step_stride = lanes * n
+            keep.append(neuron.refractory)
         # connectivity
         w = conn.packed_weight().detach()
         if w.dtype != torch.float32:
@@ -189,41 +179,23 @@ class Engine:
         p.csc_ptr = conn._csc_ptr.data_ptr()
         p.csc_src = conn._csc_src.data_ptr()
         p.csc_w = w_dst.data_ptr()
-        # ring of compacted spike lists
-        ring_d = sp.ring_d
-        ring_idx = torch.empty((ring_d, lanes, n), dtype=torch.int32, device=dev)
-        ring_cnt = torch.zeros((ring_d, lanes), dtype=torch.int32, device=dev)
-        cursor = 0
-        hist = sp.history
-        nwords = (n + 63) // 64
-        hist_words = torch.zeros((ring_d, lanes, nwords), dtype=torch.int64, device=dev)
-        if hist is not None:
-            h = hist.history
-            if h is None or h.numel() != lanes * ring_d * n:
-                raise RuntimeError("SpikeHistory is not initialised for this batch")
-            h = h.reshape(lanes, ring_d, n).to(torch.float32).contiguous()
-            keep.append(h)
-            cursor = hist.cursor
-            lib = _lib.load()
-            order = (list(range(ring_d)) if hist.use_circular_buffer
-                     else [ring_d - 1 - j for j in range(ring_d)])
-            from .._kernels import pack_rows
-            for j in range(ring_d):
-                hist_words[j] = pack_rows(h[:, order[j]])
-                _lib.check(lib.btx_compact_bits(hist_words[j].data_ptr(), lanes, n, nwords,
-                                                ring_idx[j].data_ptr(), n,
-                                                ring_cnt[j].data_ptr(), _lib.stream_ptr()),
-                           "btx_compact_bits")
-        keep.append(hist_words)
-        p.hist_words = hist_words.data_ptr()
-        p.ring_d = ring_d
-        p.ring_idx = ring_idx.data_ptr()
-        p.ring_cnt = ring_cnt.data_ptr()
-        p.ring_cursor = cursor
+        p.ring_d = sp.ring_d
         p.nbins = len(sp.bins)
         for j, (age, sub) in enumerate(sp.bins):
             p.bin_delay[j] = age
             p.bin_sub[j] = sub
+        return p, n, dev
+
+    @staticmethod
+    def _ring_buffers(ring_d, lanes, n, dev):
+        nwords = (n + 63) // 64
+        nchunks = (n + _lib.RING_CHUNK - 1) // _lib.RING_CHUNK
+        return (torch.empty((ring_d, lanes, n), dtype=torch.int32, device=dev),
+                torch.zeros((ring_d, lanes, nchunks), dtype=torch.int32, device=dev),
+                torch.zeros((ring_d, lanes, nwords), dtype=torch.int64, device=dev))
+
+    @staticmethod
+    def _set_outputs(p, T, lanes, n, drive, counts, sample):
         if drive is not None:
             for key, dt_ in (("seed", torch.int64), ("prob", torch.float32), ("t0", torch.int64)):
                 t_ = drive[key]
@@ -249,7 +221,50 @@ class Engine:
                 raise ValueError("sample must be contiguous float32 (T, lanes, S), 1 <= S <= n")
             p.rec_sample = sample.data_ptr()
             p.n_sample = sample.shape[2]
-        # outputs
+
+    def run(self, x: torch.Tensor | None, T: int, lanes: int, record: set[str],
+            teacher: torch.Tensor | None = None, timings: dict | None = None,
+            drive: dict | None = None, counts: torch.Tensor | None = None,
+            sample: torch.Tensor | None = None):
+        """x: (T, lanes, n) fp32 contiguous or None.  Returns (raster, records).
+
+        drive: {"seed": (lanes,) int64, "prob": (lanes,) f32, "t0": (lanes,) int64,
+        "kick": float, optional "t_end": (lanes,) int64} in-kernel Poisson input;
+        counts: (lanes, n) int32 accumulated spike counts; sample: (T, lanes, S)
+        f32 membrane of neurons 0..S-1 each step."""
+        sp = self.spec
+        keep: list = []
+        p, n, dev = self._base_plan(lanes, keep)
+        if x is not None:
+            if x.shape != (T, lanes, n) or not x.is_contiguous() or x.dtype != torch.float32:
+                raise ValueError("engine input must be contiguous float32 (T, lanes, n)")
+            p.x = x.data_ptr()
+            p.x_step_stride = lanes * n
+        # delay ring: lists rebuilt on the device from the history memory's words
+        ring_d = sp.ring_d
+        ring_idx, ring_cnt, ring_words = self._ring_buffers(ring_d, lanes, n, dev)
+        cursor = 0
+        hist = sp.history
+        if hist is not None:
+            h = hist.history
+            if h is None or h.numel() != lanes * ring_d * n:
+                raise RuntimeError("SpikeHistory is not initialised for this batch")
+            h = h.reshape(lanes, ring_d, n).to(torch.float32).contiguous()
+            keep.append(h)
+            cursor = hist.cursor
+            order = (list(range(ring_d)) if hist.use_circular_buffer
+                     else [ring_d - 1 - j for j in range(ring_d)])
+            from .._kernels import pack_rows
+            for j in range(ring_d):
+                ring_words[j] = pack_rows(h[:, order[j]])
+        keep += [ring_idx, ring_cnt, ring_words]
+        p.ring_idx = ring_idx.data_ptr()
+        p.ring_cnt = ring_cnt.data_ptr()
+        p.ring_words = ring_words.data_ptr()
+        p.ring_cursor = cursor
+        p.ring_rebuild = 1  # ring lists from ring_words before the first step
+        self._set_outputs(p, T, lanes, n, drive, counts, sample)
+        nwords = (n + 63) // 64
         raster = torch.empty((T, lanes, nwords), dtype=torch.int64, device=dev)
         p.raster = raster.data_ptr()
         if teacher is not None:
@@ -281,11 +296,21 @@ class Engine:
             per = (ctypes.c_double * (4 * cap))()
             _lib.check(_lib.load().btx_rsnn_run_timed(p, T, stream, ms, calls, per, cap),
                        "btx_rsnn_run_timed")
-            for k, name in enumerate(("step", "compact", "push", "finish")):
+            for k, name in enumerate(("step", "reduce", "push", "finish")):
                 timings[name] = (float(ms[k]), int(calls[k]))
                 timings[name + "_each"] = [per[k * cap + i] for i in range(min(calls[k], cap))]
         self._writeback_history(raster, T, lanes, n, cursor)
         return raster, recs
+
+    def resident(self, lanes: int, T: int, drive: dict | None = None,
+                 counts: torch.Tensor | None = None,
+                 sample: torch.Tensor | None = None) -> "ResidentRun":
+        """A fixed launch of T steps over `lanes` lanes whose delay ring stays
+        on the device between launches (the module's history memory is not
+        read or written).  Every pointer is fixed, so a launch can be captured
+        in a HIP graph.  T must be a multiple of the ring depth, so that every
+        launch starts at the same ring slot."""
+        return ResidentRun(self, lanes, T, drive, counts, sample)
 
     def _writeback_history(self, raster, T, lanes, n, cursor):
         hist = self.spec.history
@@ -330,3 +355,52 @@ class Engine:
                     ts = t - j
                     out[t, :, j] = z[ts] if ts >= 0 else init[:, j - t - 1]
         return out
+
+
+class ResidentRun:
+    """T steps of a matched cell with a device-resident delay ring (see
+    ``Engine.resident``).  ``launch()`` enqueues one run on the current
+    stream; ``reset_lanes(sl)`` empties the ring of a lane range (a new job);
+    ``raster`` holds the spike words of the last launch."""
+
+    def __init__(self, engine: Engine, lanes: int, T: int, drive, counts, sample):
+        sp = engine.spec
+        if T % sp.ring_d:
+            raise ValueError(f"T ({T}) must be a multiple of the ring depth ({sp.ring_d})")
+        self.keep: list = []
+        p, n, dev = engine._base_plan(lanes, self.keep)
+        # the state this run owns (the module may later be given other tensors)
+        neuron = sp.neuron
+        self.v = neuron.v
+        self.refractory = neuron.refractory if neuron._use_refractory else None
+        self.psc = sp.exp_psc.psc
+        self.v_reset = neuron.v_reset
+        self.idx, self.cnt, self.words = engine._ring_buffers(sp.ring_d, lanes, n, dev)
+        self.keep += [self.idx, self.cnt, self.words]
+        p.ring_idx = self.idx.data_ptr()
+        p.ring_cnt = self.cnt.data_ptr()
+        p.ring_words = self.words.data_ptr()
+        p.ring_cursor = 0
+        p.ring_rebuild = 0  # lists stay current between launches (all empty at start)
+        engine._set_outputs(p, T, lanes, n, drive, counts, sample)
+        self.raster = torch.empty((T, lanes, (n + 63) // 64), dtype=torch.int64, device=dev)
+        p.raster = self.raster.data_ptr()
+        self.ws = torch.zeros(_lib.load().btx_rsnn_workspace_bytes(lanes, n, p.slots, p.ordered),
+                              dtype=torch.uint8, device=dev)
+        p.workspace = self.ws.data_ptr()
+        p.workspace_bytes = self.ws.numel()
+        self.plan, self.T, self.lanes = p, T, lanes
+
+    def launch(self) -> None:
+        _lib.check(_lib.load().btx_rsnn_run(self.plan, self.T, _lib.stream_ptr()),
+                   "btx_rsnn_run")
+
+    def reset_lanes(self, sl: slice) -> None:
+        """Lanes `sl` at rest: v = v_reset, no refractory time, no current,
+        empty delay ring."""
+        self.v.view(self.lanes, -1)[sl] = self.v_reset.to(self.v.dtype)
+        if self.refractory is not None:
+            self.refractory.view(self.lanes, -1)[sl] = 0.0
+        self.psc.view(self.lanes, -1)[sl] = 0.0
+        self.cnt[:, sl] = 0
+        self.words[:, sl] = 0

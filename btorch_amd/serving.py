@@ -113,7 +113,7 @@ class LanePool:
 
     def __init__(self, model, lanes: int, chunk: int, n_sample: int = 16,
                  base_rate_hz: float = 5.0, kick: float = 12.0, dt: float = 0.1,
-                 stream: torch.cuda.Stream | None = None):
+                 stream: torch.cuda.Stream | None = None, graph: bool = False):
         spec = match_cell(model)
         if spec is None:
             raise ValueError("the pool needs a RecurrentNN cell the fused engine runs")
@@ -138,6 +138,37 @@ class LanePool:
                                   device=dev)
         self.alloc = LaneAllocator(lanes)
         self.jobs: list[_Job] = []
+        from .models import environ
+
+        with environ.context(dt=dt):
+            self.run = self.engine.resident(
+                lanes, chunk, drive={"seed": self.seed, "prob": self.prob, "t0": self.t0,
+                                     "kick": kick, "t_end": self.t_end},
+                counts=self.counts, sample=self.sample)
+        self.graph = None
+        if self.stream is not None:
+            # the buffers above were allocated and filled on the current
+            # stream; the pool's stream must not run ahead of those fills
+            self.stream.wait_stream(torch.cuda.current_stream())
+        if graph:
+            self._capture()
+
+    def _capture(self) -> None:
+        """Capture one chunk (the fused run and the clock advance) in a HIP
+        graph.  Every lane is at rest here (v = v_reset, no current, no
+        drive), so the eager warm-up launch leaves the state unchanged."""
+        from .models import environ
+
+        stream = self.stream or torch.cuda.Stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream), environ.context(dt=self.dt):
+            self.run.launch()
+        stream.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream), environ.context(dt=self.dt):
+            self.run.launch()
+            self.t0 += self.chunk
+        self.graph = g
 
     def bytes_per_lane(self) -> int:
         """Device bytes one lane holds (state, ring, accumulators, outputs)."""
@@ -168,16 +199,7 @@ class LanePool:
         return True
 
     def _reset_lanes(self, sl) -> None:
-        neuron, syn = self.spec.neuron, self.spec.exp_psc
-        neuron.v[sl] = neuron.v_reset.to(neuron.v.dtype)
-        if neuron._use_refractory:
-            neuron.refractory[sl] = 0.0
-        syn.psc[sl] = 0.0
-        if self.spec.history is not None:
-            self.spec.history.history[sl] = 0.0
-        for m in self.spec.psc_modules:
-            if m is not syn:
-                m.psc[sl] = 0.0
+        self.run.reset_lanes(sl)  # state at rest, empty delay ring
         self.counts[sl] = 0
         self.prob[sl] = 0.0
         self.t_end[sl] = 0
@@ -189,17 +211,17 @@ class LanePool:
 
         ctx = torch.cuda.stream(self.stream) if self.stream is not None else _nullctx()
         with ctx, environ.context(dt=self.dt):
-            self.engine.run(None, self.chunk, self.lanes, set(),
-                            drive={"seed": self.seed, "prob": self.prob, "t0": self.t0,
-                                   "kick": self.kick, "t_end": self.t_end},
-                            counts=self.counts, sample=self.sample)
+            if self.graph is not None:
+                self.graph.replay()
+            else:
+                self.run.launch()
+                self.t0 += self.chunk
             for job in self.jobs:
                 lo = job.steps_done
                 k = min(self.chunk, job.req.T - lo)
                 if k > 0:
                     job.traces[lo:lo + k] = self.sample[:k, job.lane0:job.lane0 + job.req.batch]
                 job.steps_done += self.chunk
-            self.t0 += self.chunk
 
     @torch.no_grad()
     def retire(self, now_fn) -> list[SimResult]:

@@ -133,3 +133,19 @@ def poisson_input(T, lanes, n, rate_hz, dt_ms, seed=0) -> np.ndarray:
     rng = np.random.default_rng(seed + 2000)
     p = rate_hz * dt_ms * 1e-3
     return (rng.random((T, lanes, n)) < p).astype(np.float32)
+
+
+def er_net_fast(n=4096, p=0.02, exc_frac=0.8, seed=0, w_mu=-1.0, w_sigma=0.5,
+                inh_scale=4.0) -> Connectome:
+    """Erdos-Renyi net with Binomial(n-1, p) out-degrees and uniform targets
+    (drawn with replacement, then de-duplicated per row; no autapses)."""
+    rng = np.random.default_rng(seed)
+    n_exc = int(round(exc_frac * n))
+    k = rng.binomial(n - 1, p, size=n)
+    pre = np.repeat(np.arange(n, dtype=np.int64), k)
+    post = rng.integers(0, n - 1, size=pre.size, dtype=np.int64)
+    post[post >= pre] += 1
+    key = np.unique(pre * n + post)
+    pre, post = key // n, key % n
+    w = _signed_lognormal(rng, pre, n_exc, inh_scale, w_mu, w_sigma)
+    return Connectome(n, pre, post, w, None, n_exc)

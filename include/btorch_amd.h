@@ -27,6 +27,12 @@ extern "C" {
 
 #define BTX_MAX_BINS 64
 
+/* Neurons per segment of the delay ring's compacted spike lists: each
+ * segment of BTX_RING_CHUNK consecutive neurons holds its own ascending list
+ * and count, so the concatenation of a lane's segments in order is the
+ * lane's ascending active list. */
+#define BTX_RING_CHUNK 1024
+
 /* Library version (major*10000 + minor*100 + patch). */
 int btx_version(void);
 
@@ -134,11 +140,17 @@ int btx_spmv_weight_grad(const int64_t* row_ptr, const int32_t* col, int64_t n_s
  *   z_t   = neuron(I_t)                         (charge -> fire -> reset)
  *   psc_t = psc_{t-1} + k*((-psc_{t-1})/tau_s)  (exp-Euler decay)
  *           + W_exp^T * flatten(history after pushing z_t)
- * Each step, the push reads the compacted spike list of the ring slot each
- * delay bin points at and streams that bin's virtual rows; all deliveries of
- * a step land in one accumulator that the next fused step consumes.
- * Ordered mode keeps each term's virtual-row id so every destination sums in
- * ascending virtual-row order.
+ * Each step, the fused kernel writes the spikes it pushes (its own, or the
+ * teacher's) as segmented ascending lists into the ring slot of this step;
+ * the push reads the lists of the ring slot each delay bin points at and
+ * streams that bin's virtual rows.
+ * Ordered mode: the push bins every term (virtual row, weight) by
+ * (lane, destination tile) with one atomic reservation per workgroup and
+ * tile; a reduction kernel then sorts each tile's terms in LDS by
+ * destination and, per destination, by virtual row, and adds the ascending
+ * sum to psc.  Destinations with many terms, and tiles whose terms overflow
+ * the bin, pull their input over the destination-major edges instead (same
+ * order, same bits).  Fast mode adds every term to psc with a float atomic.
  * Per-neuron parameters are fp32 arrays with a stride (0 = one shared value).
  * Every pointer refers to device memory that outlives the enqueued work. */
 typedef struct btx_rsnn_plan {
@@ -148,7 +160,8 @@ typedef struct btx_rsnn_plan {
     int hard_reset;       /* 0 = soft (v - (vth - vr) s), 1 = hard (v - (v - vr) s) */
     int use_refractory;   /* 1 = refractory counter enabled */
     int ordered;          /* 1 = ascending-source bit-exact accumulation */
-    int slots;            /* per-destination slots for the ordered mode */
+    int slots;            /* ordered mode: a destination with more terms in one
+                           * step pulls its input (1..32) */
     float dt;             /* environ dt as fp32 */
     /* neuron parameters, (n,) or stride 0 */
     const float* tau;      int64_t tau_stride;
@@ -180,26 +193,27 @@ typedef struct btx_rsnn_plan {
     const float* w;
     int64_t nnz;          /* edges (sizes the push group width) */
     int csr_d;            /* virtual rows per neuron (delay-expanded bins) */
-    /* delay ring of compacted spike lists: ring_d slots x lanes x n */
+    /* delay ring of compacted spike lists: ring_d slots x lanes x n, in
+     * segments of BTX_RING_CHUNK neurons (segment c of a lane: entries
+     * [c*BTX_RING_CHUNK, c*BTX_RING_CHUNK + count)) */
     int ring_d;
     int32_t* ring_idx;    /* [ring_d][lanes][n] */
-    int32_t* ring_cnt;    /* [ring_d][lanes] */
+    int32_t* ring_cnt;    /* [ring_d][lanes][ceil(n/BTX_RING_CHUNK)] */
     int ring_cursor;      /* ring slot that receives z of the first step */
     int nbins;            /* delay bins read each step */
     int bin_delay[BTX_MAX_BINS];  /* ring age of bin j (0 = this step) */
     int bin_sub[BTX_MAX_BINS];    /* virtual-row offset of bin j */
     /* destination-major copy of the connectivity (the reference's coalesced
-     * (dst, src) order: indices/magnitude) and the spike words of the ring_d
-     * steps before the run ([ring_d][lanes][ceil(n/64)], ring-slot order);
-     * with them the ordered mode reduces a destination that received more
-     * than `slots` terms exactly, by pulling its incoming edges */
+     * (dst, src) order: indices/magnitude); with it the ordered mode reduces
+     * a destination with many terms exactly, by pulling its incoming edges */
     const int64_t* csc_ptr;   /* (n+1) */
     const int32_t* csc_src;   /* (nnz) virtual source rows, ascending per destination */
     const float* csc_w;       /* (nnz) */
-    const uint64_t* hist_words;
-    /* hist_steps > 0: hist_words is instead the raster of the hist_steps
-     * (>= ring_d) steps that precede the run, [hist_steps][lanes][ceil(n/64)] */
-    int64_t hist_steps;
+    /* spike words pushed into the ring, [ring_d][lanes][ceil(n/64)] in
+     * ring-slot order: on entry slot (cursor - 1 - k) mod ring_d holds the
+     * step k + 1 steps before the run; the run keeps them current (each
+     * step writes its slot), so they carry over to the next run */
+    uint64_t* ring_words;
     /* teacher forcing: spike words [T][lanes][ceil(n/64)] pushed into the ring
      * instead of the produced spikes (NULL = free running) */
     const uint64_t* teacher;
@@ -217,6 +231,13 @@ typedef struct btx_rsnn_plan {
     /* per-lane end of trial (local step, with in_t0): counts and samples stop
      * there; NULL = no end */
     const int64_t* lane_t_end;
+    /* 1 = build the ring lists from ring_words before the first step;
+     * 0 = the ring lists are already current (a resident ring carried over
+     * from the previous run) */
+    int ring_rebuild;
+    /* ordered mode: terms per (lane, destination tile) bin, 0 = the maximum
+     * (6144); a bin that receives more is reduced by pulling */
+    int bin_cap;
     /* workspace from btx_rsnn_workspace_bytes(), zero-filled once */
     void* workspace;      size_t workspace_bytes;
 } btx_rsnn_plan;
@@ -245,8 +266,8 @@ int btx_exp_decay(float* psc, int64_t lanes, int64_t n, const float* tau_syn,
 /* btx_rsnn_run with a HIP event pair around every kernel launch of the
  * step sequence, recorded on `stream`.  On return (after a stream
  * synchronisation) ms_out[k] holds the summed duration of kernel kind k and
- * calls_out[k] its launch count, k = 0 fused step, 1 compaction, 2 push,
- * 3 final psc.  If per_launch_ms is not NULL, per_launch_ms[k*per_launch_cap + i]
+ * calls_out[k] its launch count, k = 0 fused step (with compaction),
+ * 1 ordered reduction, 2 push, 3 final psc.  If per_launch_ms is not NULL, per_launch_ms[k*per_launch_cap + i]
  * receives the duration of the i-th launch of kind k (i < per_launch_cap).
  * For measurement only: the events serialise the launches. */
 int btx_rsnn_run_timed(const btx_rsnn_plan* plan, int64_t n_steps, void* stream,

@@ -315,15 +315,18 @@ def test_engine_float64_error_statement():
 
 
 @pytest.mark.parametrize("circular", [True, False])
-def test_engine_pull_path_bursts_bit_exact(circular):
+@pytest.mark.parametrize("bin_cap", [0, 96])
+def test_engine_pull_path_bursts_bit_exact(circular, bin_cap):
     # slots=2 with a dense, bursty delayed net: most destinations receive > 2
     # terms per step and are reduced by the exact pull over incoming edges,
-    # including pulls that read the spikes of steps before the run.
+    # including pulls that read the spikes of steps before the run; a small
+    # bin capacity makes whole tiles overflow and pull.
     n, B, T, D = 300, 2, 80, 5
     c = workloads.powerlaw_net_fast(n=n, k_min=40, k_max=150, delay_lo=1, delay_hi=D - 1,
                                     seed=31)
     cexp = c.expanded(D)
-    syn = HeterSynapsePSC(n, 1, None, SparseConn(cexp, sparse_config={"slots": 2}),
+    syn = HeterSynapsePSC(n, 1, None, SparseConn(cexp, sparse_config={"slots": 2,
+                                                                      "bin_cap": bin_cap}),
                           base_psc=ExponentialPSC, max_delay_steps=D,
                           use_circular_buffer=circular, tau_syn=5.0)
     model = RecurrentNN(LIF(n, tau_ref=0.5), syn, step_mode="m",

@@ -113,3 +113,29 @@ def test_serving_private_networks_overlap_on_own_streams():
         (r,) = run_plain(lambda _r: pool, [req]).results
         assert torch.equal(r.counts, got[req.rid].counts), req.rid
         assert torch.equal(r.traces, got[req.rid].traces), req.rid
+
+
+def test_serving_graph_replay_matches_eager():
+    # the pool's chunk captured once in a HIP graph and replayed for every
+    # chunk, with admissions, trace copies and retirements (eager work and
+    # synchronisations) between replays; every job equals the eager pool's
+    n, D = 400, 5
+    model, cexp = _model(n, D, 27)
+    chunk = 4 * D
+    kw = dict(n_sample=6, base_rate_hz=1500.0, kick=12.0)
+    rng = np.random.default_rng(8)
+    reqs = [SimRequest(i, T=int(rng.integers(15, 90)), batch=int(rng.integers(1, 4)),
+                       rate_scale=float(rng.uniform(0.5, 2.0)), seed=int(rng.integers(1 << 30)))
+            for i in range(7)]
+    eager = Server(LanePool(model, lanes=6, chunk=chunk, **kw)).serve(reqs, realtime=False)
+    graphed = Server(LanePool(model, lanes=6, chunk=chunk, graph=True,
+                              stream=torch.cuda.Stream(), **kw)).serve(reqs, realtime=False)
+    got = {r.rid: r for r in graphed.results}
+    assert sorted(got) == list(range(7))
+    for r in eager.results:
+        assert torch.equal(r.counts, got[r.rid].counts), r.rid
+        assert torch.equal(r.traces, got[r.rid].traces), r.rid
+    req = reqs[0]
+    counts, traces = _oracle_job(n, D, cexp, req, LanePool(model, lanes=1, chunk=chunk, **kw))
+    assert torch.equal(got[req.rid].counts.cpu(), counts)
+    assert torch.equal(got[req.rid].traces.cpu(), traces)
