@@ -203,29 +203,27 @@ def main():
            if not k.endswith("_each")}
     pct = {k[:-5]: [round(float(np.percentile(v, q)) * 1e3, 2) for q in (50, 90, 99)]
            for k, v in timings.items() if k.endswith("_each") and v}
-    kinds = [k for k in ("step", "push", "reduce") if per.get(k, (0, 0, 0))[1] > 0]
-    dom = max(kinds, key=lambda k: per[k][2])
+    # every step is one launch (F and P workgroups) when no delay bin has age 0
+    dom = "step"
     nwords = (n + 63) // 64
     a_rows_per_launch = rows_k / KS
     ev_per_launch = ev_k / KS
     spikes_per_launch = float(unpack_count(r_k)) / KS
     bytes_model = {
-        # v, refractory, psc read+write and the input x read: 28 B per
-        # neuron-lane; per-neuron tau; spike words; ring indices of the spikes
-        "step": 28.0 * n * lanes + 4.0 * n + 8.0 * nwords * lanes + 4.0 * spikes_per_launch,
-        # per active row: index + int64 row_ptr pair; per term: col + weight
-        # read, term record (row, weight, destination) written
-        "push": 20.0 * a_rows_per_launch + 18.0 * ev_per_launch,
-        # per term: record read; per term (one destination each, nearly all):
-        # psc read + write
-        "reduce": 18.0 * ev_per_launch,
+        # F: v, refractory, psc read+write and the input x read (28 B per
+        # neuron-lane), per-neuron tau, spike words to the raster and the
+        # ring, ring indices of the spikes; the previous push's term records
+        # read (10 B per term: row, weight, destination).  P: per active row
+        # the index and the int64 row_ptr pair (20 B), per term col + weight
+        # read and the term record written (18 B).
+        "step": (28.0 * n * lanes + 4.0 * n + 16.0 * nwords * lanes + 4.0 * spikes_per_launch
+                 + 20.0 * a_rows_per_launch + 28.0 * ev_per_launch),
     }
     avg_ms = per[dom][0]
     achieved = bytes_model[dom] / (avg_ms * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                "kernel": {"step": "k_lif_exp_step4", "push": "k_push_step",
-                           "reduce": "k_reduce_bins"}[dom],
+                "kernel": "k_step (fused step + push + reduction)",
                 "bytes_per_launch": bytes_model[dom], "avg_us": avg_ms * 1e3,
                 "kernel_avg_us": {k: round(v[0] * 1e3, 3) for k, v in per.items()},
                 "kernel_p50_p90_p99_us": pct}
@@ -298,7 +296,7 @@ def serve_bench(args, model, dev):
     models = {k: _private_net(v, 100 + j, dev) for j, (k, v) in enumerate(nets.items())}
     models["shared"] = model
     D = args.delay_max + 1
-    chunk = {"shared": 10 * D}
+    chunk = {"shared": 10 * (D + 1)}
     kw = dict(n_sample=16, base_rate_hz=5.0, kick=args.kick)
     shared = LanePool(model, lanes=args.serve_lanes, chunk=chunk["shared"], graph=True, **kw)
 

@@ -87,6 +87,13 @@ class RsnnPlan(ctypes.Structure):
         ("rec_sample", _vp),
         ("n_sample", _c_i64),
         ("lane_t_end", _vp),
+        ("n_asc", _c_int),
+        ("v_rest", _vp), ("v_rest_stride", _c_i64),
+        ("k_v", _vp), ("k_v_stride", _c_i64),
+        ("asc_k", _vp),
+        ("asc_kk", _vp),
+        ("asc_amp", _vp),
+        ("iasc", _vp),
         ("ring_rebuild", _c_int),
         ("bin_cap", _c_int),
         ("workspace", _vp),

@@ -1,23 +1,30 @@
 // Fused recurrent time step on gfx950.
 //
-// Per step t, in stream order:
-//   F(t)  one elementwise pass over (lane, neuron): the current psc_{t-1}
-//         (already holding the previous push), LIF charge/fire/refractory/
-//         reset, the exponential-synapse decay psc_dec, spike words -> raster,
-//         and the pushed spikes (own or teacher's) as ascending lists per
-//         1024-neuron segment into ring slot (cursor + t) % ring_d.
-//   P(t)  event-driven push: for every delay bin j, the active neurons of the
-//         ring slot bin j points at, each streaming its virtual row
-//         n*csr_d + bin_sub[j] of the CSR.
-//         ordered: terms are binned by (lane, 4096-destination tile) with one
-//                  global reservation per workgroup and tile;
-//         fast:    terms are added to psc with float atomics.
-//   R(t)  (ordered) one workgroup per bin sorts the bin's terms in LDS by
-//         destination, then each destination's terms by virtual row, and adds
-//         the ascending sum to psc; a destination with many terms, or every
-//         destination of an overflowing bin, pulls its input over its
-//         incoming edges in the same order instead.
-// After the last step FIN normalises psc (psc + 0.0f).
+// Every step is one launch whose workgroups play two roles:
+//   F (lane, 1024-neuron segment c): first reduces the previous step's push
+//     into its own neurons -- the terms binned for (lane, c), sorted in LDS by
+//     neuron and, per neuron, by virtual source row, summed from +0.0f in that
+//     order -- then integrates the step: psc_{t-1} = psc_dec + sum, LIF
+//     charge/fire/refractory/reset, exponential-synapse decay psc_dec, spike
+//     words -> raster, and the pushed spikes (own or teacher's) as the
+//     segment's ascending list and words into ring slot (cursor + t) % R.
+//   P (lane, range of segments): the event-driven push of step t -- for every
+//     delay bin j, the neurons the ring lists for step t - bin_delay[j], each
+//     streaming its CSR virtual row src*csr_d + bin_sub[j]; every term
+//     (virtual row, weight) is binned by (lane, destination segment) with one
+//     global reservation per workgroup and segment (ordered mode), or added
+//     to a float accumulator with an atomic (fast mode).
+// The push of step t only reads ring slots of earlier steps when no delay
+// bin has age 0, so both roles share the launch; otherwise the F and P
+// workgroups of a step are two launches.  Term bins and accumulators are
+// double-buffered by step parity.  The ring has R = ring_d + 1 slots so that
+// the slot a step writes is never one the concurrent reduction still reads.
+// After the last step FIN reduces the last push into psc.
+//
+// A neuron with more than `slots` terms, or every neuron of a bin that
+// overflowed, takes its input by pulling its incoming edges in the
+// destination-major (reference coalesced) order instead: the same terms in
+// the same order, so the same bits.
 //
 // Op order follows the reference's eager expressions exactly (compiled with
 // -ffp-contract=off):
@@ -29,8 +36,6 @@
 //   adaptation  psc + W^T z, W^T z summed per destination from +0.0f in
 //               ascending source row order (torch.sparse.mm on the coalesced
 //               (dst, src) matrix, linear.py:474-483)    synapse.py:82-87
-// The reference adds the product's +0.0f to destinations without terms,
-// which turns a -0.0f psc into +0.0f; the next step's psc + 0.0f does the same.
 
 #include <hip/hip_runtime.h>
 
@@ -42,16 +47,46 @@
 
 namespace btx {
 
-constexpr int kChunk = BTX_RING_CHUNK;  // neurons per fused-step workgroup / ring segment
-constexpr int kTileShift = 12;
-constexpr int kTile = 1 << kTileShift;  // destinations per ordered-mode bin
-constexpr int kMaxTiles = 4096;         // n <= 16.7M
-constexpr int kBinCap = 6144;           // terms per bin held in LDS by the reduction
-constexpr int kSortMax = 32;            // most terms a destination sorts in LDS
-constexpr int kRedThreads = 512;
-constexpr int kPushThreads = 256;
+constexpr int kChunk = BTX_RING_CHUNK;  // neurons per F workgroup = ring segment = bin
+constexpr int kChunkShift = 10;
+constexpr int kThreads = 256;           // 4 neurons per thread in the F role
+constexpr int kBinCap = 2048;           // terms per bin held in LDS
+constexpr int kSortMax = 32;            // most terms a neuron sorts in LDS
+constexpr int kMaxChunks = 4096;        // n <= 4.19M (push reservation table)
 
-static_assert(kChunk == 1024, "fused-step workgroups cover 1024 neurons");
+static_assert(kChunk == (1 << kChunkShift) && kChunk == 4 * kThreads, "segment layout");
+
+// ---------------------------------------------------------------------------
+// Arguments
+// ---------------------------------------------------------------------------
+
+struct PullArgs {
+    const int64_t* csc_ptr;
+    const int32_t* csc_src;
+    const float* csc_w;
+    const uint64_t* ring_words;  // [R][lanes][nwords], ring-slot order
+    int64_t lanes;
+    int64_t nwords;
+    int R;
+    int64_t clock;               // cursor + step of the push being reduced
+    int csr_d;
+    const int* age_of_sub;       // [csr_d] ring age by virtual-row offset, -1 = not read
+};
+
+// Reduction of one step's push into the neurons of one segment.
+struct ReduceIn {
+    int ordered;
+    int cap;
+    int sort_max;
+    int* bin_cnt;                // [lanes][nchunks] of the buffer being consumed
+    const int32_t* ev_r;         // [lanes][nchunks][kBinCap]
+    const float* ev_w;
+    const uint16_t* ev_d;
+    float* acc;                  // fast mode: [lanes][n] of the buffer being consumed
+    PullArgs pull;
+};
+
+constexpr int kMaxAsc = 4;
 
 struct StepArgs {
     int64_t n;
@@ -60,7 +95,15 @@ struct StepArgs {
     float dt;
     int hard;
     int use_ref;
-    int first;                   // first step of the run: psc is the stored state
+    int glif;              // 1 = GLIF3 membrane with after-spike currents
+    int n_asc;
+    const float* vrest;    int64_t vrest_s;
+    const float* kv;       int64_t kv_s;
+    const float* asc_k;    // (n, n_asc)
+    const float* asc_kk;
+    const float* asc_amp;
+    float* iasc;           // (lanes, n, n_asc)
+    int prev;                    // 1 = the previous step's push is pending
     const float* tau;      int64_t tau_s;
     const float* vth;      int64_t vth_s;
     const float* vr;       int64_t vr_s;
@@ -72,7 +115,7 @@ struct StepArgs {
     float* ref;
     float* psc;
     const float* x;        // step t, (lanes, n) or NULL
-    uint64_t* raster;      // step t, (lanes, nwords)
+    uint64_t* raster;      // step t, (lanes, nwords) or NULL
     const uint64_t* teacher;  // step t, (lanes, nwords) or NULL
     int32_t* ring_idx;     // this step's ring slot, [lanes][n], or NULL
     int32_t* ring_cnt;     // this step's ring slot, [lanes][nchunks]
@@ -93,7 +136,57 @@ struct StepArgs {
     float* rec_sample;     // step t, [lanes][n_sample]
     int64_t n_sample;
     const int64_t* lane_t_end;
+    int vec4;              // 16-byte aligned state and n % 4 == 0
+    ReduceIn red;          // the previous step's push
 };
+
+struct PushStep {
+    const int64_t* row_ptr;
+    const int32_t* col;
+    const float* w;
+    int64_t n;
+    int64_t lanes;
+    int64_t nchunks;
+    int cpb;                     // segments per P workgroup
+    int nbins;
+    int csr_d;
+    int R;
+    int64_t clock;               // cursor + t
+    const Ctl* ctl;              // bin_delay / bin_sub tables
+    const int32_t* ring_idx;     // [R][lanes][n]
+    const int32_t* ring_cnt;     // [R][lanes][nchunks]
+    int cap;
+    int* bin_cnt;                // [lanes][nchunks] of this step's buffer
+    int32_t* ev_r;               // [lanes][nchunks][kBinCap]
+    float* ev_w;
+    uint16_t* ev_d;
+    float* acc;                  // fast mode: [lanes][n] of this step's buffer
+};
+
+// LDS of one workgroup, shared by the two roles.
+struct RedLds {
+    int cnt[kChunk];
+    int pos[kChunk];
+    int r[kBinCap];
+    float w[kBinCap];
+    uint16_t d[kBinCap];
+    uint16_t ord[kBinCap];
+};
+struct PushLds {
+    int off[kThreads + 1];
+    int slot[BTX_MAX_BINS];
+    int sub[BTX_MAX_BINS];
+    int tile[kMaxChunks];
+    int base[kMaxChunks];
+};
+union StepLds {
+    RedLds red;
+    PushLds push;
+};
+
+// ---------------------------------------------------------------------------
+// Neuron update
+// ---------------------------------------------------------------------------
 
 __device__ __forceinline__ float prm(const float* p, int64_t stride, int64_t n) {
     return p[stride * n];
@@ -115,14 +208,15 @@ __device__ __forceinline__ float poisson_drive(const StepArgs& a, int64_t lane, 
     return u < a.in_prob[lane] ? a.in_kick : 0.f;
 }
 
-// One neuron of the fused step.  `psc` enters as psc_{t-1} (decayed current
-// of the previous step plus its push), leaves as this step's decayed current.
-// Returns the spike (0/1).
+// One neuron of the fused step.  `psc` enters as the stored psc (decayed
+// current of the previous step) and `wz` as the previous step's summed push;
+// leaves as this step's decayed current.  `ia` holds the neuron's after-spike
+// currents (GLIF3).  Returns the spike (0/1).
 __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
-                                                int64_t i, float& psc, float xin, float& v,
-                                                float& ref) {
-    if (!a.first) {
-        psc = psc + 0.f;
+                                                int64_t i, float& psc, float wz, float xin,
+                                                float& v, float& ref, float* ia) {
+    if (a.prev) {
+        psc = psc + wz;
         if (a.rec_psc_prev) a.rec_psc_prev[i] = psc;
     }
     xin = xin + poisson_drive(a, lane, nidx);
@@ -132,8 +226,24 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
     const float vr = prm(a.vr, a.vr_s, nidx);
     const float cm = prm(a.cm, a.cm_s, nidx);
     if (a.rec_v_prev) a.rec_v_prev[i] = v;
-    const float dv = (-(v - vr)) / tau + I / cm;
-    const float v1 = v + a.dt * dv;
+    float v1;
+    if (a.glif) {
+        // charge (exponential Euler, Iasc of the previous step)
+        float isum = 0.f;
+#pragma unroll
+        for (int j = 0; j < kMaxAsc; ++j)
+            if (j < a.n_asc) isum = isum + ia[j];
+        const float dv = (-(v - prm(a.vrest, a.vrest_s, nidx))) / tau + (I + isum) / cm;
+        v1 = v + prm(a.kv, a.kv_s, nidx) * dv;
+        // adaptation
+        const int64_t q = nidx * a.n_asc;
+#pragma unroll
+        for (int j = 0; j < kMaxAsc; ++j)
+            if (j < a.n_asc) ia[j] = ia[j] + a.asc_kk[q + j] * ((-a.asc_k[q + j]) * ia[j]);
+    } else {
+        const float dv = (-(v - vr)) / tau + I / cm;
+        v1 = v + a.dt * dv;
+    }
     if (a.rec_v_pre) a.rec_v_pre[i] = v1;
     const float u = (v1 - vth) / (vth - vr);
     float s = (u >= 0.f) ? 1.f : 0.f;
@@ -144,6 +254,12 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         if (a.rec_ref) a.rec_ref[i] = ref;
     }
     v = a.hard ? (v1 - (v1 - vr) * s) : (v1 - (vth - vr) * s);
+    if (a.glif) {
+        const int64_t q = nidx * a.n_asc;
+#pragma unroll
+        for (int j = 0; j < kMaxAsc; ++j)
+            if (j < a.n_asc) ia[j] = ia[j] + a.asc_amp[q + j] * s;
+    }
     if (a.rec_v) a.rec_v[i] = v;
     const bool live = a.lane_t_end == nullptr || a.in_t0[lane] + a.t < a.lane_t_end[lane];
     if (a.counts && s != 0.f && live) a.counts[i] += 1;
@@ -157,261 +273,9 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
     return s;
 }
 
-// Ascending list of one 1024-neuron segment: each thread of the NW-wave
-// workgroup holds the pushed bits `bits` of neurons seg_base + first + b
-// (bit b, b < 32).  Threads own increasing neuron ranges in thread order.
-// Every thread of the workgroup must call it.
-template <int NW>
-__device__ __forceinline__ void write_segment(uint32_t bits, int first, int64_t seg_base,
-                                              int32_t* __restrict__ idx,
-                                              int32_t* __restrict__ cnt) {
-    int total;
-    int pos = block_excl_scan<NW>(__popc(bits), total);
-    while (bits) {
-        const int b = __ffs(bits) - 1;
-        idx[seg_base + pos++] = (int32_t)(seg_base + first + b);
-        bits &= bits - 1;
-    }
-    if (threadIdx.x == 0) *cnt = total;
-}
-
-// Scalar form: one neuron per thread, 1024 neurons per workgroup, spike word
-// by wavefront ballot.
-__global__ __launch_bounds__(1024) void k_lif_exp_step(StepArgs a) {
-    const int64_t lane = blockIdx.y;
-    const int64_t seg_base = (int64_t)blockIdx.x * kChunk;
-    const int64_t nidx = seg_base + threadIdx.x;
-    int spk = 0;
-    if (nidx < a.n) {
-        const int64_t i = lane * a.n + nidx;
-        float psc = a.psc ? a.psc[i] : 0.f;
-        const float xin = a.x ? a.x[i] : 0.f;
-        float v = a.v[i];
-        float ref = a.use_ref ? a.ref[i] : 0.f;
-        const float s = lif_exp_neuron(a, lane, nidx, i, psc, xin, v, ref);
-        a.v[i] = v;
-        if (a.use_ref) a.ref[i] = ref;
-        if (a.psc) a.psc[i] = psc;
-        spk = s != 0.f;
-    }
-    const uint64_t m = __ballot(spk);
-    const int64_t word = nidx >> 6;
-    if (a.raster != nullptr && (threadIdx.x & 63) == 0 && word < a.nwords)
-        a.raster[lane * a.nwords + word] = m;
-    if (a.ring_idx != nullptr) {
-        uint32_t pushed = (uint32_t)spk;
-        if (a.teacher != nullptr)
-            pushed = nidx < a.n
-                         ? (uint32_t)((a.teacher[lane * a.nwords + word] >> (nidx & 63)) & 1ull)
-                         : 0u;
-        const uint64_t pw = a.teacher != nullptr ? __ballot(pushed != 0u) : m;
-        if ((threadIdx.x & 63) == 0 && word < a.nwords) a.ring_words[lane * a.nwords + word] = pw;
-        write_segment<16>(pushed, threadIdx.x, seg_base, a.ring_idx + lane * a.n,
-                          a.ring_cnt + lane * a.nchunks + blockIdx.x);
-    }
-}
-
-// Vector form (n % 4 == 0, 16-byte aligned state): four consecutive neurons
-// per thread with 16-byte loads/stores, 1024 neurons per workgroup; the
-// 64-bit spike word of 16 threads is assembled by an OR across them.
-__global__ __launch_bounds__(256) void k_lif_exp_step4(StepArgs a) {
-    const int64_t lane = blockIdx.y;
-    const int64_t seg_base = (int64_t)blockIdx.x * kChunk;
-    const int64_t n0 = seg_base + (int64_t)threadIdx.x * 4;
-    uint32_t nib = 0;
-    if (n0 < a.n) {
-        const int64_t i0 = lane * a.n + n0;
-        float4 v4 = *(const float4*)(a.v + i0);
-        float4 p4 = a.psc ? *(const float4*)(a.psc + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
-        float4 r4 = a.use_ref ? *(const float4*)(a.ref + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 x4 = a.x ? *(const float4*)(a.x + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float s0 = lif_exp_neuron(a, lane, n0, i0, p4.x, x4.x, v4.x, r4.x);
-        const float s1 = lif_exp_neuron(a, lane, n0 + 1, i0 + 1, p4.y, x4.y, v4.y, r4.y);
-        const float s2 = lif_exp_neuron(a, lane, n0 + 2, i0 + 2, p4.z, x4.z, v4.z, r4.z);
-        const float s3 = lif_exp_neuron(a, lane, n0 + 3, i0 + 3, p4.w, x4.w, v4.w, r4.w);
-        *(float4*)(a.v + i0) = v4;
-        if (a.use_ref) *(float4*)(a.ref + i0) = r4;
-        if (a.psc) *(float4*)(a.psc + i0) = p4;
-        nib = (uint32_t)(s0 != 0.f) | ((uint32_t)(s1 != 0.f) << 1) |
-              ((uint32_t)(s2 != 0.f) << 2) | ((uint32_t)(s3 != 0.f) << 3);
-    }
-    const int64_t wi = n0 >> 6;
-    uint64_t word = (uint64_t)nib << (4 * (threadIdx.x & 15));
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) word |= __shfl_xor(word, o, 64);
-    if (a.raster != nullptr && (threadIdx.x & 15) == 0 && wi < a.nwords)
-        a.raster[lane * a.nwords + wi] = word;
-    if (a.ring_idx != nullptr) {
-        uint32_t pushed = nib;
-        uint64_t pw = word;
-        if (a.teacher != nullptr) {
-            pw = wi < a.nwords ? a.teacher[lane * a.nwords + wi] : 0ull;
-            pushed = n0 < a.n ? (uint32_t)((pw >> (4 * (threadIdx.x & 15))) & 15ull) : 0u;
-        }
-        if ((threadIdx.x & 15) == 0 && wi < a.nwords) a.ring_words[lane * a.nwords + wi] = pw;
-        write_segment<4>(pushed, threadIdx.x * 4, seg_base, a.ring_idx + lane * a.n,
-                         a.ring_cnt + lane * a.nchunks + blockIdx.x);
-    }
-}
-
-// Ring lists of all ring slots from spike words [slots][lanes][nwords]
-// (grid: segments x lanes x slots; thread = 4 neurons of the segment).
-__global__ __launch_bounds__(256) void k_ring_from_words(const uint64_t* __restrict__ words,
-                                                         int64_t n, int64_t nwords,
-                                                         int64_t lanes, int64_t nchunks,
-                                                         int32_t* __restrict__ ring_idx,
-                                                         int32_t* __restrict__ ring_cnt) {
-    const int64_t seg = blockIdx.x, lane = blockIdx.y, slot = blockIdx.z;
-    const int64_t seg_base = seg * kChunk;
-    const int64_t n0 = seg_base + (int64_t)threadIdx.x * 4;
-    const int64_t wi = n0 >> 6;
-    uint32_t nib = 0;
-    if (n0 < n) {
-        const uint64_t w = words[(slot * lanes + lane) * nwords + wi];
-        nib = (uint32_t)((w >> (4 * (threadIdx.x & 15))) & 15ull);
-        const int64_t room = n - n0;  // bits past n never count
-        if (room < 4) nib &= (1u << room) - 1u;
-    }
-    write_segment<4>(nib, threadIdx.x * 4, seg_base, ring_idx + (slot * lanes + lane) * n,
-                     ring_cnt + (slot * lanes + lane) * nchunks + seg);
-}
-
 // ---------------------------------------------------------------------------
-// Push
+// Reduction of a push into one segment
 // ---------------------------------------------------------------------------
-
-struct PushStep {
-    const int64_t* row_ptr;
-    const int32_t* col;
-    const float* w;
-    int64_t n;
-    int64_t lanes;
-    int64_t nchunks;
-    int cpb;                     // segments per workgroup
-    int nbins;
-    int csr_d;
-    int ring_d;
-    int64_t clock;               // cursor + t: ring slot of this step's spikes
-    const Ctl* ctl;              // bin_delay / bin_sub tables
-    const int32_t* ring_idx;     // [ring_d][lanes][n]
-    const int32_t* ring_cnt;     // [ring_d][lanes][nchunks]
-    int ntiles;
-    int cap;
-    int* bin_cnt;                // [lanes][ntiles]
-    int32_t* ev_r;               // [lanes][ntiles][cap]
-    float* ev_w;
-    uint16_t* ev_d;
-    float* psc;                  // fast mode target
-};
-
-__device__ __forceinline__ int ring_slot(int64_t clock, int delay, int ring_d) {
-    const int64_t s = (clock - delay) % ring_d;
-    return (int)(s < 0 ? s + ring_d : s);
-}
-
-// Workgroup (x, lane) pushes segments [x*cpb, x*cpb + cpb) of every delay bin
-// of its lane; groups of G threads stream one active row each.
-template <int G, bool ORDERED>
-__global__ __launch_bounds__(kPushThreads) void k_push_step(PushStep a) {
-    __shared__ int s_off[kPushThreads + 1];
-    __shared__ int s_slot[BTX_MAX_BINS];
-    __shared__ int s_sub[BTX_MAX_BINS];
-    __shared__ int s_tile[ORDERED ? kMaxTiles : 1];
-    __shared__ int s_base[ORDERED ? kMaxTiles : 1];
-    const int64_t lane = blockIdx.y;
-    const int64_t c0 = (int64_t)blockIdx.x * a.cpb;
-    const int ncb = (int)(a.nchunks - c0 < a.cpb ? a.nchunks - c0 : a.cpb);
-    const int tid = threadIdx.x;
-    if (tid < a.nbins) {
-        s_slot[tid] = ring_slot(a.clock, a.ctl->bin_delay[tid], a.ring_d);
-        s_sub[tid] = a.ctl->bin_sub[tid];
-    }
-    if (ORDERED)
-        for (int t = tid; t < a.ntiles; t += kPushThreads) s_tile[t] = 0;
-    __syncthreads();
-    const int nseg = a.nbins * ncb;  // <= kPushThreads (host check)
-    int c = 0;
-    if (tid < nseg) {
-        const int j = tid / ncb, cc = tid - j * ncb;
-        c = a.ring_cnt[((int64_t)s_slot[j] * a.lanes + lane) * a.nchunks + c0 + cc];
-        // a segment never lists more neurons than it has
-        const int64_t room = a.n - (c0 + cc) * kChunk;
-        const int cap = room < kChunk ? (int)room : kChunk;
-        c = c < 0 ? 0 : (c > cap ? cap : c);
-    }
-    int total;
-    const int off = block_excl_scan<kPushThreads / 64>(c, total);
-    if (tid < nseg) s_off[tid] = off;
-    if (tid == 0) s_off[nseg] = total;
-    __syncthreads();
-    if (total == 0) return;
-
-    constexpr int gpb = kPushThreads / G;
-    const int gl = tid % G;
-    const int64_t key0 = lane * a.n;
-    for (int pass = 0; pass < (ORDERED ? 2 : 1); ++pass) {
-        for (int item = tid / G; item < total; item += gpb) {
-            int lo = 0, hi = nseg;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_off[mid] <= item) lo = mid;
-                else hi = mid;
-            }
-            const int j = lo / ncb, cc = lo - j * ncb;
-            const int k = item - s_off[lo];
-            const int src = a.ring_idx[((int64_t)s_slot[j] * a.lanes + lane) * a.n +
-                                       (c0 + cc) * kChunk + k];
-            if ((uint32_t)src >= (uint32_t)a.n) continue;  // never indexes past the CSR
-            const int r = src * a.csr_d + s_sub[j];
-            const int64_t beg = a.row_ptr[r], end = a.row_ptr[r + 1];
-            for (int64_t e = beg + gl; e < end; e += G) {
-                const int dst = a.col[e];
-                if (!ORDERED) {
-                    atomicAdd(&a.psc[key0 + dst], a.w[e]);
-                } else if (pass == 0) {
-                    atomicAdd(&s_tile[dst >> kTileShift], 1);
-                } else {
-                    const int tl = dst >> kTileShift;
-                    const int pos = s_base[tl] + atomicAdd(&s_tile[tl], 1);
-                    if (pos < a.cap) {
-                        const int64_t q = (lane * a.ntiles + tl) * (int64_t)a.cap + pos;
-                        a.ev_r[q] = r;
-                        a.ev_w[q] = a.w[e];
-                        a.ev_d[q] = (uint16_t)(dst & (kTile - 1));
-                    }
-                }
-            }
-        }
-        if (ORDERED && pass == 0) {
-            __syncthreads();
-            for (int t = tid; t < a.ntiles; t += kPushThreads) {
-                const int ct = s_tile[t];
-                if (ct) {
-                    s_base[t] = atomicAdd(&a.bin_cnt[lane * a.ntiles + t], ct);
-                    s_tile[t] = 0;
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Ordered reduction
-// ---------------------------------------------------------------------------
-
-struct PullArgs {
-    const int64_t* csc_ptr;
-    const int32_t* csc_src;
-    const float* csc_w;
-    const uint64_t* ring_words;  // [ring_d][lanes][nwords], ring-slot order
-    int64_t lanes;
-    int64_t nwords;
-    int ring_d;
-    int64_t clock;               // cursor + step of the push being reduced
-    int csr_d;
-    const int* age_of_sub;       // [csr_d] ring age by virtual-row offset, -1 = not read
-};
 
 // Virtual row r = src*csr_d + sub is active in the push being reduced when
 // neuron src spiked age_of_sub[sub] steps before it (ring slot clock - age).
@@ -419,17 +283,16 @@ __device__ __forceinline__ bool pulled_active(const PullArgs& a, int64_t lane, i
     const int src = r / a.csr_d;
     const int age = a.age_of_sub[r - src * a.csr_d];
     if (age < 0) return false;
-    int64_t slot = (a.clock - age) % a.ring_d;
-    if (slot < 0) slot += a.ring_d;
+    int64_t slot = (a.clock - age) % a.R;
+    if (slot < 0) slot += a.R;
     const uint64_t* row = a.ring_words + (slot * a.lanes + lane) * a.nwords;
     return (row[src >> 6] >> (src & 63)) & 1ull;
 }
 
-// Input of destination `dst` of `lane`: its active incoming edges summed in
-// the coalesced (dst, src) order from +0.0f.  Returns false when none is active.
-__device__ bool pull_sum(const PullArgs& a, int64_t lane, int64_t dst, float& out) {
+// Input of neuron `dst` of `lane`: its active incoming edges summed in the
+// coalesced (dst, src) order from +0.0f.
+__device__ float pull_sum(const PullArgs& a, int64_t lane, int64_t dst) {
     float acc = 0.f;
-    bool any = false;
     int64_t e = a.csc_ptr[dst];
     const int64_t e1 = a.csc_ptr[dst + 1];
     for (; e + 4 <= e1; e += 4) {
@@ -441,124 +304,356 @@ __device__ bool pull_sum(const PullArgs& a, int64_t lane, int64_t dst, float& ou
         if (b1) acc = acc + a.csc_w[e + 1];
         if (b2) acc = acc + a.csc_w[e + 2];
         if (b3) acc = acc + a.csc_w[e + 3];
-        any = any || b0 || b1 || b2 || b3;
     }
     for (; e < e1; ++e)
-        if (pulled_active(a, lane, a.csc_src[e])) {
-            acc = acc + a.csc_w[e];
-            any = true;
-        }
-    out = acc;
-    return any;
+        if (pulled_active(a, lane, a.csc_src[e])) acc = acc + a.csc_w[e];
+    return acc;
 }
 
-struct ReduceArgs {
-    int64_t n;
-    int ntiles;
-    int cap;
-    int sort_max;              // destinations with more terms pull
-    int* bin_cnt;
-    const int32_t* ev_r;
-    const float* ev_w;
-    const uint16_t* ev_d;
-    float* psc;
-    PullArgs pull;
-};
-
-// One workgroup per (tile, lane) bin.  Thread tid owns destinations
-// d = k*512 + tid (k < 8) of the tile.
-__global__ __launch_bounds__(kRedThreads) void k_reduce_bins(ReduceArgs a) {
-    __shared__ int s_cnt[kTile];
-    __shared__ int s_pos[kTile];
-    __shared__ int s_r[kBinCap];
-    __shared__ float s_w[kBinCap];
-    __shared__ uint16_t s_d[kBinCap];
-    __shared__ uint16_t s_ord[kBinCap];
-    constexpr int kPer = kTile / kRedThreads;
-    const int64_t lane = blockIdx.y;
-    const int tile = blockIdx.x;
-    const int64_t bin = lane * a.ntiles + tile;
+// s4[k] = summed input of neuron seg*1024 + tid*4 + k of `lane` (+0.0f when
+// it received none).  Every thread of the workgroup calls it.
+__device__ void reduce_segment(const ReduceIn& a, RedLds& L, int64_t n, int64_t nchunks,
+                               int64_t lane, int64_t seg, float s4[4]) {
     const int tid = threadIdx.x;
-    const int c = a.bin_cnt[bin];
-    if (c == 0) return;
-    const int64_t dst0 = (int64_t)tile * kTile;
-    const int ndst = (int)(a.n - dst0 < kTile ? a.n - dst0 : kTile);
-    float* pl = a.psc + lane * a.n + dst0;
-    for (int d = tid; d < kTile; d += kRedThreads) s_cnt[d] = 0;
-    __syncthreads();
-    if (tid == 0) a.bin_cnt[bin] = 0;  // every thread has read c
-    if (c > a.cap) {
-        // overflowed bin: every destination of the tile pulls
-        for (int d = tid; d < ndst; d += kRedThreads) {
-            float s;
-            if (pull_sum(a.pull, lane, dst0 + d, s)) pl[d] = pl[d] + s;
+    const int64_t n0 = seg * kChunk + tid * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s4[k] = 0.f;
+    if (!a.ordered) {
+        if (n0 < n) {
+            float* q = a.acc + lane * n + n0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (n0 + k < n) {
+                    s4[k] = q[k];
+                    if (s4[k] != 0.f) q[k] = 0.f;
+                }
+            }
         }
         return;
     }
-    const int64_t q0 = bin * (int64_t)a.cap;
-    for (int i = tid; i < c; i += kRedThreads) {
-        const int d = a.ev_d[q0 + i] & (kTile - 1);
-        s_d[i] = (uint16_t)d;
-        s_r[i] = a.ev_r[q0 + i];
-        s_w[i] = a.ev_w[q0 + i];
-        atomicAdd(&s_cnt[d], 1);
+    int* cntp = a.bin_cnt + lane * nchunks + seg;
+    const int c = *cntp;
+    if (c == 0) return;  // uniform across the workgroup
+#pragma unroll
+    for (int k = 0; k < 4; ++k) L.cnt[tid * 4 + k] = 0;
+    __syncthreads();
+    if (tid == 0) *cntp = 0;  // every thread has read c
+    if (c < 0 || c > a.cap) {
+        // overflowed bin: every neuron of the segment pulls
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (n0 + k < n) s4[k] = pull_sum(a.pull, lane, n0 + k);
+        return;
+    }
+    const int64_t q0 = (lane * nchunks + seg) * (int64_t)kBinCap;
+    for (int i = tid; i < c; i += kThreads) {
+        const int d = a.ev_d[q0 + i] & (kChunk - 1);
+        L.d[i] = (uint16_t)d;
+        L.r[i] = a.ev_r[q0 + i];
+        L.w[i] = a.ev_w[q0 + i];
+        atomicAdd(&L.cnt[d], 1);
     }
     __syncthreads();
-    int v[kPer];
+    int v[4];
     int sum = 0;
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        v[k] = s_cnt[k * kRedThreads + tid];
+    for (int k = 0; k < 4; ++k) {
+        v[k] = L.cnt[tid * 4 + k];
         sum += v[k];
     }
     int total;
-    int base = block_excl_scan<kRedThreads / 64>(sum, total);
+    int base = block_excl_scan<kThreads / 64>(sum, total);
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        s_pos[k * kRedThreads + tid] = base;
+    for (int k = 0; k < 4; ++k) {
+        L.pos[tid * 4 + k] = base;
         base += v[k];
     }
     __syncthreads();
-    for (int i = tid; i < c; i += kRedThreads) {
-        const int p = atomicAdd(&s_pos[s_d[i]], 1);
-        s_ord[p] = (uint16_t)i;
+    for (int i = tid; i < c; i += kThreads) {
+        const int p = atomicAdd(&L.pos[L.d[i]], 1);
+        L.ord[p] = (uint16_t)i;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int d = k * kRedThreads + tid;
+    for (int k = 0; k < 4; ++k) {
         const int cd = v[k];
-        if (cd == 0 || d >= ndst) continue;
-        float s = 0.f;
+        if (cd == 0 || n0 + k >= n) continue;
         if (cd <= a.sort_max) {
-            const int end = s_pos[d], start = end - cd;
-            // insertion sort of the destination's terms by virtual row
+            const int end = L.pos[tid * 4 + k], start = end - cd;
+            // insertion sort of the neuron's terms by virtual row
             for (int x = start + 1; x < end; ++x) {
-                const int key = s_ord[x];
-                const int rk = s_r[key];
+                const int key = L.ord[x];
+                const int rk = L.r[key];
                 int y = x - 1;
-                while (y >= start && s_r[s_ord[y]] > rk) {
-                    s_ord[y + 1] = s_ord[y];
+                while (y >= start && L.r[L.ord[y]] > rk) {
+                    L.ord[y + 1] = L.ord[y];
                     --y;
                 }
-                s_ord[y + 1] = (uint16_t)key;
+                L.ord[y + 1] = (uint16_t)key;
             }
-            for (int x = start; x < end; ++x) s = s + s_w[s_ord[x]];
+            float s = 0.f;
+            for (int x = start; x < end; ++x) s = s + L.w[L.ord[x]];
+            s4[k] = s;
         } else {
-            pull_sum(a.pull, lane, dst0 + d, s);
+            s4[k] = pull_sum(a.pull, lane, n0 + k);
         }
-        pl[d] = pl[d] + s;
     }
 }
 
-// psc_{T-1} normalised (psc + 0.0f) and recorded.
-__global__ __launch_bounds__(256) void k_finish_psc(float* __restrict__ psc, int64_t m,
-                                                    float* __restrict__ rec) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= m) return;
-    const float p = psc[i] + 0.f;
-    psc[i] = p;
-    if (rec) rec[i] = p;
+// ---------------------------------------------------------------------------
+// F role
+// ---------------------------------------------------------------------------
+
+// Ascending list of one segment: thread tid holds the pushed bits `bits` of
+// neurons seg_base + tid*4 + b.  Every thread of the workgroup calls it.
+__device__ __forceinline__ void write_segment(uint32_t bits, int64_t seg_base,
+                                              int32_t* __restrict__ idx,
+                                              int32_t* __restrict__ cnt) {
+    int total;
+    int pos = block_excl_scan<kThreads / 64>(__popc(bits), total);
+    while (bits) {
+        const int b = __ffs(bits) - 1;
+        idx[seg_base + pos++] = (int32_t)(seg_base + threadIdx.x * 4 + b);
+        bits &= bits - 1;
+    }
+    if (threadIdx.x == 0) *cnt = total;
+}
+
+__device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lane) {
+    const int tid = threadIdx.x;
+    const int64_t seg_base = seg * kChunk;
+    const int64_t n0 = seg_base + tid * 4;
+    const bool full = n0 + 4 <= a.n;
+    const int64_t i0 = lane * a.n + n0;
+    // state loads first, so they are in flight during the reduction
+    float v[4] = {0.f, 0.f, 0.f, 0.f}, p[4] = {0.f, 0.f, 0.f, 0.f};
+    float r[4] = {0.f, 0.f, 0.f, 0.f}, x[4] = {0.f, 0.f, 0.f, 0.f};
+    if (a.vec4 && full) {
+        const float4 v4 = *(const float4*)(a.v + i0);
+        v[0] = v4.x; v[1] = v4.y; v[2] = v4.z; v[3] = v4.w;
+        if (a.psc) {
+            const float4 q = *(const float4*)(a.psc + i0);
+            p[0] = q.x; p[1] = q.y; p[2] = q.z; p[3] = q.w;
+        }
+        if (a.use_ref) {
+            const float4 q = *(const float4*)(a.ref + i0);
+            r[0] = q.x; r[1] = q.y; r[2] = q.z; r[3] = q.w;
+        }
+        if (a.x) {
+            const float4 q = *(const float4*)(a.x + i0);
+            x[0] = q.x; x[1] = q.y; x[2] = q.z; x[3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (n0 + k < a.n) {
+                v[k] = a.v[i0 + k];
+                if (a.psc) p[k] = a.psc[i0 + k];
+                if (a.use_ref) r[k] = a.ref[i0 + k];
+                if (a.x) x[k] = a.x[i0 + k];
+            }
+        }
+    }
+    float ia[4][kMaxAsc];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < kMaxAsc; ++j)
+            ia[k][j] = (a.glif && j < a.n_asc && n0 + k < a.n)
+                           ? a.iasc[(i0 + k) * a.n_asc + j] : 0.f;
+    float wz[4] = {0.f, 0.f, 0.f, 0.f};
+    if (a.prev) reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz);
+    uint32_t nib = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (n0 + k < a.n) {
+            const float s = lif_exp_neuron(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], v[k], r[k],
+                                           ia[k]);
+            nib |= (uint32_t)(s != 0.f) << k;
+        }
+    }
+    if (a.glif) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int j = 0; j < kMaxAsc; ++j)
+                if (j < a.n_asc && n0 + k < a.n) a.iasc[(i0 + k) * a.n_asc + j] = ia[k][j];
+    }
+    if (a.vec4 && full) {
+        *(float4*)(a.v + i0) = make_float4(v[0], v[1], v[2], v[3]);
+        if (a.use_ref) *(float4*)(a.ref + i0) = make_float4(r[0], r[1], r[2], r[3]);
+        if (a.psc) *(float4*)(a.psc + i0) = make_float4(p[0], p[1], p[2], p[3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (n0 + k < a.n) {
+                a.v[i0 + k] = v[k];
+                if (a.use_ref) a.ref[i0 + k] = r[k];
+                if (a.psc) a.psc[i0 + k] = p[k];
+            }
+        }
+    }
+    // spike word of 16 threads (64 neurons), by an OR across them
+    const int64_t wi = n0 >> 6;
+    uint64_t word = (uint64_t)nib << (4 * (tid & 15));
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) word |= __shfl_xor(word, o, 64);
+    const bool lead = (tid & 15) == 0 && wi < a.nwords;
+    if (a.raster != nullptr && lead) a.raster[lane * a.nwords + wi] = word;
+    if (a.ring_idx != nullptr) {
+        uint32_t pushed = nib;
+        uint64_t pw = word;
+        if (a.teacher != nullptr) {
+            pw = wi < a.nwords ? a.teacher[lane * a.nwords + wi] : 0ull;
+            pushed = (uint32_t)((pw >> (4 * (tid & 15))) & 15ull);
+            const int64_t room = a.n - n0;  // bits past n never count
+            if (room < 4) pushed &= room > 0 ? (1u << room) - 1u : 0u;
+        }
+        if (lead) a.ring_words[lane * a.nwords + wi] = pw;
+        write_segment(pushed, seg_base, a.ring_idx + lane * a.n,
+                      a.ring_cnt + lane * a.nchunks + seg);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P role
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int ring_slot(int64_t clock, int delay, int R) {
+    const int64_t s = (clock - delay) % R;
+    return (int)(s < 0 ? s + R : s);
+}
+
+template <int G, bool ORDERED>
+__device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) {
+    const int64_t c0 = px * a.cpb;
+    const int ncb = (int)(a.nchunks - c0 < a.cpb ? a.nchunks - c0 : a.cpb);
+    const int tid = threadIdx.x;
+    if (tid < a.nbins) {
+        L.slot[tid] = ring_slot(a.clock, a.ctl->bin_delay[tid], a.R);
+        L.sub[tid] = a.ctl->bin_sub[tid];
+    }
+    if (ORDERED)
+        for (int t = tid; t < a.nchunks; t += kThreads) L.tile[t] = 0;
+    __syncthreads();
+    const int nseg = a.nbins * ncb;  // <= kThreads (host check)
+    int c = 0;
+    if (tid < nseg) {
+        const int j = tid / ncb, cc = tid - j * ncb;
+        c = a.ring_cnt[((int64_t)L.slot[j] * a.lanes + lane) * a.nchunks + c0 + cc];
+        // a segment never lists more neurons than it has
+        const int64_t room = a.n - (c0 + cc) * kChunk;
+        const int cap = room < kChunk ? (int)room : kChunk;
+        c = c < 0 ? 0 : (c > cap ? cap : c);
+    }
+    int total;
+    const int off = block_excl_scan<kThreads / 64>(c, total);
+    if (tid < nseg) L.off[tid] = off;
+    if (tid == 0) L.off[nseg] = total;
+    __syncthreads();
+    if (total == 0) return;
+
+    constexpr int gpb = kThreads / G;
+    const int gl = tid % G;
+    const int64_t key0 = lane * a.n;
+    for (int pass = 0; pass < (ORDERED ? 2 : 1); ++pass) {
+        for (int item = tid / G; item < total; item += gpb) {
+            int lo = 0, hi = nseg;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (L.off[mid] <= item) lo = mid;
+                else hi = mid;
+            }
+            const int j = lo / ncb, cc = lo - j * ncb;
+            const int k = item - L.off[lo];
+            const int src = a.ring_idx[((int64_t)L.slot[j] * a.lanes + lane) * a.n +
+                                       (c0 + cc) * kChunk + k];
+            if ((uint32_t)src >= (uint32_t)a.n) continue;  // never indexes past the CSR
+            const int r = src * a.csr_d + L.sub[j];
+            const int64_t beg = a.row_ptr[r], end = a.row_ptr[r + 1];
+            for (int64_t e = beg + gl; e < end; e += G) {
+                const int dst = a.col[e];
+                if (!ORDERED) {
+                    atomicAdd(&a.acc[key0 + dst], a.w[e]);
+                } else if (pass == 0) {
+                    atomicAdd(&L.tile[dst >> kChunkShift], 1);
+                } else {
+                    const int tl = dst >> kChunkShift;
+                    const int pos = L.base[tl] + atomicAdd(&L.tile[tl], 1);
+                    if (pos < a.cap) {
+                        const int64_t q = (lane * a.nchunks + tl) * (int64_t)kBinCap + pos;
+                        a.ev_r[q] = r;
+                        a.ev_w[q] = a.w[e];
+                        a.ev_d[q] = (uint16_t)(dst & (kChunk - 1));
+                    }
+                }
+            }
+        }
+        if (ORDERED && pass == 0) {
+            __syncthreads();
+            for (int t = tid; t < a.nchunks; t += kThreads) {
+                const int ct = L.tile[t];
+                if (ct) {
+                    L.base[t] = atomicAdd(&a.bin_cnt[lane * a.nchunks + t], ct);
+                    L.tile[t] = 0;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Workgroups [0, nf) of a row play F, [nf, gridDim.x) play P.
+template <int G, bool ORDERED>
+__global__ __launch_bounds__(kThreads) void k_step(StepArgs a, PushStep p, int nf) {
+    __shared__ StepLds lds;
+    const int64_t lane = blockIdx.y;
+    if ((int)blockIdx.x < nf) f_role(a, lds, blockIdx.x, lane);
+    else p_role<G, ORDERED>(p, lds.push, (int64_t)blockIdx.x - nf, lane);
+}
+
+// FIN: the last push reduced into psc (and recorded).
+__global__ __launch_bounds__(kThreads) void k_finish(StepArgs a, float* __restrict__ rec) {
+    __shared__ StepLds lds;
+    const int64_t lane = blockIdx.y, seg = blockIdx.x;
+    float wz[4];
+    reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz);
+    const int64_t n0 = seg * kChunk + threadIdx.x * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (n0 + k < a.n) {
+            const int64_t i = lane * a.n + n0 + k;
+            const float q = a.psc[i] + wz[k];
+            a.psc[i] = q;
+            if (rec) rec[i] = q;
+        }
+    }
+}
+
+// Ring lists of all ring slots from spike words [R][lanes][nwords]
+// (grid: segments x lanes x slots).
+__global__ __launch_bounds__(kThreads) void k_ring_from_words(const uint64_t* __restrict__ words,
+                                                              int64_t n, int64_t nwords,
+                                                              int64_t lanes, int64_t nchunks,
+                                                              int32_t* __restrict__ ring_idx,
+                                                              int32_t* __restrict__ ring_cnt) {
+    const int64_t seg = blockIdx.x, lane = blockIdx.y, slot = blockIdx.z;
+    const int64_t seg_base = seg * kChunk;
+    const int64_t n0 = seg_base + (int64_t)threadIdx.x * 4;
+    const int64_t wi = n0 >> 6;
+    uint32_t nib = 0;
+    if (n0 < n) {
+        const uint64_t w = words[(slot * lanes + lane) * nwords + wi];
+        nib = (uint32_t)((w >> (4 * (threadIdx.x & 15))) & 15ull);
+        const int64_t room = n - n0;  // bits past n never count
+        if (room < 4) nib &= (1u << room) - 1u;
+    }
+    write_segment(nib, seg_base, ring_idx + (slot * lanes + lane) * n,
+                  ring_cnt + (slot * lanes + lane) * nchunks + seg);
 }
 
 // Standalone exponential-synapse decay psc += k*((-psc)/tau_s) (ode.py:50).
@@ -598,26 +693,32 @@ static inline int64_t mod(int64_t a, int64_t b) {
     return r < 0 ? r + b : r;
 }
 
-// Engine workspace: control tables, then (ordered) bin counters and terms.
+// Engine workspace: control tables, then two buffers of term bins (ordered)
+// or of accumulators (fast).
 struct EngineWs {
-    size_t ctl = 0, bin_cnt = 0, ev_r = 0, ev_w = 0, ev_d = 0, total = 0;
-    int64_t ntiles = 0;
+    size_t bin_cnt = 0, ev_r = 0, ev_w = 0, ev_d = 0, acc = 0, total = 0;
+    int64_t bins = 0;  // per buffer
+    int64_t m = 0;     // accumulators per buffer
 };
 
 static EngineWs engine_ws(int64_t lanes, int64_t n, int ordered) {
     EngineWs L;
-    L.ntiles = (n + kTile - 1) / kTile;
+    const int64_t nchunks = (n + kChunk - 1) / kChunk;
     size_t off = align256(sizeof(Ctl));
     if (ordered) {
-        const int64_t bins = lanes * L.ntiles;
+        L.bins = lanes * nchunks;
         L.bin_cnt = off;
-        off = align256(off + sizeof(int) * bins);
+        off = align256(off + sizeof(int) * 2 * L.bins);
         L.ev_r = off;
-        off = align256(off + sizeof(int32_t) * bins * kBinCap);
+        off = align256(off + sizeof(int32_t) * 2 * L.bins * kBinCap);
         L.ev_w = off;
-        off = align256(off + sizeof(float) * bins * kBinCap);
+        off = align256(off + sizeof(float) * 2 * L.bins * kBinCap);
         L.ev_d = off;
-        off = align256(off + sizeof(uint16_t) * bins * kBinCap);
+        off = align256(off + sizeof(uint16_t) * 2 * L.bins * kBinCap);
+    } else {
+        L.m = lanes * n;
+        L.acc = off;
+        off = align256(off + sizeof(float) * 2 * L.m);
     }
     L.total = off;
     return L;
@@ -644,16 +745,34 @@ static StepArgs neuron_args(const btx_rsnn_plan* p) {
     a.ksyn = p->k_syn; a.ksyn_s = p->k_syn_stride;
     a.v = p->v;
     a.ref = p->refractory;
+    a.glif = p->neuron_model == BTX_NEURON_GLIF3;
+    if (a.glif) {
+        a.n_asc = p->n_asc;
+        a.vrest = p->v_rest; a.vrest_s = p->v_rest_stride;
+        a.kv = p->k_v; a.kv_s = p->k_v_stride;
+        a.asc_k = p->asc_k;
+        a.asc_kk = p->asc_kk;
+        a.asc_amp = p->asc_amp;
+        a.iasc = p->iasc;
+    }
     return a;
 }
 
 static int check_neuron(const btx_rsnn_plan* p) {
-    if (p->neuron_model != BTX_NEURON_LIF) return fail("unsupported neuron model");
+    if (p->neuron_model != BTX_NEURON_LIF && p->neuron_model != BTX_NEURON_GLIF3)
+        return fail("unsupported neuron model");
+    if (p->neuron_model == BTX_NEURON_GLIF3 &&
+        (p->n_asc < 1 || p->n_asc > kMaxAsc || p->v_rest == nullptr || p->k_v == nullptr ||
+         p->asc_k == nullptr || p->asc_kk == nullptr || p->asc_amp == nullptr ||
+         p->iasc == nullptr))
+        return fail("GLIF3 needs 1..4 after-spike currents, their parameters and state");
     if (p->lanes > 65535) return fail("lanes exceeds 65535");
     if (p->use_refractory && (p->refractory == nullptr || p->tau_ref == nullptr))
         return fail("refractory state missing");
     return 0;
 }
+
+static bool aligned16(const void* q) { return ((uintptr_t)q & 15) == 0; }
 
 BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float* v_pre,
                          void* stream) {
@@ -661,12 +780,15 @@ BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float
     if (p->lanes <= 0 || p->n <= 0) return 0;
     if (int rc = check_neuron(p)) return rc;
     StepArgs a = neuron_args(p);
-    a.first = 1;
     a.x = x;
     a.rec_z = z;
     a.rec_v_pre = v_pre;
+    a.vec4 = (p->n % 4 == 0) && aligned16(p->v) &&
+             (!p->use_refractory || aligned16(p->refractory)) && (!x || aligned16(x));
+    PushStep none{};
     const dim3 grid((unsigned)a.nchunks, (unsigned)p->lanes);
-    hipLaunchKernelGGL(k_lif_exp_step, grid, dim3(kChunk), 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL((k_step<16, true>), grid, dim3(kThreads), 0, (hipStream_t)stream, a,
+                       none, (int)a.nchunks);
     return check_launch("btx_lif_step");
 }
 
@@ -731,12 +853,13 @@ struct KernelTimer {
 static int rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s, KernelTimer& kt);
 
 template <bool ORDERED>
-static void launch_push_step(const PushStep& ps, int group, dim3 grid, hipStream_t s) {
+static void launch_step(const StepArgs& a, const PushStep& ps, int nf, int group, dim3 grid,
+                        hipStream_t s) {
     switch (group) {
-        case 8: hipLaunchKernelGGL((k_push_step<8, ORDERED>), grid, dim3(kPushThreads), 0, s, ps); break;
-        case 16: hipLaunchKernelGGL((k_push_step<16, ORDERED>), grid, dim3(kPushThreads), 0, s, ps); break;
-        case 32: hipLaunchKernelGGL((k_push_step<32, ORDERED>), grid, dim3(kPushThreads), 0, s, ps); break;
-        default: hipLaunchKernelGGL((k_push_step<64, ORDERED>), grid, dim3(kPushThreads), 0, s, ps); break;
+        case 8: hipLaunchKernelGGL((k_step<8, ORDERED>), grid, dim3(kThreads), 0, s, a, ps, nf); break;
+        case 16: hipLaunchKernelGGL((k_step<16, ORDERED>), grid, dim3(kThreads), 0, s, a, ps, nf); break;
+        case 32: hipLaunchKernelGGL((k_step<32, ORDERED>), grid, dim3(kThreads), 0, s, a, ps, nf); break;
+        default: hipLaunchKernelGGL((k_step<64, ORDERED>), grid, dim3(kThreads), 0, s, a, ps, nf); break;
     }
 }
 
@@ -765,38 +888,41 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     if (int rc = check_neuron(p)) return rc;
     if (p->psc == nullptr || p->row_ptr == nullptr || p->raster == nullptr)
         return fail("btx_rsnn_run: psc, connectivity and raster are required");
-    if (p->ring_idx == nullptr || p->ring_cnt == nullptr)
-        return fail("btx_rsnn_run: the ring lists are required");
+    if (p->ring_idx == nullptr || p->ring_cnt == nullptr || p->ring_words == nullptr)
+        return fail("btx_rsnn_run: the ring lists and words are required");
     if (p->n * (int64_t)p->csr_d >= ((int64_t)1 << 31))
         return fail("btx_rsnn_run: n*csr_d exceeds int32 row ids");
     if (p->nbins < 1 || p->nbins > BTX_MAX_BINS) return fail("btx_rsnn_run: bad nbins");
-    if (p->ring_d < 1) return fail("btx_rsnn_run: ring_d must be >= 1");
+    if (p->ring_d < 1 || p->ring_d >= BTX_MAX_BINS)
+        return fail("btx_rsnn_run: ring_d must be in [1, 63]");
     if (p->csr_d < 1 || p->csr_d > BTX_MAX_BINS)
         return fail("btx_rsnn_run: csr_d must be in [1, 64]");
+    int min_delay = p->ring_d;
     for (int j = 0; j < p->nbins; ++j) {
         if (p->bin_delay[j] < 0 || p->bin_delay[j] >= p->ring_d)
             return fail("btx_rsnn_run: bin delay outside the ring");
         if (p->bin_sub[j] < 0 || p->bin_sub[j] >= p->csr_d)
             return fail("btx_rsnn_run: bin offset outside csr_d");
+        if (p->bin_delay[j] < min_delay) min_delay = p->bin_delay[j];
     }
     const int64_t n = p->n, lanes = p->lanes, m = n * lanes;
     const int64_t nwords = (n + 63) / 64;
     const int64_t nchunks = (n + kChunk - 1) / kChunk;
-    const EngineWs L = engine_ws(lanes, n, p->ordered);
-    if (L.ntiles > kMaxTiles) return fail("btx_rsnn_run: n exceeds the tile table");
-    if (p->ring_words == nullptr) return fail("btx_rsnn_run: the ring words are required");
+    const int R = p->ring_d + 1;
+    if (nchunks > kMaxChunks) return fail("btx_rsnn_run: n exceeds 4096 ring segments");
     if (p->ordered && (p->csc_ptr == nullptr || p->csc_src == nullptr || p->csc_w == nullptr))
         return fail("btx_rsnn_run: ordered mode needs the dst-major arrays");
     if (p->ordered && (p->slots < 1 || p->slots > kSortMax))
         return fail("btx_rsnn_run: ordered mode needs 1 <= slots <= 32");
     if (p->bin_cap < 0 || p->bin_cap > kBinCap)
-        return fail("btx_rsnn_run: bin_cap must be in [0, 6144]");
+        return fail("btx_rsnn_run: bin_cap must be in [0, 2048]");
     if (p->in_prob != nullptr && (p->in_seed == nullptr || p->in_t0 == nullptr))
         return fail("btx_rsnn_run: Poisson drive needs per-lane seeds and clocks");
     if (p->lane_t_end != nullptr && p->in_t0 == nullptr)
         return fail("btx_rsnn_run: lane_t_end needs per-lane clocks");
     if (p->rec_sample != nullptr && (p->n_sample < 1 || p->n_sample > p->n))
         return fail("btx_rsnn_run: n_sample must be in [1, n]");
+    const EngineWs L = engine_ws(lanes, n, p->ordered);
     if (p->workspace == nullptr || p->workspace_bytes < L.total)
         return fail("btx_rsnn_run: workspace too small");
     char* ws = (char*)p->workspace;
@@ -816,10 +942,25 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     hipLaunchKernelGGL(k_set_tables, dim3(1), dim3(64), 0, s, tb, ctl);
     if (p->ring_rebuild)
         hipLaunchKernelGGL(k_ring_from_words,
-                           dim3((unsigned)nchunks, (unsigned)lanes, (unsigned)p->ring_d),
-                           dim3(256), 0, s, p->ring_words, n, nwords, lanes, nchunks,
+                           dim3((unsigned)nchunks, (unsigned)lanes, (unsigned)R),
+                           dim3(kThreads), 0, s, p->ring_words, n, nwords, lanes, nchunks,
                            p->ring_idx, p->ring_cnt);
 
+    const int cap = p->bin_cap > 0 ? p->bin_cap : kBinCap;
+    // buffer b of the term bins / accumulators
+    auto bins_of = [&](int b, ReduceIn& r) {
+        r.ordered = p->ordered;
+        r.cap = cap;
+        r.sort_max = p->slots;
+        if (p->ordered) {
+            r.bin_cnt = (int*)(ws + L.bin_cnt) + b * L.bins;
+            r.ev_r = (const int32_t*)(ws + L.ev_r) + b * L.bins * kBinCap;
+            r.ev_w = (const float*)(ws + L.ev_w) + b * L.bins * kBinCap;
+            r.ev_d = (const uint16_t*)(ws + L.ev_d) + b * L.bins * kBinCap;
+        } else {
+            r.acc = (float*)(ws + L.acc) + b * L.m;
+        }
+    };
     StepArgs a = neuron_args(p);
     a.psc = p->psc;
     a.in_seed = p->in_seed;
@@ -829,6 +970,17 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     a.counts = p->counts;
     a.n_sample = p->n_sample;
     a.lane_t_end = p->lane_t_end;
+    a.vec4 = (n % 4 == 0) && aligned16(p->v) && aligned16(p->psc) &&
+             (!p->use_refractory || aligned16(p->refractory)) && (!p->x || aligned16(p->x));
+    a.red.pull.csc_ptr = p->csc_ptr;
+    a.red.pull.csc_src = p->csc_src;
+    a.red.pull.csc_w = p->csc_w;
+    a.red.pull.ring_words = p->ring_words;
+    a.red.pull.lanes = lanes;
+    a.red.pull.nwords = nwords;
+    a.red.pull.R = R;
+    a.red.pull.csr_d = p->csr_d;
+    a.red.pull.age_of_sub = ctl->age_of_sub;
 
     PushStep ps{};
     ps.row_ptr = p->row_ptr;
@@ -839,61 +991,36 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ps.nchunks = nchunks;
     ps.nbins = p->nbins;
     ps.csr_d = p->csr_d;
-    ps.ring_d = p->ring_d;
+    ps.R = R;
     ps.ctl = ctl;
     ps.ring_idx = p->ring_idx;
     ps.ring_cnt = p->ring_cnt;
-    ps.ntiles = (int)L.ntiles;
-    ps.cap = p->bin_cap > 0 ? p->bin_cap : kBinCap;
-    ps.bin_cnt = p->ordered ? (int*)(ws + L.bin_cnt) : nullptr;
-    ps.ev_r = p->ordered ? (int32_t*)(ws + L.ev_r) : nullptr;
-    ps.ev_w = p->ordered ? (float*)(ws + L.ev_w) : nullptr;
-    ps.ev_d = p->ordered ? (uint16_t*)(ws + L.ev_d) : nullptr;
-    ps.psc = p->psc;
-    // segments per push workgroup: about 64 workgroups per lane, at most
-    // kPushThreads segments (delay bins x segments) each
+    ps.cap = cap;
+    // segments per P workgroup: about 64 workgroups per lane, at most
+    // kThreads segments (delay bins x segments) each
     int cpb = (int)(nchunks / 64);
     if (cpb < 1) cpb = 1;
-    if (cpb * p->nbins > kPushThreads) cpb = kPushThreads / p->nbins;
+    if (cpb * p->nbins > kThreads) cpb = kThreads / p->nbins;
     ps.cpb = cpb;
-    const dim3 pgrid((unsigned)((nchunks + cpb - 1) / cpb), (unsigned)lanes);
+    const int npb = (int)((nchunks + cpb - 1) / cpb);
     const int group = pick_group((double)p->nnz / (double)(n * p->csr_d));
+    // one launch per step when no bin reads the step's own spikes
+    const bool fused = min_delay >= 1;
+    const int nf = (int)nchunks;
 
-    ReduceArgs ra{};
-    ra.n = n;
-    ra.ntiles = (int)L.ntiles;
-    ra.cap = ps.cap;
-    ra.sort_max = p->slots;
-    ra.bin_cnt = ps.bin_cnt;
-    ra.ev_r = ps.ev_r;
-    ra.ev_w = ps.ev_w;
-    ra.ev_d = ps.ev_d;
-    ra.psc = p->psc;
-    ra.pull.csc_ptr = p->csc_ptr;
-    ra.pull.csc_src = p->csc_src;
-    ra.pull.csc_w = p->csc_w;
-    ra.pull.ring_words = p->ring_words;
-    ra.pull.lanes = lanes;
-    ra.pull.nwords = nwords;
-    ra.pull.ring_d = p->ring_d;
-    ra.pull.csr_d = p->csr_d;
-    ra.pull.age_of_sub = ctl->age_of_sub;
-    const dim3 rgrid((unsigned)L.ntiles, (unsigned)lanes);
-
-    // the vector step needs n % 4 == 0 and 16-byte aligned state arrays
-    auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
-    const bool vec4 = (n % 4 == 0) && al16(p->v) && al16(p->psc) &&
-                      (!p->use_refractory || al16(p->refractory)) && (!p->x || al16(p->x));
-    const dim3 sgrid((unsigned)nchunks, (unsigned)lanes);
     const int64_t rec_step = m;
     for (int64_t t = 0; t < n_steps; ++t) {
-        a.first = (t == 0);
+        a.prev = t > 0;
+        if (a.prev) {
+            bins_of((int)((t - 1) & 1), a.red);
+            a.red.pull.clock = p->ring_cursor + t - 1;
+        }
         a.x = p->x ? p->x + t * p->x_step_stride : nullptr;
         a.t = t;
         a.rec_sample = p->rec_sample ? p->rec_sample + t * lanes * p->n_sample : nullptr;
         a.raster = p->raster + t * lanes * nwords;
         a.teacher = p->teacher ? p->teacher + t * lanes * nwords : nullptr;
-        const int64_t slot = mod(p->ring_cursor + t, p->ring_d);
+        const int64_t slot = mod(p->ring_cursor + t, R);
         a.ring_idx = p->ring_idx + slot * lanes * n;
         a.ring_cnt = p->ring_cnt + slot * lanes * nchunks;
         a.ring_words = p->ring_words + slot * lanes * nwords;
@@ -903,25 +1030,38 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         a.rec_v_pre = p->rec_v_pre ? p->rec_v_pre + t * rec_step : nullptr;
         a.rec_v_prev = p->rec_v_prev ? p->rec_v_prev + t * rec_step : nullptr;
         a.rec_psc_prev = (p->rec_psc && t > 0) ? p->rec_psc + (t - 1) * rec_step : nullptr;
-        kt.begin(0);
-        if (vec4) hipLaunchKernelGGL(k_lif_exp_step4, sgrid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_lif_exp_step, sgrid, dim3(kChunk), 0, s, a);
-        kt.end(0);
         ps.clock = p->ring_cursor + t;
-        kt.begin(2);
-        if (p->ordered) launch_push_step<true>(ps, group, pgrid, s);
-        else launch_push_step<false>(ps, group, pgrid, s);
-        kt.end(2);
-        if (p->ordered) {
-            ra.pull.clock = p->ring_cursor + t;
-            kt.begin(1);
-            hipLaunchKernelGGL(k_reduce_bins, rgrid, dim3(kRedThreads), 0, s, ra);
-            kt.end(1);
+        ReduceIn cur{};
+        bins_of((int)(t & 1), cur);
+        ps.bin_cnt = cur.bin_cnt;
+        ps.ev_r = (int32_t*)cur.ev_r;
+        ps.ev_w = (float*)cur.ev_w;
+        ps.ev_d = (uint16_t*)cur.ev_d;
+        ps.acc = cur.acc;
+        if (fused) {
+            kt.begin(0);
+            const dim3 grid((unsigned)(nf + npb), (unsigned)lanes);
+            if (p->ordered) launch_step<true>(a, ps, nf, group, grid, s);
+            else launch_step<false>(a, ps, nf, group, grid, s);
+            kt.end(0);
+        } else {
+            kt.begin(0);
+            const dim3 fgrid((unsigned)nf, (unsigned)lanes);
+            if (p->ordered) launch_step<true>(a, ps, nf, group, fgrid, s);
+            else launch_step<false>(a, ps, nf, group, fgrid, s);
+            kt.end(0);
+            kt.begin(2);
+            const dim3 pgrid((unsigned)npb, (unsigned)lanes);
+            if (p->ordered) launch_step<true>(a, ps, 0, group, pgrid, s);
+            else launch_step<false>(a, ps, 0, group, pgrid, s);
+            kt.end(2);
         }
     }
+    bins_of((int)((n_steps - 1) & 1), a.red);
+    a.red.pull.clock = p->ring_cursor + n_steps - 1;
     kt.begin(3);
-    hipLaunchKernelGGL(k_finish_psc, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
-                       p->psc, m, p->rec_psc ? p->rec_psc + (n_steps - 1) * rec_step : nullptr);
+    hipLaunchKernelGGL(k_finish, dim3((unsigned)nchunks, (unsigned)lanes), dim3(kThreads), 0, s,
+                       a, p->rec_psc ? p->rec_psc + (n_steps - 1) * rec_step : nullptr);
     kt.end(3);
     return check_launch("btx_rsnn_run");
 }

@@ -50,6 +50,8 @@ def _surrogate_code(fn, require_spiking: bool = True) -> tuple[int, float, float
 
 def _check_supported(spec, lanes):
     neuron = spec.neuron
+    if hasattr(neuron, "Iasc"):
+        raise NotImplementedError("the fused adjoint covers LIF neurons (GLIF3 is forward-only)")
     if spec.history is not None or spec.csr_d != 1 or spec.bins != [(0, 0)]:
         raise NotImplementedError("the fused adjoint supports synapses without delays")
     for name in ("v_threshold", "v_reset", "c_m", "tau_ref"):

@@ -4,7 +4,7 @@ from C, with no per-step Python and no (B, N*D) history matrix.
 
 Supported cells (everything else runs the generic per-step loop):
   neuron   LIF / IF (soft or hard reset, optional refractory, scalar or
-           per-neuron parameters)
+           per-neuron parameters); GLIF3 with 1..4 after-spike currents
   synapse  ExponentialPSC(linear=SparseConn(N, N))
            DelayedPSC(ExponentialPSC(SparseConn(N, N)), max_delay_steps=m)
            HeterSynapsePSC(base_psc=ExponentialPSC, n_receptor=1,
@@ -27,13 +27,14 @@ from .. import _lib
 from . import _plan
 from .history import SpikeHistory
 from .linear import BaseSparseConn
+from .neurons.glif import GLIF3
 from .neurons.lif import LIF
 from .synapse import DelayedPSC, ExponentialPSC, HeterSynapsePSC
 
 
 @dataclass
 class CellSpec:
-    neuron: LIF
+    neuron: LIF | GLIF3
     exp_psc: ExponentialPSC          # module whose psc decays and receives W z
     conn: BaseSparseConn
     history: SpikeHistory | None
@@ -53,7 +54,9 @@ def _bin_nonempty(conn: BaseSparseConn, csr_d: int) -> list[bool]:
 def match_cell(rnn) -> CellSpec | None:
     """CellSpec when ``rnn`` is a cell the engine runs, else None."""
     neuron, syn = getattr(rnn, "neuron", None), getattr(rnn, "synapse", None)
-    if type(neuron) not in (LIF,) and not isinstance(neuron, LIF):
+    if not isinstance(neuron, (LIF, GLIF3)):
+        return None
+    if isinstance(neuron, GLIF3) and not 1 <= neuron.n_Iasc <= 4:
         return None
     if getattr(rnn, "syn_inp_module", None) is not None:
         return None
@@ -188,11 +191,13 @@ class Engine:
 
     @staticmethod
     def _ring_buffers(ring_d, lanes, n, dev):
+        """Device ring of ring_d + 1 slots (lists, counts per segment, words)."""
+        R = ring_d + 1
         nwords = (n + 63) // 64
         nchunks = (n + _lib.RING_CHUNK - 1) // _lib.RING_CHUNK
-        return (torch.empty((ring_d, lanes, n), dtype=torch.int32, device=dev),
-                torch.zeros((ring_d, lanes, nchunks), dtype=torch.int32, device=dev),
-                torch.zeros((ring_d, lanes, nwords), dtype=torch.int64, device=dev))
+        return (torch.empty((R, lanes, n), dtype=torch.int32, device=dev),
+                torch.zeros((R, lanes, nchunks), dtype=torch.int32, device=dev),
+                torch.zeros((R, lanes, nwords), dtype=torch.int64, device=dev))
 
     @staticmethod
     def _set_outputs(p, T, lanes, n, drive, counts, sample):
@@ -240,7 +245,10 @@ class Engine:
                 raise ValueError("engine input must be contiguous float32 (T, lanes, n)")
             p.x = x.data_ptr()
             p.x_step_stride = lanes * n
-        # delay ring: lists rebuilt on the device from the history memory's words
+        # delay ring (ring_d + 1 slots): the history memory's spike words go
+        # into slots 0..ring_d-1 (slot ring_d - k holds the step k steps
+        # before the run) and the run starts at slot ring_d; the lists are
+        # rebuilt on the device from the words
         ring_d = sp.ring_d
         ring_idx, ring_cnt, ring_words = self._ring_buffers(ring_d, lanes, n, dev)
         cursor = 0
@@ -252,7 +260,11 @@ class Engine:
             h = h.reshape(lanes, ring_d, n).to(torch.float32).contiguous()
             keep.append(h)
             cursor = hist.cursor
-            order = (list(range(ring_d)) if hist.use_circular_buffer
+            # circular: module slot of delay d is (cursor - 1 - d) mod D, so
+            # device slot j = ring_d - 1 - d maps to module slot (cursor + j) mod D;
+            # cat: module index d is delay d
+            order = ([(cursor + j) % ring_d for j in range(ring_d)]
+                     if hist.use_circular_buffer
                      else [ring_d - 1 - j for j in range(ring_d)])
             from .._kernels import pack_rows
             for j in range(ring_d):
@@ -261,7 +273,7 @@ class Engine:
         p.ring_idx = ring_idx.data_ptr()
         p.ring_cnt = ring_cnt.data_ptr()
         p.ring_words = ring_words.data_ptr()
-        p.ring_cursor = cursor
+        p.ring_cursor = ring_d
         p.ring_rebuild = 1  # ring lists from ring_words before the first step
         self._set_outputs(p, T, lanes, n, drive, counts, sample)
         nwords = (n + 63) // 64
@@ -365,8 +377,9 @@ class ResidentRun:
 
     def __init__(self, engine: Engine, lanes: int, T: int, drive, counts, sample):
         sp = engine.spec
-        if T % sp.ring_d:
-            raise ValueError(f"T ({T}) must be a multiple of the ring depth ({sp.ring_d})")
+        if T % (sp.ring_d + 1):
+            raise ValueError(f"T ({T}) must be a multiple of the device ring's "
+                             f"{sp.ring_d + 1} slots")
         self.keep: list = []
         p, n, dev = engine._base_plan(lanes, self.keep)
         # the state this run owns (the module may later be given other tensors)
@@ -374,6 +387,7 @@ class ResidentRun:
         self.v = neuron.v
         self.refractory = neuron.refractory if neuron._use_refractory else None
         self.psc = sp.exp_psc.psc
+        self.iasc = getattr(neuron, "Iasc", None)
         self.v_reset = neuron.v_reset
         self.idx, self.cnt, self.words = engine._ring_buffers(sp.ring_d, lanes, n, dev)
         self.keep += [self.idx, self.cnt, self.words]
@@ -402,5 +416,7 @@ class ResidentRun:
         if self.refractory is not None:
             self.refractory.view(self.lanes, -1)[sl] = 0.0
         self.psc.view(self.lanes, -1)[sl] = 0.0
+        if self.iasc is not None:
+            self.iasc.view(self.lanes, -1)[sl] = 0.0
         self.cnt[:, sl] = 0
         self.words[:, sl] = 0

@@ -51,4 +51,37 @@ def lif_plan(neuron, lanes: int, device, keep: list) -> _lib.RsnnPlan:
         keep.append(vec)
         p.tau_ref = vec.data_ptr()
         p.tau_ref_stride = stride
+    if hasattr(neuron, "Iasc"):
+        glif3_fields(p, neuron, lanes, device, keep)
     return p
+
+
+def glif3_fields(p, neuron, lanes: int, device, keep: list) -> None:
+    """GLIF3 fields: v_rest, the membrane's exponential-Euler coefficient, the
+    after-spike current rates, coefficients and amplitudes as (n, n_asc), and
+    the Iasc state (lanes, n, n_asc)."""
+    n, shape, m = neuron.size, tuple(neuron.n_neuron), neuron.n_Iasc
+    dt = float(environ.get("dt"))
+    p.neuron_model = _lib.NEURON_GLIF3
+    p.n_asc = m
+    vec, stride = param_vec(neuron.v_rest, n, shape, device)
+    keep.append(vec)
+    p.v_rest, p.v_rest_stride = vec.data_ptr(), stride
+    vec, stride = param_vec(exp_euler_coeff(neuron.tau, dt), n, shape, device)
+    keep.append(vec)
+    p.k_v, p.k_v_stride = vec.data_ptr(), stride
+    k_cpu = torch.as_tensor(neuron.k).detach().to("cpu", torch.float32)
+    lin = -k_cpu
+    kk = torch.expm1(dt * lin) / lin
+    for field, t in (("asc_k", k_cpu), ("asc_kk", kk), ("asc_amp", neuron.asc_amps)):
+        t = torch.as_tensor(t).detach().to(device=device, dtype=torch.float32)
+        t = torch.broadcast_to(t, shape + (m,)).reshape(n, m).contiguous()
+        keep.append(t)
+        setattr(p, field, t.data_ptr())
+    ia = neuron.Iasc
+    if ia is None or ia.numel() != lanes * n * m or ia.dtype != torch.float32:
+        raise RuntimeError(f"GLIF3.Iasc is not initialised for {lanes} lanes x {n} neurons")
+    if not ia.is_contiguous():
+        neuron.Iasc = ia = ia.contiguous()
+    keep.append(ia)
+    p.iasc = ia.data_ptr()

@@ -117,9 +117,9 @@ class LanePool:
         spec = match_cell(model)
         if spec is None:
             raise ValueError("the pool needs a RecurrentNN cell the fused engine runs")
-        if chunk % spec.ring_d:
-            raise ValueError(f"chunk ({chunk}) must be a multiple of the ring depth "
-                             f"({spec.ring_d}) so the ring position repeats")
+        if chunk % (spec.ring_d + 1):
+            raise ValueError(f"chunk ({chunk}) must be a multiple of the device ring's "
+                             f"{spec.ring_d + 1} slots so the ring position repeats")
         self.model, self.spec, self.engine = model, spec, Engine(spec)
         self.n = spec.neuron.size
         self.lanes, self.chunk, self.n_sample = lanes, chunk, min(n_sample, self.n)

@@ -231,6 +231,21 @@ typedef struct btx_rsnn_plan {
     /* per-lane end of trial (local step, with in_t0): counts and samples stop
      * there; NULL = no end */
     const int64_t* lane_t_end;
+    /* GLIF3 (neuron_model == BTX_NEURON_GLIF3, glif.py:264-342), in the order
+     * charge -> adaptation -> fire -> reset of base.py:1108-1116:
+     *   v    += k_v * ((-(v - v_rest))/tau + (I + sum_j Iasc_j)/c_m)
+     *   Iasc_j += asc_kk_j * ((-asc_k_j) * Iasc_j)
+     *   fire and reset as LIF; Iasc_j += asc_amp_j * s
+     * with the exponential-Euler coefficients k_v = expm1(dt*l)/l, l = -1/tau,
+     * and asc_kk = expm1(dt*(-asc_k))/(-asc_k) evaluated by the caller
+     * (ode.py:20-50); sum_j runs from +0.0f in ascending j. */
+    int n_asc;                 /* after-spike currents per neuron, 1..4 */
+    const float* v_rest;   int64_t v_rest_stride;
+    const float* k_v;      int64_t k_v_stride;
+    const float* asc_k;        /* (n, n_asc) */
+    const float* asc_kk;       /* (n, n_asc) */
+    const float* asc_amp;      /* (n, n_asc) */
+    float* iasc;               /* state (lanes, n, n_asc), updated in place */
     /* 1 = build the ring lists from ring_words before the first step;
      * 0 = the ring lists are already current (a resident ring carried over
      * from the previous run) */

@@ -134,6 +134,59 @@ def lif_step(v, ref, x, p: LIFParams, dt: float):
     return spike, v, ref, v_pre
 
 
+@dataclass
+class GLIF3Params:
+    """GLIF3 parameters (glif.py:128-237): 0-d or (N,) tensors; k and
+    asc_amps (n_asc,) or (N, n_asc); v_rest defaults to v_reset."""
+
+    tau: torch.Tensor
+    v_threshold: torch.Tensor
+    v_reset: torch.Tensor
+    v_rest: torch.Tensor
+    c_m: torch.Tensor
+    tau_ref: torch.Tensor | None
+    k: torch.Tensor
+    asc_amps: torch.Tensor
+    hard_reset: bool = False
+
+    @staticmethod
+    def make(n, tau=20.0, v_threshold=-50.0, v_reset=-70.0, v_rest=None, c_m=0.05,
+             tau_ref=0.0, k=(0.2,), asc_amps=(0.0,), hard_reset=False, dtype=torch.float32):
+        def t(v):
+            return None if v is None else torch.as_tensor(v, dtype=dtype)
+        return GLIF3Params(t(tau), t(v_threshold), t(v_reset),
+                           t(v_reset if v_rest is None else v_rest), t(c_m), t(tau_ref),
+                           t(k), t(asc_amps), hard_reset)
+
+
+def glif3_step(v, iasc, ref, x, p: GLIF3Params, dt: float):
+    """One GLIF3 step, charge -> adaptation -> fire -> reset (base.py:1108-1116)
+    with glif.py:264-342 and exp_euler_step (ode.py:20-50).
+
+    Returns (spike, v_new, iasc_new, ref_new, v_pre)."""
+    # charge: exp_euler_step(dV, v, Iasc, x), glif.py:264-297
+    derivative = -(v - p.v_rest) / p.tau + (x + iasc.sum(-1, dtype=iasc.dtype)) / p.c_m
+    linear = -1.0 / p.tau
+    v = v + torch.expm1(dt * linear) / linear * derivative
+    # adaptation: exp_euler_step(dIasc, Iasc), glif.py:260-262, 299-300
+    d_i, lin_i = -p.k * iasc, -p.k
+    iasc = iasc + torch.expm1(dt * lin_i) / lin_i * d_i
+    # fire, glif.py:302-310
+    v_pre = v
+    spike = heaviside((v - p.v_threshold) / (p.v_threshold - p.v_reset))
+    if p.tau_ref is not None:
+        spike = spike * (ref == 0).to(v.dtype)
+    # reset, glif.py:312-336
+    if p.hard_reset:
+        v = v - (v - p.v_reset) * spike
+    else:
+        v = v - (p.v_threshold - p.v_reset) * spike
+    iasc = iasc + p.asc_amps * spike[..., None]
+    if p.tau_ref is not None:
+        ref = torch.relu(ref + spike * p.tau_ref - dt)
+    return spike, v, iasc, ref, v_pre
+
+
 def exp_psc_decay(psc, tau_syn, dt: float):
     """ExponentialPSC.conductance_charge (synapse.py:171-178) through
     exp_euler_step (ode.py:20-50): x + expm1(dt*A)/A * f(x)."""
@@ -191,9 +244,17 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
     raster.  Returns dict of stacked z, v, refractory, psc, v_pre (T, B, N)
     and the final state."""
     T, B, n = x_seq.shape
-    p = LIFParams(*(None if t is None else t.to(dtype) for t in
-                    (net.lif.tau, net.lif.v_threshold, net.lif.v_reset, net.lif.c_m,
-                     net.lif.tau_ref)), net.lif.hard_reset)
+    glif = isinstance(net.lif, GLIF3Params)
+    if glif:
+        g = net.lif
+        p = GLIF3Params(*(None if t is None else t.to(dtype) for t in
+                          (g.tau, g.v_threshold, g.v_reset, g.v_rest, g.c_m, g.tau_ref, g.k,
+                           g.asc_amps)), g.hard_reset)
+        n_asc = int(p.k.shape[-1]) if p.k.ndim else 1
+    else:
+        p = LIFParams(*(None if t is None else t.to(dtype) for t in
+                        (net.lif.tau, net.lif.v_threshold, net.lif.v_reset, net.lif.c_m,
+                         net.lif.tau_ref)), net.lif.hard_reset)
     tau_syn = torch.as_tensor(net.tau_syn).to(dtype)
     mm = net.conn if isinstance(net.conn, NativeSparse) else NativeSparse(net.conn, dtype)
     st = state or {}
@@ -202,6 +263,8 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
     v = st.get("v", v0).to(dtype).clone()
     ref = st.get("refractory", torch.zeros(B, n, dtype=dtype)).to(dtype).clone()
     psc = st.get("psc", torch.zeros(B, n, dtype=dtype)).to(dtype).clone()
+    iasc = (st.get("iasc", torch.zeros(B, n, n_asc, dtype=dtype)).to(dtype).clone()
+            if glif else None)
     ring = None
     if net.delay_mode == "hetero" and net.max_delay_steps > 1:
         ring = Ring(B, n, net.max_delay_steps, net.circular, dtype, st.get("history"),
@@ -212,7 +275,10 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
     rec = {k: [] for k in ("z", "v", "refractory", "psc", "v_pre")}
     x_seq = x_seq.to(dtype)
     for t in range(T):
-        z, v, ref, v_pre = lif_step(v, ref, psc + x_seq[t], p, net.dt)
+        if glif:
+            z, v, iasc, ref, v_pre = glif3_step(v, iasc, ref, psc + x_seq[t], p, net.dt)
+        else:
+            z, v, ref, v_pre = lif_step(v, ref, psc + x_seq[t], p, net.dt)
         zin = teacher[t].to(dtype) if teacher is not None else z
         psc = exp_psc_decay(psc, tau_syn, net.dt)
         if ring is None:
@@ -228,6 +294,8 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
             rec[k].append(val)
     out = {k: torch.stack(vals) for k, vals in rec.items()}
     out["final"] = {"v": v, "refractory": ref, "psc": psc}
+    if glif:
+        out["final"]["iasc"] = iasc
     if ring is not None:
         out["final"]["history"] = ring.h
         out["final"]["cursor"] = ring.cursor

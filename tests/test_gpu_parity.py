@@ -341,3 +341,61 @@ def test_engine_pull_path_bursts_bit_exact(circular, bin_cap):
     assert ref["z"].sum() > 500
     assert bits_equal(torch.cat([z1, z2]), ref["z"])
     assert bits_equal(torch.cat([s1["synapse.psc"], s2["synapse.psc"]]), ref["psc"])
+
+
+# ---------------------------------------------------------------------------
+# GLIF3
+# ---------------------------------------------------------------------------
+
+def _glif_kwargs(n, seed):
+    rng = np.random.default_rng(seed)
+    tau = torch.from_numpy(rng.uniform(10.0, 30.0, n).astype(np.float32))
+    return dict(v_threshold=-50.0, v_reset=-70.0, c_m=0.05, tau=tau, k=[0.3, 0.02],
+                asc_amps=[-0.6, 0.9], tau_ref=2.0)
+
+
+def test_glif3_single_step_bit_exact():
+    from btorch_amd.models import GLIF3
+
+    n, B = 1000, 3
+    kw = _glif_kwargs(n, 3)
+    neuron = GLIF3(n, **kw).to(DEV)
+    functional.init_net_state(neuron, batch_size=B, device=DEV, dtype=torch.float32)
+    p = rsnn.GLIF3Params.make(n, **kw)
+    v = torch.full((B, n), -70.0)
+    ia = torch.zeros(B, n, 2)
+    ref = torch.zeros(B, n)
+    g = torch.Generator().manual_seed(4)
+    for t in range(40):
+        x = (torch.rand(B, n, generator=g) < 0.2).float() * 6.0
+        with environ.context(dt=0.1), torch.no_grad():
+            z = neuron(x.to(DEV))
+        zr, v, ia, ref, _ = rsnn.glif3_step(v, ia, ref, x, p, 0.1)
+        assert bits_equal(z, zr), t
+        assert bits_equal(neuron.v, v) and bits_equal(neuron.Iasc, ia)
+        assert bits_equal(neuron.refractory, ref)
+    assert float(zr.sum()) >= 0
+
+
+@pytest.mark.parametrize("n", [600, 703])
+def test_engine_glif3_hetero_delays_bit_exact(n):
+    from btorch_amd.models import GLIF3
+
+    B, T, D = 2, 150, 5
+    c, cexp = _delayed_net(n, D, 17)
+    kw = _glif_kwargs(n, 5)
+    syn = HeterSynapsePSC(n, 1, None, SparseConn(cexp), base_psc=ExponentialPSC,
+                          max_delay_steps=D, use_circular_buffer=True, tau_syn=5.0)
+    model = RecurrentNN(GLIF3(n, **kw), syn, step_mode="m",
+                        update_state_names=["neuron.v", "synapse.psc"]).to(DEV)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    x = torch.from_numpy(workloads.poisson_input(T, B, n, 400.0, 0.1, seed=5) * 8.0)
+    with environ.context(dt=0.1), torch.no_grad():
+        z, st = model(x.to(DEV))
+    net = rsnn.Net(n, cexp, rsnn.GLIF3Params.make(n, **kw), torch.tensor(5.0), 0.1,
+                   delay_mode="hetero", max_delay_steps=D, circular=True)
+    ref = rsnn.simulate(net, x)
+    assert ref["z"].sum() > 100
+    assert bits_equal(z, ref["z"])
+    assert bits_equal(st["neuron.v"], ref["v"]) and bits_equal(st["synapse.psc"], ref["psc"])
+    assert bits_equal(model.neuron.Iasc, ref["final"]["iasc"])

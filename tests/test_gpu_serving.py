@@ -42,7 +42,7 @@ def _oracle_job(n, D, cexp, req, pool):
 def test_pool_poisson_counts_and_traces_bit_exact():
     n, D = 400, 5
     model, cexp = _model(n, D, 21)
-    pool = LanePool(model, lanes=6, chunk=4 * D, n_sample=7, base_rate_hz=2000.0, kick=12.0)
+    pool = LanePool(model, lanes=6, chunk=4 * (D + 1), n_sample=7, base_rate_hz=2000.0, kick=12.0)
     req = SimRequest(0, T=53, batch=4, rate_scale=1.5, seed=99)
     assert pool.admit(req, 0.0)
     out = []
@@ -59,7 +59,7 @@ def test_pool_poisson_counts_and_traces_bit_exact():
 def test_serving_alone_vs_mix_bit_exact():
     n, D = 500, 5
     model, cexp = _model(n, D, 23)
-    chunk = 6 * D
+    chunk = 6 * (D + 1)
     kw = dict(n_sample=5, base_rate_hz=1500.0, kick=12.0)
     rng = np.random.default_rng(5)
     reqs = [SimRequest(i, T=int(rng.integers(20, 140)), batch=int(rng.integers(1, 5)),
@@ -88,7 +88,7 @@ def test_serving_alone_vs_mix_bit_exact():
 def test_serving_private_networks_overlap_on_own_streams():
     n, D = 300, 5
     model, _ = _model(n, D, 25)
-    chunk = 4 * D
+    chunk = 4 * (D + 1)
     kw = dict(n_sample=4, base_rate_hz=1500.0, kick=12.0)
     nets = {}
 
@@ -121,7 +121,7 @@ def test_serving_graph_replay_matches_eager():
     # synchronisations) between replays; every job equals the eager pool's
     n, D = 400, 5
     model, cexp = _model(n, D, 27)
-    chunk = 4 * D
+    chunk = 4 * (D + 1)
     kw = dict(n_sample=6, base_rate_hz=1500.0, kick=12.0)
     rng = np.random.default_rng(8)
     reqs = [SimRequest(i, T=int(rng.integers(15, 90)), batch=int(rng.integers(1, 4)),

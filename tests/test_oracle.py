@@ -189,3 +189,51 @@ def test_poisson_drive_invariants():
     m = 2 * T * n
     assert abs(k / m - p) < 6 * math.sqrt(p * (1 - p) / m)
     assert float(rsnn.poisson_drive([1], [0.0], [0], 5, n, 12.0).sum()) == 0.0
+
+
+def test_glif3_no_spike_matches_exponential_form():
+    # Below threshold, exponential Euler with the after-spike currents held
+    # over the step is v' = v_rest + (v - v_rest) e + (1 - e) tau (x + sum I)/c_m,
+    # e = exp(-dt/tau), and I' = I exp(-k dt): a different formula (exp, not
+    # expm1) evaluated independently in float64.
+    dt, n, T = 0.1, 5, 300
+    tau = np.array([5.0, 10.0, 20.0, 35.0, 80.0])
+    k = np.array([0.2, 0.03])
+    c_m, v_rest, x = 0.05, -70.0, 0.3
+    p = rsnn.GLIF3Params.make(n, tau=torch.from_numpy(tau), v_threshold=1e6, v_reset=-70.0,
+                              c_m=c_m, tau_ref=None, k=torch.from_numpy(k),
+                              asc_amps=(1.0, -2.0), dtype=torch.float64)
+    v = torch.full((1, n), -65.0, dtype=torch.float64)
+    ia = torch.tensor([[[0.4, -0.7]] * n], dtype=torch.float64)
+    ref = torch.zeros(1, n, dtype=torch.float64)
+    xv = torch.full((1, n), x, dtype=torch.float64)
+    ve, ie = v.numpy()[0].copy(), ia.numpy()[0].copy()
+    e = np.exp(-dt / tau)
+    for _ in range(T):
+        z, v, ia, ref, _ = rsnn.glif3_step(v, ia, ref, xv, p, dt)
+        ve = v_rest + (ve - v_rest) * e + (1 - e) * tau * (x + ie.sum(-1)) / c_m
+        ie = ie * np.exp(-k * dt)
+        assert float(z.sum()) == 0.0
+    np.testing.assert_allclose(v.numpy()[0], ve, rtol=1e-12, atol=1e-10)
+    np.testing.assert_allclose(ia.numpy()[0], ie, rtol=1e-12, atol=1e-14)
+
+
+def test_glif3_spike_resets_and_asc_jump():
+    dt = 0.1
+    p = rsnn.GLIF3Params.make(3, tau=20.0, v_threshold=-50.0, v_reset=-70.0, c_m=0.05,
+                              tau_ref=1.0, k=(0.2, 0.05), asc_amps=(1.5, -0.5),
+                              dtype=torch.float64)
+    v = torch.tensor([[-49.0, -60.0, -49.5]], dtype=torch.float64)
+    ia = torch.zeros(1, 3, 2, dtype=torch.float64)
+    ref = torch.zeros(1, 3, dtype=torch.float64)
+    x = torch.zeros(1, 3, dtype=torch.float64)
+    z, v1, ia1, ref1, v_pre = rsnn.glif3_step(v, ia, ref, x, p, dt)
+    assert z.tolist() == [[1.0, 0.0, 1.0]]
+    # soft reset subtracts the threshold gap; asc jump by the amplitudes
+    np.testing.assert_allclose((v_pre - v1)[0].numpy(), [20.0, 0.0, 20.0], atol=1e-12)
+    np.testing.assert_allclose(ia1[0, 0].numpy(), [1.5, -0.5], atol=1e-15)
+    np.testing.assert_allclose(ia1[0, 1].numpy(), [0.0, 0.0], atol=1e-15)
+    np.testing.assert_allclose(ref1[0].numpy(), [0.9, 0.0, 0.9], atol=1e-12)
+    # refractory: above threshold again but no spike
+    z2, *_ = rsnn.glif3_step(torch.full((1, 3), -40.0, dtype=torch.float64), ia1, ref1, x, p, dt)
+    assert z2.tolist() == [[0.0, 1.0, 0.0]]
