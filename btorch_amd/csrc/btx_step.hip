@@ -50,9 +50,10 @@ namespace btx {
 constexpr int kChunk = BTX_RING_CHUNK;  // neurons per F workgroup = ring segment = bin
 constexpr int kChunkShift = 10;
 constexpr int kThreads = 256;           // 4 neurons per thread in the F role
-constexpr int kBinCap = 2048;           // terms per bin held in LDS
+constexpr int kBinCap = 1024;           // terms per bin held in LDS
 constexpr int kSortMax = 32;            // most terms a neuron sorts in LDS
-constexpr int kMaxChunks = 4096;        // n <= 4.19M (push reservation table)
+constexpr int kMaxChunks = 1024;        // n <= 1,048,576 (push reservation table)
+constexpr int kStage = 1024;            // terms a push workgroup stages in LDS
 
 static_assert(kChunk == (1 << kChunkShift) && kChunk == 4 * kThreads, "segment layout");
 
@@ -161,6 +162,9 @@ struct PushStep {
     float* ev_w;
     uint16_t* ev_d;
     float* acc;                  // fast mode: [lanes][n] of this step's buffer
+    const int64_t* lane_t_end;   // lanes past their trial end push nothing
+    const int64_t* in_t0;
+    int64_t t;
 };
 
 // LDS of one workgroup, shared by the two roles.
@@ -178,6 +182,10 @@ struct PushLds {
     int sub[BTX_MAX_BINS];
     int tile[kMaxChunks];
     int base[kMaxChunks];
+    int nev;                     // terms met in the first pass
+    int er[kStage];              // staged terms: virtual row, weight, destination
+    float ew[kStage];
+    int ed[kStage];
 };
 union StepLds {
     RedLds red;
@@ -190,6 +198,22 @@ union StepLds {
 
 __device__ __forceinline__ float prm(const float* p, int64_t stride, int64_t n) {
     return p[stride * n];
+}
+
+// 16-byte state accesses marked non-temporal: the state of a step is touched
+// once per step, so it streams through the caches instead of leaving dirty
+// lines behind for the end-of-kernel cache write-back.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void load4(const float* p, float* o) {
+    const f4v q = __builtin_nontemporal_load((const f4v*)p);
+    o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
+}
+
+__device__ __forceinline__ void store4(float* p, const float* v) {
+    f4v q;
+    q.x = v[0]; q.y = v[1]; q.z = v[2]; q.w = v[3];
+    __builtin_nontemporal_store(q, (f4v*)p);
 }
 
 // SplitMix64 finaliser (Steele, Lea, Flood 2014): a bijective 64-bit mix.
@@ -212,6 +236,7 @@ __device__ __forceinline__ float poisson_drive(const StepArgs& a, int64_t lane, 
 // current of the previous step) and `wz` as the previous step's summed push;
 // leaves as this step's decayed current.  `ia` holds the neuron's after-spike
 // currents (GLIF3).  Returns the spike (0/1).
+template <bool GLIF>
 __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
                                                 int64_t i, float& psc, float wz, float xin,
                                                 float& v, float& ref, float* ia) {
@@ -227,7 +252,7 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
     const float cm = prm(a.cm, a.cm_s, nidx);
     if (a.rec_v_prev) a.rec_v_prev[i] = v;
     float v1;
-    if (a.glif) {
+    if (GLIF) {
         // charge (exponential Euler, Iasc of the previous step)
         float isum = 0.f;
 #pragma unroll
@@ -254,7 +279,7 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         if (a.rec_ref) a.rec_ref[i] = ref;
     }
     v = a.hard ? (v1 - (v1 - vr) * s) : (v1 - (vth - vr) * s);
-    if (a.glif) {
+    if (GLIF) {
         const int64_t q = nidx * a.n_asc;
 #pragma unroll
         for (int j = 0; j < kMaxAsc; ++j)
@@ -312,7 +337,7 @@ __device__ float pull_sum(const PullArgs& a, int64_t lane, int64_t dst) {
 
 // s4[k] = summed input of neuron seg*1024 + tid*4 + k of `lane` (+0.0f when
 // it received none).  Every thread of the workgroup calls it.
-__device__ void reduce_segment(const ReduceIn& a, RedLds& L, int64_t n, int64_t nchunks,
+__device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int64_t n, int64_t nchunks,
                                int64_t lane, int64_t seg, float s4[4]) {
     const int tid = threadIdx.x;
     const int64_t n0 = seg * kChunk + tid * 4;
@@ -419,7 +444,17 @@ __device__ __forceinline__ void write_segment(uint32_t bits, int64_t seg_base,
     if (threadIdx.x == 0) *cnt = total;
 }
 
+template <bool GLIF>
 __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lane) {
+    if (a.lane_t_end != nullptr && a.in_t0[lane] + a.t >= a.lane_t_end[lane]) {
+        // no trial on this lane: consume what the previous push left and keep
+        // the state (the lane is reset when a job is admitted to it)
+        if (a.prev) {
+            float wz[4];
+            reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz);
+        }
+        return;
+    }
     const int tid = threadIdx.x;
     const int64_t seg_base = seg * kChunk;
     const int64_t n0 = seg_base + tid * 4;
@@ -429,20 +464,10 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     float v[4] = {0.f, 0.f, 0.f, 0.f}, p[4] = {0.f, 0.f, 0.f, 0.f};
     float r[4] = {0.f, 0.f, 0.f, 0.f}, x[4] = {0.f, 0.f, 0.f, 0.f};
     if (a.vec4 && full) {
-        const float4 v4 = *(const float4*)(a.v + i0);
-        v[0] = v4.x; v[1] = v4.y; v[2] = v4.z; v[3] = v4.w;
-        if (a.psc) {
-            const float4 q = *(const float4*)(a.psc + i0);
-            p[0] = q.x; p[1] = q.y; p[2] = q.z; p[3] = q.w;
-        }
-        if (a.use_ref) {
-            const float4 q = *(const float4*)(a.ref + i0);
-            r[0] = q.x; r[1] = q.y; r[2] = q.z; r[3] = q.w;
-        }
-        if (a.x) {
-            const float4 q = *(const float4*)(a.x + i0);
-            x[0] = q.x; x[1] = q.y; x[2] = q.z; x[3] = q.w;
-        }
+        load4(a.v + i0, v);
+        if (a.psc) load4(a.psc + i0, p);
+        if (a.use_ref) load4(a.ref + i0, r);
+        if (a.x) load4(a.x + i0, x);
     } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -454,12 +479,12 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
             }
         }
     }
-    float ia[4][kMaxAsc];
+    float ia[4][GLIF ? kMaxAsc : 1];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-        for (int j = 0; j < kMaxAsc; ++j)
-            ia[k][j] = (a.glif && j < a.n_asc && n0 + k < a.n)
+        for (int j = 0; j < (GLIF ? kMaxAsc : 1); ++j)
+            ia[k][j] = (GLIF && j < a.n_asc && n0 + k < a.n)
                            ? a.iasc[(i0 + k) * a.n_asc + j] : 0.f;
     float wz[4] = {0.f, 0.f, 0.f, 0.f};
     if (a.prev) reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz);
@@ -467,22 +492,22 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (n0 + k < a.n) {
-            const float s = lif_exp_neuron(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], v[k], r[k],
-                                           ia[k]);
+            const float s = lif_exp_neuron<GLIF>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k],
+                                                  v[k], r[k], ia[k]);
             nib |= (uint32_t)(s != 0.f) << k;
         }
     }
-    if (a.glif) {
+    if (GLIF) {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (int j = 0; j < kMaxAsc; ++j)
+            for (int j = 0; j < (GLIF ? kMaxAsc : 1); ++j)
                 if (j < a.n_asc && n0 + k < a.n) a.iasc[(i0 + k) * a.n_asc + j] = ia[k][j];
     }
     if (a.vec4 && full) {
-        *(float4*)(a.v + i0) = make_float4(v[0], v[1], v[2], v[3]);
-        if (a.use_ref) *(float4*)(a.ref + i0) = make_float4(r[0], r[1], r[2], r[3]);
-        if (a.psc) *(float4*)(a.psc + i0) = make_float4(p[0], p[1], p[2], p[3]);
+        store4(a.v + i0, v);
+        if (a.use_ref) store4(a.ref + i0, r);
+        if (a.psc) store4(a.psc + i0, p);
     } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -526,6 +551,7 @@ __device__ __forceinline__ int ring_slot(int64_t clock, int delay, int R) {
 
 template <int G, bool ORDERED>
 __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) {
+    if (a.lane_t_end != nullptr && a.in_t0[lane] + a.t >= a.lane_t_end[lane]) return;
     const int64_t c0 = px * a.cpb;
     const int ncb = (int)(a.nchunks - c0 < a.cpb ? a.nchunks - c0 : a.cpb);
     const int tid = threadIdx.x;
@@ -533,8 +559,10 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
         L.slot[tid] = ring_slot(a.clock, a.ctl->bin_delay[tid], a.R);
         L.sub[tid] = a.ctl->bin_sub[tid];
     }
-    if (ORDERED)
+    if (ORDERED) {
         for (int t = tid; t < a.nchunks; t += kThreads) L.tile[t] = 0;
+        if (tid == 0) L.nev = 0;
+    }
     __syncthreads();
     const int nseg = a.nbins * ncb;  // <= kThreads (host check)
     int c = 0;
@@ -556,7 +584,12 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
     constexpr int gpb = kThreads / G;
     const int gl = tid % G;
     const int64_t key0 = lane * a.n;
+    // pass 0: every term of the workgroup's rows, counted per destination
+    // segment and staged in LDS (fast mode: added to the accumulator);
+    // pass 1 (only when the stage overflowed) streams the rows again
+    int staged = 0;
     for (int pass = 0; pass < (ORDERED ? 2 : 1); ++pass) {
+        if (pass == 1 && staged <= kStage) break;
         for (int item = tid / G; item < total; item += gpb) {
             int lo = 0, hi = nseg;
             while (hi - lo > 1) {
@@ -573,24 +606,33 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
             const int64_t beg = a.row_ptr[r], end = a.row_ptr[r + 1];
             for (int64_t e = beg + gl; e < end; e += G) {
                 const int dst = a.col[e];
+                const float wv = a.w[e];
                 if (!ORDERED) {
-                    atomicAdd(&a.acc[key0 + dst], a.w[e]);
+                    atomicAdd(&a.acc[key0 + dst], wv);
                 } else if (pass == 0) {
                     atomicAdd(&L.tile[dst >> kChunkShift], 1);
+                    const int q = atomicAdd(&L.nev, 1);
+                    if (q < kStage) {
+                        L.er[q] = r;
+                        L.ew[q] = wv;
+                        L.ed[q] = dst;
+                    }
                 } else {
                     const int tl = dst >> kChunkShift;
                     const int pos = L.base[tl] + atomicAdd(&L.tile[tl], 1);
                     if (pos < a.cap) {
                         const int64_t q = (lane * a.nchunks + tl) * (int64_t)kBinCap + pos;
                         a.ev_r[q] = r;
-                        a.ev_w[q] = a.w[e];
+                        a.ev_w[q] = wv;
                         a.ev_d[q] = (uint16_t)(dst & (kChunk - 1));
                     }
                 }
             }
         }
         if (ORDERED && pass == 0) {
+            // one reservation per destination segment this workgroup touches
             __syncthreads();
+            staged = L.nev;
             for (int t = tid; t < a.nchunks; t += kThreads) {
                 const int ct = L.tile[t];
                 if (ct) {
@@ -599,6 +641,19 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
                 }
             }
             __syncthreads();
+            if (staged <= kStage) {
+                for (int i = tid; i < staged; i += kThreads) {
+                    const int dst = L.ed[i];
+                    const int tl = dst >> kChunkShift;
+                    const int pos = L.base[tl] + atomicAdd(&L.tile[tl], 1);
+                    if (pos < a.cap) {
+                        const int64_t q = (lane * a.nchunks + tl) * (int64_t)kBinCap + pos;
+                        a.ev_r[q] = L.er[i];
+                        a.ev_w[q] = L.ew[i];
+                        a.ev_d[q] = (uint16_t)(dst & (kChunk - 1));
+                    }
+                }
+            }
         }
     }
 }
@@ -607,13 +662,14 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 // Kernels
 // ---------------------------------------------------------------------------
 
-// Workgroups [0, nf) of a row play F, [nf, gridDim.x) play P.
-template <int G, bool ORDERED>
-__global__ __launch_bounds__(kThreads) void k_step(StepArgs a, PushStep p, int nf) {
+// Workgroups [0, np) of a row play P (the longer chain, dispatched first),
+// [np, gridDim.x) play F.
+template <int G, bool ORDERED, bool GLIF>
+__global__ __launch_bounds__(kThreads) void k_step(StepArgs a, PushStep p, int np) {
     __shared__ StepLds lds;
     const int64_t lane = blockIdx.y;
-    if ((int)blockIdx.x < nf) f_role(a, lds, blockIdx.x, lane);
-    else p_role<G, ORDERED>(p, lds.push, (int64_t)blockIdx.x - nf, lane);
+    if ((int)blockIdx.x < np) p_role<G, ORDERED>(p, lds.push, blockIdx.x, lane);
+    else f_role<GLIF>(a, lds, (int64_t)blockIdx.x - np, lane);
 }
 
 // FIN: the last push reduced into psc (and recorded).
@@ -787,8 +843,12 @@ BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float
              (!p->use_refractory || aligned16(p->refractory)) && (!x || aligned16(x));
     PushStep none{};
     const dim3 grid((unsigned)a.nchunks, (unsigned)p->lanes);
-    hipLaunchKernelGGL((k_step<16, true>), grid, dim3(kThreads), 0, (hipStream_t)stream, a,
-                       none, (int)a.nchunks);
+    if (a.glif)
+        hipLaunchKernelGGL((k_step<16, true, true>), grid, dim3(kThreads), 0, (hipStream_t)stream,
+                           a, none, 0);
+    else
+        hipLaunchKernelGGL((k_step<16, true, false>), grid, dim3(kThreads), 0, (hipStream_t)stream,
+                           a, none, 0);
     return check_launch("btx_lif_step");
 }
 
@@ -852,14 +912,25 @@ struct KernelTimer {
 
 static int rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s, KernelTimer& kt);
 
-template <bool ORDERED>
-static void launch_step(const StepArgs& a, const PushStep& ps, int nf, int group, dim3 grid,
-                        hipStream_t s) {
+template <bool ORDERED, bool GLIF>
+static void launch_step_g(const StepArgs& a, const PushStep& ps, int np, int group, dim3 grid,
+                          hipStream_t s) {
     switch (group) {
-        case 8: hipLaunchKernelGGL((k_step<8, ORDERED>), grid, dim3(kThreads), 0, s, a, ps, nf); break;
-        case 16: hipLaunchKernelGGL((k_step<16, ORDERED>), grid, dim3(kThreads), 0, s, a, ps, nf); break;
-        case 32: hipLaunchKernelGGL((k_step<32, ORDERED>), grid, dim3(kThreads), 0, s, a, ps, nf); break;
-        default: hipLaunchKernelGGL((k_step<64, ORDERED>), grid, dim3(kThreads), 0, s, a, ps, nf); break;
+        case 8: hipLaunchKernelGGL((k_step<8, ORDERED, GLIF>), grid, dim3(kThreads), 0, s, a, ps, np); break;
+        case 16: hipLaunchKernelGGL((k_step<16, ORDERED, GLIF>), grid, dim3(kThreads), 0, s, a, ps, np); break;
+        case 32: hipLaunchKernelGGL((k_step<32, ORDERED, GLIF>), grid, dim3(kThreads), 0, s, a, ps, np); break;
+        default: hipLaunchKernelGGL((k_step<64, ORDERED, GLIF>), grid, dim3(kThreads), 0, s, a, ps, np); break;
+    }
+}
+
+static void launch_step(int ordered, const StepArgs& a, const PushStep& ps, int np, int group,
+                        dim3 grid, hipStream_t s) {
+    if (ordered) {
+        if (a.glif) launch_step_g<true, true>(a, ps, np, group, grid, s);
+        else launch_step_g<true, false>(a, ps, np, group, grid, s);
+    } else {
+        if (a.glif) launch_step_g<false, true>(a, ps, np, group, grid, s);
+        else launch_step_g<false, false>(a, ps, np, group, grid, s);
     }
 }
 
@@ -909,13 +980,13 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     const int64_t nwords = (n + 63) / 64;
     const int64_t nchunks = (n + kChunk - 1) / kChunk;
     const int R = p->ring_d + 1;
-    if (nchunks > kMaxChunks) return fail("btx_rsnn_run: n exceeds 4096 ring segments");
+    if (nchunks > kMaxChunks) return fail("btx_rsnn_run: n exceeds 1024 ring segments");
     if (p->ordered && (p->csc_ptr == nullptr || p->csc_src == nullptr || p->csc_w == nullptr))
         return fail("btx_rsnn_run: ordered mode needs the dst-major arrays");
     if (p->ordered && (p->slots < 1 || p->slots > kSortMax))
         return fail("btx_rsnn_run: ordered mode needs 1 <= slots <= 32");
     if (p->bin_cap < 0 || p->bin_cap > kBinCap)
-        return fail("btx_rsnn_run: bin_cap must be in [0, 2048]");
+        return fail("btx_rsnn_run: bin_cap must be in [0, 1024]");
     if (p->in_prob != nullptr && (p->in_seed == nullptr || p->in_t0 == nullptr))
         return fail("btx_rsnn_run: Poisson drive needs per-lane seeds and clocks");
     if (p->lane_t_end != nullptr && p->in_t0 == nullptr)
@@ -996,6 +1067,8 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ps.ring_idx = p->ring_idx;
     ps.ring_cnt = p->ring_cnt;
     ps.cap = cap;
+    ps.lane_t_end = p->lane_t_end;
+    ps.in_t0 = p->in_t0;
     // segments per P workgroup: about 64 workgroups per lane, at most
     // kThreads segments (delay bins x segments) each
     int cpb = (int)(nchunks / 64);
@@ -1031,6 +1104,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         a.rec_v_prev = p->rec_v_prev ? p->rec_v_prev + t * rec_step : nullptr;
         a.rec_psc_prev = (p->rec_psc && t > 0) ? p->rec_psc + (t - 1) * rec_step : nullptr;
         ps.clock = p->ring_cursor + t;
+        ps.t = t;
         ReduceIn cur{};
         bins_of((int)(t & 1), cur);
         ps.bin_cnt = cur.bin_cnt;
@@ -1041,19 +1115,16 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         if (fused) {
             kt.begin(0);
             const dim3 grid((unsigned)(nf + npb), (unsigned)lanes);
-            if (p->ordered) launch_step<true>(a, ps, nf, group, grid, s);
-            else launch_step<false>(a, ps, nf, group, grid, s);
+            launch_step(p->ordered, a, ps, npb, group, grid, s);
             kt.end(0);
         } else {
             kt.begin(0);
             const dim3 fgrid((unsigned)nf, (unsigned)lanes);
-            if (p->ordered) launch_step<true>(a, ps, nf, group, fgrid, s);
-            else launch_step<false>(a, ps, nf, group, fgrid, s);
+            launch_step(p->ordered, a, ps, 0, group, fgrid, s);
             kt.end(0);
             kt.begin(2);
             const dim3 pgrid((unsigned)npb, (unsigned)lanes);
-            if (p->ordered) launch_step<true>(a, ps, 0, group, pgrid, s);
-            else launch_step<false>(a, ps, 0, group, pgrid, s);
+            launch_step(p->ordered, a, ps, npb, group, pgrid, s);
             kt.end(2);
         }
     }

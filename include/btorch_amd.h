@@ -228,8 +228,10 @@ typedef struct btx_rsnn_plan {
     int32_t* counts;      /* [lanes][n] spike counts, accumulated, or NULL */
     float* rec_sample;    /* [T][lanes][n_sample] membrane of neurons 0..n_sample-1, or NULL */
     int64_t n_sample;
-    /* per-lane end of trial (local step, with in_t0): counts and samples stop
-     * there; NULL = no end */
+    /* per-lane end of trial (local step, with in_t0): from there on the
+     * lane is idle -- no counts, samples, state update or push, and its state
+     * is left as it was (a caller resets the lane before reusing it); NULL =
+     * no end */
     const int64_t* lane_t_end;
     /* GLIF3 (neuron_model == BTX_NEURON_GLIF3, glif.py:264-342), in the order
      * charge -> adaptation -> fire -> reset of base.py:1108-1116:
