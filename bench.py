@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--serve-lanes", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="arrivals per second")
     ap.add_argument("--serve-tmax", type=int, default=5000)
+    ap.add_argument("--train-steps", type=int, default=2,
+                    help="timed training steps of the differentiable 16k-neuron net (0 = skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 512 MiB device copy first (HBM counter calibration)")
     return ap.parse_args()
@@ -256,6 +258,7 @@ def main():
                 "per_launch_event_p50_p90_p99_us": pct}
 
     serving = serve_bench(args, model, dev) if args.serve_requests > 0 else None
+    training = train_bench(args, dev) if args.train_steps > 0 else None
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
@@ -287,6 +290,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
+        "training": training,
     }
     print(json.dumps(line), flush=True)
 
@@ -358,6 +362,79 @@ def serve_bench(args, model, dev):
             "continuous": m, "plain": pl,
             "throughput_vs_plain": m["lane_steps_per_s"] / pl["lane_steps_per_s"],
             "build_s": round(t_build, 2)}
+
+
+def train_bench(args, dev):
+    """One training step of the differentiable RSNN through the hand-written
+    adjoint: N=16,384 LIF, Erdos-Renyi p=0.05 (13.4M synapses, weights
+    N(0, 1/sqrt(pN))), 200 Poisson input channels with per-class rate
+    patterns U(0, 50) Hz through a fixed 5%-density lognormal projection
+    (scaled by 0.2), 10 classes,
+    a linear readout of spike counts, cross-entropy, fast-sigmoid surrogate
+    (slope 25), batch 64, T=200, dt=1 ms, recompute checkpointing every 50
+    steps; gradients of the synaptic weights and per-neuron tau_m, one SGD
+    update.  Returns the JSON object."""
+    import math
+
+    from btorch_amd import workloads
+    from btorch_amd.models import (
+        LIF, ExponentialPSC, RecurrentNN, SparseConn, SuperSpike, environ, functional,
+    )
+
+    n, p, B, T, C, n_in = 16384, 0.05, 64, 200, 10, 200
+    rng = np.random.default_rng(args.seed + 2)
+    c = workloads.er_net_fast(n=n, p=p, seed=args.seed + 2)
+    w = rng.normal(0.0, 1.0 / math.sqrt(p * n), c.nnz).astype(np.float32)
+    idx = torch.from_numpy(np.stack([c.pre, c.post])).to(dev)
+    rec = SparseConn(torch.sparse_coo_tensor(idx, torch.from_numpy(w).to(dev), size=(n, n)),
+                     enforce_dale=False, device=dev)
+    win = workloads.input_weights(n_in, n, 0.05, args.seed)
+    win.data *= np.float32(0.2)
+    inp = SparseConn(win, enforce_dale=False, device=dev)
+    inp.magnitude.requires_grad_(False)
+    tau = torch.from_numpy(rng.uniform(10.0, 30.0, n).astype(np.float32))
+    sur = SuperSpike(alpha=25.0 / (math.sqrt(2) - 1))
+    model = RecurrentNN(LIF(n, tau=tau, tau_ref=2.0, trainable_param={"tau"},
+                            surrogate_function=sur),
+                        ExponentialPSC(n, 5.0, rec), neuron_inp_module=inp, step_mode="m",
+                        grad_checkpoint=True, chunk_size=50, unroll=50,
+                        update_state_names=[]).to(dev)
+    readout = torch.nn.Linear(n, C).to(dev)
+    params = [rec.magnitude, model.neuron.tau, *readout.parameters()]
+    opt = torch.optim.SGD(params, lr=1e-3)
+    rates = torch.from_numpy(rng.uniform(0.0, 50.0, (C, n_in)).astype(np.float32)).to(dev)
+    g = torch.Generator(device=dev).manual_seed(args.seed)
+
+    def batch():
+        labels = torch.randint(0, C, (B,), device=dev, generator=g)
+        prob = rates[labels] * 1e-3  # per 1 ms step
+        x = (torch.rand((T, B, n_in), device=dev, generator=g) < prob).float()
+        return x, labels
+
+    def step():
+        x, labels = batch()
+        functional.init_net_state(model, batch_size=B, device=dev, dtype=torch.float32)
+        opt.zero_grad(set_to_none=True)
+        with environ.context(dt=1.0):
+            z, _ = model(x)
+        logits = readout(z.mean(0))  # readout of spike counts / T
+        loss = torch.nn.functional.cross_entropy(logits, labels)
+        loss.backward()
+        opt.step()
+        return float(loss.detach()), float(z.detach().mean())
+
+    step()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = [step() for _ in range(args.train_steps)]
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / args.train_steps
+    return {"workload": "N=16384 LIF, ER p=0.05 (13.4M synapses), 200 Poisson inputs, "
+                        "10 classes, batch 64, T=200, dt=1 ms, checkpoint every 50 steps",
+            "synapses": int(c.nnz), "s_per_step": el, "samples_per_s": B / el,
+            "neuron_steps_per_s": B * T * n / el, "loss": out[-1][0],
+            "mean_rate_per_step": out[-1][1],
+            "grads": "synaptic weights (13.4M), tau_m (16384), readout"}
 
 
 def cpu_baseline(args, conn, snap, x_cpu, r_t, deg, tables, W):

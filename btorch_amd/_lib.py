@@ -161,6 +161,10 @@ _SIGNATURES = {
     "btx_adj_lif_step": (_c_int, [ctypes.POINTER(AdjLifArgs), _vp]),
     "btx_adj_transpose": (_c_int, [_vp, _vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp]),
     "btx_adj_wgrad": (_c_int, [_vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp, _vp]),
+    "btx_adj_transpose_delayed": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int, _c_i64, _vp, _c_i64,
+                                           _c_i64, _vp, _vp, _c_int, _vp]),
+    "btx_adj_wgrad_delayed": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_i64, _vp, _vp, _c_i64,
+                                       _c_i64, _vp, _vp, _vp]),
 }
 
 _lock = threading.Lock()

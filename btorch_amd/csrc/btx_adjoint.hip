@@ -164,6 +164,72 @@ __global__ __launch_bounds__(256) void k_adj_wgrad(const int64_t* __restrict__ r
     }
 }
 
+// Delayed synapses (HeterSynapsePSC / DelayedPSC): psc_s receives
+// W_sub^T z_{s - age(sub)} through the virtual rows j*csr_d + sub, so the
+// spike z_t of neuron j collects sum over sub of W_sub dL/dpsc_{t + age(sub)}
+// -- the reverse delay ring.  G holds dL/dpsc of the K steps of a window,
+// [K][n][lanes]; step tau of the window reads G[tau + age] when that index
+// lies in [0, K) (later steps belong to another window).
+__global__ __launch_bounds__(256) void k_adj_transpose_delayed(
+    const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+    const float* __restrict__ w, int64_t n, int csr_d, int64_t lanes,
+    const float* __restrict__ G, int64_t K, int64_t tau, const int* __restrict__ ages,
+    float* __restrict__ g_src, int accumulate) {
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ln = threadIdx.x & 63;
+    if (j >= n) return;
+    for (int64_t b0 = 0; b0 < lanes; b0 += 64) {
+        const int64_t b = b0 + ln;
+        const bool on = b < lanes;
+        float acc = 0.f;
+        for (int sub = 0; sub < csr_d; ++sub) {
+            const int age = ages[sub];
+            const int64_t st = tau + age;
+            if (age < 0 || st < 0 || st >= K) continue;
+            const float* g = G + st * n * lanes;
+            const int64_t r = j * csr_d + sub;
+            for (int64_t e = rp[r]; e < rp[r + 1]; ++e) {
+                const int64_t d = col[e];
+                const float wv = w[e];
+                if (on) acc += wv * g[d * lanes + b];
+            }
+        }
+        if (on) g_src[j * lanes + b] = accumulate ? g_src[j * lanes + b] + acc : acc;
+    }
+}
+
+// gw[e] += sum over lanes b spiking at neuron j (mask bit b) of
+// G[tau + age(sub)][col[e], b], for every virtual row j*csr_d + sub whose
+// step lies in the window.  One wavefront per neuron; lanes stride the edges.
+__global__ __launch_bounds__(256) void k_adj_wgrad_delayed(
+    const int64_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n, int csr_d,
+    int64_t lanes, const uint64_t* __restrict__ masks, const float* __restrict__ G, int64_t K,
+    int64_t tau, const int* __restrict__ ages, float* __restrict__ gw) {
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ln = threadIdx.x & 63;
+    if (j >= n) return;
+    const uint64_t mk = masks[j];
+    if (mk == 0) return;
+    for (int sub = 0; sub < csr_d; ++sub) {
+        const int age = ages[sub];
+        const int64_t st = tau + age;
+        if (age < 0 || st < 0 || st >= K) continue;
+        const float* g = G + st * n * lanes;
+        const int64_t r = j * csr_d + sub;
+        for (int64_t e = rp[r] + ln; e < rp[r + 1]; e += 64) {
+            const int64_t d = col[e];
+            float acc = 0.f;
+            uint64_t q = mk;
+            while (q) {
+                const int b = __ffsll((unsigned long long)q) - 1;
+                acc += g[d * lanes + b];
+                q &= q - 1;
+            }
+            gw[e] += acc;
+        }
+    }
+}
+
 }  // namespace btx
 
 using namespace btx;
@@ -221,4 +287,29 @@ BTX_API int btx_adj_wgrad(const int64_t* row_ptr, const int32_t* col, int64_t n_
     hipLaunchKernelGGL(k_adj_wgrad, dim3((unsigned)((n_rows + 3) / 4)), dim3(256), 0,
                        (hipStream_t)stream, row_ptr, col, n_rows, lanes, masks, g_dst, gw);
     return check_launch("btx_adj_wgrad");
+}
+
+BTX_API int btx_adj_transpose_delayed(const int64_t* row_ptr, const int32_t* col, const float* w,
+                                      int64_t n, int csr_d, int64_t lanes, const float* G,
+                                      int64_t K, int64_t tau, const int* ages, float* g_src,
+                                      int accumulate, void* stream) {
+    if (n <= 0 || lanes <= 0) return 0;
+    if (csr_d < 1 || csr_d > BTX_MAX_BINS) return fail("btx_adj_transpose_delayed: bad csr_d");
+    hipLaunchKernelGGL(k_adj_transpose_delayed, dim3((unsigned)((n + 3) / 4)), dim3(256), 0,
+                       (hipStream_t)stream, row_ptr, col, w, n, csr_d, lanes, G, K, tau, ages,
+                       g_src, accumulate);
+    return check_launch("btx_adj_transpose_delayed");
+}
+
+BTX_API int btx_adj_wgrad_delayed(const int64_t* row_ptr, const int32_t* col, int64_t n,
+                                  int csr_d, int64_t lanes, const uint64_t* masks,
+                                  const float* G, int64_t K, int64_t tau, const int* ages,
+                                  float* gw, void* stream) {
+    if (n <= 0 || lanes <= 0) return 0;
+    if (lanes > 64) return fail("btx_adj_wgrad_delayed: at most 64 lanes per mask word");
+    if (csr_d < 1 || csr_d > BTX_MAX_BINS) return fail("btx_adj_wgrad_delayed: bad csr_d");
+    hipLaunchKernelGGL(k_adj_wgrad_delayed, dim3((unsigned)((n + 3) / 4)), dim3(256), 0,
+                       (hipStream_t)stream, row_ptr, col, n, csr_d, lanes, masks, G, K, tau,
+                       ages, gw);
+    return check_launch("btx_adj_wgrad_delayed");
 }

@@ -4,7 +4,11 @@ The forward of a chunk of K steps runs the fused engine; its backward walks
 the chunk in reverse with ``btx_adj_transpose`` (W dL/dpsc),
 ``btx_adj_wgrad`` (event-driven dL/dW over the step's spiking neurons) and
 ``btx_adj_lif_step`` (surrogate derivative, reset and synapse-decay adjoint,
-dL/dtau).  With ``grad_checkpoint`` a chunk keeps only its entry state and
+dL/dtau).  With delayed synapses the spike of step t reaches psc at steps
+t + age, so the transposed product and the weight gradient read dL/dpsc of
+those later steps (``btx_adj_transpose_delayed`` / ``btx_adj_wgrad_delayed``,
+the reverse delay ring); the spikes of the ring_d steps before a chunk enter
+it as a differentiable input and receive their gradient from it.  With ``grad_checkpoint`` a chunk keeps only its entry state and
 recomputes its forward in the backward (checkpoint every ``chunk_size``
 steps, rnn.py:177-240); otherwise the forward records are kept.
 
@@ -52,8 +56,6 @@ def _check_supported(spec, lanes):
     neuron = spec.neuron
     if hasattr(neuron, "Iasc"):
         raise NotImplementedError("the fused adjoint covers LIF neurons (GLIF3 is forward-only)")
-    if spec.history is not None or spec.csr_d != 1 or spec.bins != [(0, 0)]:
-        raise NotImplementedError("the fused adjoint supports synapses without delays")
     for name in ("v_threshold", "v_reset", "c_m", "tau_ref"):
         t = getattr(neuron, name, None)
         if isinstance(t, torch.nn.Parameter) and t.requires_grad:
@@ -62,8 +64,20 @@ def _check_supported(spec, lanes):
         raise NotImplementedError("the fused adjoint supports at most 64 lanes")
 
 
+def _delayed(spec) -> bool:
+    return spec.history is not None or spec.csr_d != 1 or spec.bins != [(0, 0)]
+
+
+def _ages(spec, device) -> torch.Tensor:
+    """Ring age of every virtual-row offset (-1 = unused), int32 on device."""
+    ages = [-1] * spec.csr_d
+    for age, sub in spec.bins:
+        ages[sub] = age
+    return torch.tensor(ages, dtype=torch.int32, device=device)
+
+
 def adjoint_chunk(spec, traj: dict, grads: dict, w_csr: torch.Tensor, dt: float,
-                  valued_weights: bool = False) -> dict:
+                  valued_weights: bool = False, past: torch.Tensor | None = None) -> dict:
     """Reverse a chunk.
 
     traj: (K, B, N) ``v_prev``, ``v_pre``, ``z`` and ``ref_prev`` (or None),
@@ -72,8 +86,11 @@ def adjoint_chunk(spec, traj: dict, grads: dict, w_csr: torch.Tensor, dt: float,
         gradients of the outputs and of the state after the chunk.
     valued_weights: z holds real values (relaxed forward) -- the weight
         gradient then uses the valued product instead of spike masks.
+    past: delayed synapses only -- (D, B, N) spikes of the D = ring_d steps
+        before the chunk, oldest first.
     Returns ``gx`` (K,B,N), ``gw`` (E,) in CSR order, ``gtau`` ((N,) or (1,)),
-    ``gv``/``gpsc`` (B, N) of the state entering the chunk.
+    ``gv``/``gpsc`` (B, N) of the state entering the chunk, and with delays
+    ``gpast`` (D, B, N), the gradient of the spikes before the chunk.
     """
     neuron, syn, conn = spec.neuron, spec.exp_psc, spec.conn
     z = traj["z"]
@@ -117,9 +134,14 @@ def adjoint_chunk(spec, traj: dict, grads: dict, w_csr: torch.Tensor, dt: float,
     a.gtau = gtau.data_ptr()
     layout = conn._layout
     w = w_csr.detach().to(torch.float32).contiguous()
+    shifts = torch.arange(B, device=dev, dtype=torch.int64)
     if not valued_weights:
-        shifts = torch.arange(B, device=dev, dtype=torch.int64)
         masks = ((zz != 0).to(torch.int64) << shifts).sum(-1).contiguous()  # (K, N)
+    if _delayed(spec):
+        if valued_weights:
+            raise NotImplementedError("the relaxed weight gradient covers undelayed synapses")
+        return _adjoint_delayed(spec, a, lib, stream, layout, w, masks, zz, v_prev, v_pre,
+                                ref_prev, gz, gv, gpsc, gsyn, gx, gw, gtau, past, shifts)
     for t in range(K - 1, -1, -1):
         _lib.check(lib.btx_adj_transpose(layout.row_ptr.data_ptr(), layout.col.data_ptr(),
                                          w.data_ptr(), n, B, gpsc.data_ptr(),
@@ -144,6 +166,77 @@ def adjoint_chunk(spec, traj: dict, grads: dict, w_csr: torch.Tensor, dt: float,
             "gv": gv.t().contiguous(), "gpsc": gpsc.t().contiguous()}
 
 
+def _adjoint_delayed(spec, a, lib, stream, layout, w, masks, zz, v_prev, v_pre, ref_prev,
+                     gz, gv, gpsc, gsyn, gx, gw, gtau, past, shifts):
+    """Reverse sweep of a chunk with delayed synapses (see adjoint_chunk)."""
+    K, n, B = zz.shape
+    dev = zz.device
+    csr_d = spec.csr_d
+    ages = _ages(spec, dev)
+    G = torch.empty(K, n, B, device=dev)  # dL/dpsc of every step of the chunk
+    rp, col = layout.row_ptr.data_ptr(), layout.col.data_ptr()
+    for t in range(K - 1, -1, -1):
+        G[t].copy_(gpsc)
+        _lib.check(lib.btx_adj_transpose_delayed(rp, col, w.data_ptr(), n, csr_d, B,
+                                                 G.data_ptr(), K, t, ages.data_ptr(),
+                                                 gsyn.data_ptr(), 0, stream),
+                   "btx_adj_transpose_delayed")
+        _lib.check(lib.btx_adj_wgrad_delayed(rp, col, n, csr_d, B, masks[t].data_ptr(),
+                                             G.data_ptr(), K, t, ages.data_ptr(), gw.data_ptr(),
+                                             stream), "btx_adj_wgrad_delayed")
+        a.v_prev = v_prev[t].data_ptr()
+        a.v_pre = v_pre[t].data_ptr()
+        a.z = zz[t].data_ptr()
+        a.ref_prev = ref_prev[t].data_ptr() if ref_prev is not None else None
+        a.gz = gz[t].data_ptr()
+        a.gx = gx[t].data_ptr()
+        _lib.check(lib.btx_adj_lif_step(a, stream), "btx_adj_lif_step")
+    # the spikes before the chunk: their share of this chunk's psc
+    D = past.shape[0]
+    pz = past.detach().transpose(-1, -2).contiguous()  # (D, N, B)
+    pmasks = ((pz != 0).to(torch.int64) << shifts).sum(-1).contiguous()
+    gpast = torch.empty(D, n, B, device=dev)
+    for p in range(D):
+        tau = p - D
+        _lib.check(lib.btx_adj_transpose_delayed(rp, col, w.data_ptr(), n, csr_d, B,
+                                                 G.data_ptr(), K, tau, ages.data_ptr(),
+                                                 gpast[p].data_ptr(), 0, stream),
+                   "btx_adj_transpose_delayed")
+        _lib.check(lib.btx_adj_wgrad_delayed(rp, col, n, csr_d, B, pmasks[p].data_ptr(),
+                                             G.data_ptr(), K, tau, ages.data_ptr(),
+                                             gw.data_ptr(), stream), "btx_adj_wgrad_delayed")
+    return {"gx": gx.permute(0, 2, 1).contiguous(), "gw": gw, "gtau": gtau,
+            "gv": gv.t().contiguous(), "gpsc": gpsc.t().contiguous(),
+            "gpast": gpast.permute(0, 2, 1).contiguous()}
+
+
+def _history_from_past(spec, past: torch.Tensor, cursor: int) -> None:
+    """Module SpikeHistory memory holding `past` ((D, B, N), oldest first: the
+    last entry is the most recent step) with the given cursor."""
+    hist = spec.history
+    D = past.shape[0]
+    p = past.detach().to(torch.float32)
+    if hist.use_circular_buffer:
+        # delay d lives in slot (cursor - 1 - d) mod D and is past[D - 1 - d]
+        order = [(D - 1 - ((cursor - 1 - j) % D)) for j in range(D)]
+        h = p[order].permute(1, 0, 2)
+    else:
+        h = p.flip(0).permute(1, 0, 2)
+    hist.history = h.reshape(hist.history.shape).contiguous()
+    hist.set_cursor(cursor)
+
+
+def _past_from_history(spec) -> torch.Tensor:
+    """(D, B, N) spikes held by the module's SpikeHistory, oldest first."""
+    hist = spec.history
+    D = spec.ring_d
+    h = hist.history.reshape(hist.history.shape[0], D, -1)
+    if hist.use_circular_buffer:
+        c = hist.cursor
+        return torch.stack([h[:, (c - 1 - (D - 1 - p)) % D] for p in range(D)]).detach()
+    return torch.stack([h[:, D - 1 - p] for p in range(D)]).detach()
+
+
 def _set_state(spec, v, ref, psc):
     neuron, syn = spec.neuron, spec.exp_psc
     neuron.v = v.detach().clone().contiguous()
@@ -152,9 +245,11 @@ def _set_state(spec, v, ref, psc):
     syn.psc = psc.detach().clone().contiguous()
 
 
-def _forward_chunk(eng, x, v0, ref0, psc0, record):
+def _forward_chunk(eng, x, v0, ref0, psc0, record, past0=None):
     spec = eng.spec
     _set_state(spec, v0, ref0, psc0)
+    if past0 is not None:
+        _history_from_past(spec, past0, 0)
     K, B = x.shape[0], x.shape[1]
     raster, recs = eng.run(x.detach().contiguous(), K, B, record)
     n = spec.neuron
@@ -166,19 +261,22 @@ class _FusedChunk(torch.autograd.Function):
     """K fused steps; differentiable w.r.t. x, packed weights, tau, v0, psc0."""
 
     @staticmethod
-    def forward(ctx, eng, cfg, x, w_csr, tau, v0, ref0, psc0):
+    def forward(ctx, eng, cfg, x, w_csr, tau, v0, ref0, psc0, past0):
         checkpoint, extra, sink = cfg
         record = {"z"} | set(extra)
         if not checkpoint:
             record |= {"v_prev", "v_pre", "refractory"}
-        recs, v1, ref1, psc1 = _forward_chunk(eng, x, v0, ref0, psc0, record)
+        delayed = past0.numel() > 0
+        recs, v1, ref1, psc1 = _forward_chunk(eng, x, v0, ref0, psc0, record,
+                                              past0 if delayed else None)
         ctx.eng, ctx.checkpoint, ctx.tau_shape = eng, checkpoint, tau.shape
         ctx.dt = eng.spec_dt
+        ctx.delayed = delayed
         if checkpoint:
-            ctx.save_for_backward(x, w_csr, v0, ref0, psc0)
+            ctx.save_for_backward(x, w_csr, v0, ref0, psc0, past0)
         else:
-            ctx.save_for_backward(x, w_csr, v0, ref0, psc0, recs["v_prev"], recs["v_pre"],
-                                  recs["z"], recs["refractory"])
+            ctx.save_for_backward(x, w_csr, v0, ref0, psc0, past0, recs["v_prev"],
+                                  recs["v_pre"], recs["z"], recs["refractory"])
         for k in extra:
             sink.setdefault(k, []).append(recs[k])
         return recs["z"], v1.clone(), ref1.clone(), psc1.clone()
@@ -195,32 +293,37 @@ class _FusedChunk(torch.autograd.Function):
     def _backward(ctx, gz, gv, gpsc):
         eng = ctx.eng
         saved = ctx.saved_tensors
-        x, w_csr, v0, ref0, psc0 = saved[:5]
+        x, w_csr, v0, ref0, psc0, past0 = saved[:6]
         if ctx.checkpoint:
             # recompute the chunk from its entry state, with records
             with torch.no_grad():
                 snapshot = _snapshot(eng.spec)
                 recs, _, _, _ = _forward_chunk(eng, x, v0, ref0, psc0,
-                                               {"z", "v_prev", "v_pre", "refractory"})
+                                               {"z", "v_prev", "v_pre", "refractory"},
+                                               past0 if ctx.delayed else None)
                 _restore(eng.spec, snapshot)
             v_prev, v_pre, z, ref = recs["v_prev"], recs["v_pre"], recs["z"], recs["refractory"]
         else:
-            v_prev, v_pre, z, ref = saved[5:]
+            v_prev, v_pre, z, ref = saved[6:]
         ref_prev = (torch.cat([ref0.detach().unsqueeze(0), ref[:-1]])
                     if eng.spec.neuron._use_refractory else None)
         traj = {"v_prev": v_prev, "v_pre": v_pre, "z": z, "ref_prev": ref_prev}
-        out = adjoint_chunk(eng.spec, traj, {"gz": gz, "gv": gv, "gpsc": gpsc}, w_csr, ctx.dt)
+        out = adjoint_chunk(eng.spec, traj, {"gz": gz, "gv": gv, "gpsc": gpsc}, w_csr, ctx.dt,
+                            past=past0 if ctx.delayed else None)
         gtau = out["gtau"]
         if len(ctx.tau_shape) == 0 or int(torch.Size(ctx.tau_shape).numel()) == 1:
             gtau = gtau.sum().reshape(ctx.tau_shape)
         else:
             gtau = gtau.reshape(ctx.tau_shape)
-        return (None, None, out["gx"], out["gw"], gtau, out["gv"], None, out["gpsc"])
+        gpast = out.get("gpast") if ctx.delayed else None
+        return (None, None, out["gx"], out["gw"], gtau, out["gv"], None, out["gpsc"], gpast)
 
 
 def _snapshot(spec):
     n = spec.neuron
-    return (n.v, n.refractory if n._use_refractory else None, spec.exp_psc.psc)
+    h = spec.history
+    return (n.v, n.refractory if n._use_refractory else None, spec.exp_psc.psc,
+            None if h is None else (h.history, h.cursor))
 
 
 def _restore(spec, snap):
@@ -229,6 +332,9 @@ def _restore(spec, snap):
     if n._use_refractory:
         n.refractory = snap[1]
     spec.exp_psc.psc = snap[2]
+    if snap[3] is not None:
+        spec.history.history = snap[3][0]
+        spec.history.set_cursor(snap[3][1])
 
 
 def run_with_grad(rnn, eng, x):
@@ -255,9 +361,22 @@ def run_with_grad(rnn, eng, x):
     psc = syn.psc
     zs, sink = [], {}
     cfg = (bool(rnn.grad_checkpoint), tuple(sorted(want)), sink)
+    delayed = _delayed(spec)
+    if delayed:
+        D = spec.ring_d
+        cursor0 = spec.history.cursor
+        past = _past_from_history(spec)  # constants: no gradient to the history memory
+    else:
+        past = torch.empty(0, device=v.device)
     for c0 in range(0, T, chunk):
-        z, v, ref, psc = _FusedChunk.apply(eng, cfg, xs[c0:c0 + chunk], w_csr, tau, v, ref, psc)
+        z, v, ref, psc = _FusedChunk.apply(eng, cfg, xs[c0:c0 + chunk], w_csr, tau, v, ref, psc,
+                                           past)
         zs.append(z)
+        if delayed:
+            past = torch.cat([past, z])[-D:]
+    if delayed:
+        _history_from_past(spec, past, (cursor0 + T) % D
+                           if spec.history.use_circular_buffer else 0)
     neuron.v = v
     if neuron._use_refractory:
         neuron.refractory = ref.detach()

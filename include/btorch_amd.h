@@ -339,6 +339,26 @@ int btx_adj_transpose(const int64_t* row_ptr, const int32_t* col, const float* w
 int btx_adj_wgrad(const int64_t* row_ptr, const int32_t* col, int64_t n_rows, int64_t lanes,
                   const uint64_t* masks, const float* g_dst, float* gw, void* stream);
 
+/* Reverse delay ring of delayed synapses (HeterSynapsePSC,
+ * btorch/models/synapse.py:626-761 with SpikeHistory.get_flattened,
+ * history.py:299-345; DelayedPSC, synapse.py:537-623): virtual row
+ * j*csr_d + sub carries the spike of neuron j age(sub) steps late, so
+ *   g_src[j, :] (+)= sum_sub sum_{e in row j*csr_d+sub} w[e] * G[tau + age(sub)][col[e], :]
+ * over the sub with 0 <= tau + age(sub) < K; G is dL/dpsc of the K steps of a
+ * window, [K][n][lanes]; ages[csr_d] device ints (-1 = row offset unused). */
+int btx_adj_transpose_delayed(const int64_t* row_ptr, const int32_t* col, const float* w,
+                              int64_t n, int csr_d, int64_t lanes, const float* G, int64_t K,
+                              int64_t tau, const int* ages, float* g_src, int accumulate,
+                              void* stream);
+
+/* Event-driven weight gradient of the delayed product: for every neuron j
+ * with a nonzero lane mask (masks[j], bit b = lane b spiked at window step
+ * tau; lanes <= 64), gw[e] += sum_b G[tau + age(sub)][col[e], b] over the
+ * edges e of its virtual rows whose step lies in [0, K). */
+int btx_adj_wgrad_delayed(const int64_t* row_ptr, const int32_t* col, int64_t n, int csr_d,
+                          int64_t lanes, const uint64_t* masks, const float* G, int64_t K,
+                          int64_t tau, const int* ages, float* gw, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -352,15 +352,22 @@ class _Spike(torch.autograd.Function):
 
 def simulate_grad(n, dst, src, magnitude, x_seq, tau, dt, tau_syn, v_th=1.0, v_reset=0.0,
                   c_m=1.0, tau_ref=None, hard_reset=False, detach_reset=False,
-                  surrogate=("superspike", 2.0, 1.0, True), state=None, traj=None):
-    """Differentiable RecurrentNN(LIF, ExponentialPSC(SparseConn)) without
-    delays, op for op; ``magnitude`` in coalesced (dst, src) order.  Returns
-    z (T,B,N) and the final (v, refractory, psc); every op records autograd."""
+                  surrogate=("superspike", 2.0, 1.0, True), state=None, traj=None,
+                  csr_d=1, ages=(0,)):
+    """Differentiable RecurrentNN(LIF, ExponentialPSC-like(SparseConn)), op
+    for op; ``magnitude`` in coalesced (dst, virtual src) order.  With
+    ``csr_d`` > 1 or a nonzero age, the synapse reads the flattened spike
+    history (HeterSynapsePSC / DelayedPSC, synapse.py:537-761 with
+    history.py:252-345): virtual row j*csr_d + sub carries z_{t - ages[sub]}
+    of neuron j (zero before the run).  Returns z (T,B,N) and the final
+    (v, refractory, psc); every op records autograd."""
     T, B, _ = x_seq.shape
     dtype = x_seq.dtype
     kind, alpha, damping, spiking = surrogate
     idx = torch.stack([torch.as_tensor(dst), torch.as_tensor(src)])
-    A = torch.sparse_coo_tensor(idx, magnitude, size=(n, n), is_coalesced=True)
+    A = torch.sparse_coo_tensor(idx, magnitude, size=(n, n * csr_d), is_coalesced=True)
+    delayed = csr_d != 1 or tuple(ages) != (0,)
+    hist = []
     vth = torch.as_tensor(v_th, dtype=dtype)
     vr = torch.as_tensor(v_reset, dtype=dtype)
     cm = torch.as_tensor(c_m, dtype=dtype)
@@ -386,7 +393,14 @@ def simulate_grad(n, dst, src, magnitude, x_seq, tau, dt, tau_syn, v_th=1.0, v_r
         if tau_ref is not None:
             ref = torch.relu(ref + sd * tau_ref - dt)
         psc = psc + torch.expm1(dt * (-1.0 / ts)) / (-1.0 / ts) * (-psc / ts)
-        psc = psc + torch.sparse.mm(A, s.T).T
+        if delayed:
+            hist.append(s)
+            zero = torch.zeros_like(s)
+            cols = [hist[-1 - a] if 0 <= a < len(hist) else zero for a in ages]
+            zf = torch.stack(cols, dim=-1).reshape(B, n * csr_d)
+            psc = psc + torch.sparse.mm(A, zf.T).T
+        else:
+            psc = psc + torch.sparse.mm(A, s.T).T
         zs.append(s)
     return torch.stack(zs), v, ref, psc
 

@@ -237,3 +237,25 @@ def test_glif3_spike_resets_and_asc_jump():
     # refractory: above threshold again but no spike
     z2, *_ = rsnn.glif3_step(torch.full((1, 3), -40.0, dtype=torch.float64), ia1, ref1, x, p, dt)
     assert z2.tolist() == [[0.0, 1.0, 0.0]]
+
+
+def test_delayed_autograd_restatement_matches_forward_oracle():
+    # simulate_grad's delayed synapse (a spike stack per virtual row) runs the
+    # same forward as simulate's SpikeHistory ring, whose layout the delay
+    # expansion known answers pin
+    from btorch_amd import workloads
+
+    n, B, T, D = 60, 2, 40, 4
+    c = workloads.powerlaw_net_fast(n=n, k_min=5, k_max=20, delay_lo=1, delay_hi=D - 1, seed=3)
+    cexp = c.expanded(D)
+    x = torch.from_numpy(workloads.poisson_input(T, B, n, 800.0, 0.1, seed=3) * 25.0)
+    net = rsnn.Net(n, cexp, rsnn.LIFParams.make(n, tau_ref=2.0), torch.tensor(5.0), 0.1,
+                   delay_mode="hetero", max_delay_steps=D, circular=True)
+    ref = rsnn.simulate(net, x)
+    dst, src, w = rsnn.coalesce(cexp)
+    z, v, r, psc = rsnn.simulate_grad(n, dst, src, torch.from_numpy(w), x,
+                                      torch.tensor(20.0), 0.1, 5.0, tau_ref=2.0,
+                                      csr_d=D, ages=tuple(range(D)))
+    assert float(ref["z"].sum()) > 20
+    assert torch.equal(z, ref["z"]) and torch.equal(v, ref["final"]["v"])
+    assert torch.equal(psc, ref["final"]["psc"])
