@@ -6,7 +6,8 @@ out-degree power law (exponent 2.1, k in [100, 5000]; E ~ 50M), lognormal
 weights (I x4 negative), integer delays U{1..20} through
 HeterSynapsePSC(ExponentialPSC tau_s=5 ms, max_delay_steps=21), dt=0.1 ms,
 8 lanes, Poisson background 5 Hz per neuron-lane as a one-step current kick
-of 12 (v jumps by dt*12 = 1.2, one spike per event from rest).  Weights
+of 12 (v jumps by dt*12 = 1.2, one spike per event from rest), drawn in the
+step kernel from a counter-based hash of (lane seed, step, neuron).  Weights
 lognormal(mu=-3.7, sigma=0.5) keep the net background-dominated (~0.04%
 active per step, the configuration's ~0.05%); --w-mu/--kick change them.
 A "step" is one simulation time step of all 8 lanes.
@@ -98,16 +99,6 @@ def build(args, dev):
     return model, conn, deg, c.nnz
 
 
-def poisson_kicks(T, lanes, n, dev, seed, rate_hz=5.0, dt_ms=0.1, kick=20.0):
-    g = torch.Generator(device=dev).manual_seed(seed)
-    p = rate_hz * dt_ms * 1e-3
-    x = torch.empty((T, lanes, n), device=dev)
-    for t0 in range(0, T, 256):
-        t1 = min(T, t0 + 256)
-        x[t0:t1] = (torch.rand((t1 - t0, lanes, n), device=dev, generator=g) < p).float() * kick
-    return x
-
-
 def delivery_table(raster, n, deg):
     """A[t, d] = deliveries at delay d caused by the spikes of step t (summed over lanes)."""
     from btorch_amd.models._engine import unpack_bits
@@ -158,8 +149,9 @@ def _traffic(args):
     except (OSError, ValueError):
         return None
     cfg = d.get("config", {})
-    if (cfg.get("n"), cfg.get("lanes"), cfg.get("kmin"), cfg.get("w_mu"), cfg.get("mode")) != \
-            (args.n, args.lanes, args.kmin, args.w_mu, args.mode):
+    if (cfg.get("n"), cfg.get("lanes"), cfg.get("kmin"), cfg.get("w_mu"), cfg.get("mode"),
+            cfg.get("input")) != (args.n, args.lanes, args.kmin, args.w_mu, args.mode,
+                                  "in-kernel Poisson"):
         return None
     return d.get("bytes_per_launch")
 
@@ -189,11 +181,24 @@ def main():
     eng = model.fused_engine()
     n, lanes, D = args.n, args.lanes, args.delay_max + 1
     W, K, KS = args.warmup, args.steps, args.kernel_steps
-    x = poisson_kicks(W + K + KS, lanes, n, dev, args.seed, kick=args.kick)
+    # Poisson background generated in the step kernel: lane b's neuron j gets
+    # the kick at local step s when the counter-based hash of (seed_b, s, j)
+    # falls below rate*dt (the CPU oracle draws the same numbers)
+    seeds = (np.arange(lanes, dtype=np.int64) * 7919 + 1009 * args.seed) & 0x7FFFFFFFFFFFFFFF
+    p_in = np.float32(5.0 * 0.1 * 1e-3)
+    drive = {"seed": torch.from_numpy(seeds).to(dev),
+             "prob": torch.full((lanes,), float(p_in), dtype=torch.float32, device=dev),
+             "t0": torch.zeros(lanes, dtype=torch.int64, device=dev), "kick": args.kick}
+
+    def run(T, timings=None):
+        raster, _ = eng.run(None, T, lanes, set(), drive=drive, timings=timings)
+        drive["t0"] += T
+        return raster
+
     torch.cuda.synchronize()
 
     with environ.context(dt=0.1), torch.no_grad():
-        r_w, _ = eng.run(x[:W].contiguous(), W, lanes, set()) if W else (None, None)
+        r_w = run(W) if W else None
         torch.cuda.synchronize()
         # snapshot of the state entering the timed region (for the CPU sample)
         snap = {"v": model.neuron.v.detach().cpu().clone(),
@@ -201,18 +206,17 @@ def main():
                 "psc": model.synapse.base_psc.psc.detach().cpu().clone(),
                 "history": model.synapse.history.history.detach().cpu().clone(),
                 "cursor": model.synapse.history.cursor}
-        xt = x[W:W + K].contiguous()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev0.record()
-        r_t, _ = eng.run(xt, K, lanes, set())
+        r_t = run(K)
         ev1.record()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         region_ms = ev0.elapsed_time(ev1)
         timings: dict = {}
-        r_k, _ = eng.run(x[W + K:W + K + KS].contiguous(), KS, lanes, set(), timings=timings)
+        r_k = run(KS, timings)
         torch.cuda.synchronize()
 
     rasters = [r for r in (r_w, r_t, r_k) if r is not None]
@@ -236,13 +240,15 @@ def main():
     ev_per_launch = ev_k / KS
     spikes_per_launch = float(unpack_count(r_k)) / KS
     bytes_model = {
-        # F: v, refractory, psc read+write and the input x read (28 B per
-        # neuron-lane), per-neuron tau, spike words to the raster and the
-        # ring, ring indices of the spikes; the previous push's term records
-        # read (10 B per term: row, weight, destination).  P: per active row
-        # the index and the int64 row_ptr pair (20 B), per term col + weight
-        # read and the term record written (18 B).
-        "step": (28.0 * n * lanes + 4.0 * n + 16.0 * nwords * lanes + 4.0 * spikes_per_launch
+        # F: v and psc read+write, refractory read (20 B per neuron-lane;
+        # the refractory counter is written back only where it changes, a
+        # few per cent of neuron-lanes, not counted), per-neuron tau, spike
+        # words to the raster and the ring, ring indices of the spikes; the
+        # previous push's term records read (10 B per term: row, weight,
+        # destination).  P: per active row the index and the int64 row_ptr
+        # pair (20 B), per term col + weight read and the term record written
+        # (18 B).  The Poisson background is generated in the kernel.
+        "step": (20.0 * n * lanes + 4.0 * n + 16.0 * nwords * lanes + 4.0 * spikes_per_launch
                  + 20.0 * a_rows_per_launch + 28.0 * ev_per_launch),
     }
     # average launch duration from HIP events around the timed region (the
@@ -262,7 +268,11 @@ def main():
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
-        cpu = cpu_baseline(args, conn, snap, x[W:W + args.cpu_steps].cpu(), r_t, deg, tables, W)
+        from oracle import rsnn
+        x_cpu = rsnn.poisson_drive(seeds, np.full(lanes, p_in, dtype=np.float32),
+                                   np.full(lanes, W, dtype=np.int64), args.cpu_steps, n,
+                                   args.kick)
+        cpu = cpu_baseline(args, conn, snap, x_cpu, r_t, deg, tables, W)
 
     line = {
         "metric": "synaptic events/s at 140k-neuron connectome",

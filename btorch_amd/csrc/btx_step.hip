@@ -223,13 +223,26 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-// In-kernel Poisson drive of neuron nidx of `lane` at this step.
-__device__ __forceinline__ float poisson_drive(const StepArgs& a, int64_t lane, int64_t nidx) {
-    if (a.in_prob == nullptr) return 0.f;
-    const uint64_t s = (uint64_t)(a.in_t0[lane] + a.t);
-    const uint64_t h = mix64(a.in_seed[lane] ^ mix64(s * 0x100000001b3ull + (uint64_t)nidx));
-    const float u = (float)(h >> 40) * (1.0f / 16777216.0f);
-    return u < a.in_prob[lane] ? a.in_kick : 0.f;
+// In-kernel Poisson drive of neurons n0 .. n0+3 (n0 even) of `lane` at this
+// step: a SplitMix64 stream per (lane, local step), stream =
+// mix64(seed ^ step*0x100000001b3); output q = mix64(stream + (q+1)*gamma)
+// gives neurons 2q (bits 40..63) and 2q+1 (bits 16..39) a uniform in [0, 1)
+// each; a neuron gets the kick when its uniform falls below the lane's rate.
+__device__ __forceinline__ void poisson_drive4(const StepArgs& a, int64_t lane, int64_t n0,
+                                               float drv[4]) {
+    if (a.in_prob == nullptr) return;
+    const uint64_t step = (uint64_t)(a.in_t0[lane] + a.t);
+    const uint64_t stream = mix64((uint64_t)a.in_seed[lane] ^ (step * 0x100000001b3ull));
+    const float pr = a.in_prob[lane];
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2) {
+        const uint64_t q = (uint64_t)((n0 >> 1) + h2);
+        const uint64_t h = mix64(stream + (q + 1) * 0x9e3779b97f4a7c15ull);
+        const float u0 = (float)(h >> 40) * (1.0f / 16777216.0f);
+        const float u1 = (float)((h >> 16) & 0xffffffull) * (1.0f / 16777216.0f);
+        drv[2 * h2] = u0 < pr ? a.in_kick : 0.f;
+        drv[2 * h2 + 1] = u1 < pr ? a.in_kick : 0.f;
+    }
 }
 
 // One neuron of the fused step.  `psc` enters as the stored psc (decayed
@@ -239,12 +252,12 @@ __device__ __forceinline__ float poisson_drive(const StepArgs& a, int64_t lane, 
 template <bool GLIF>
 __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
                                                 int64_t i, float& psc, float wz, float xin,
-                                                float& v, float& ref, float* ia) {
+                                                float drive, float& v, float& ref, float* ia) {
     if (a.prev) {
         psc = psc + wz;
         if (a.rec_psc_prev) a.rec_psc_prev[i] = psc;
     }
-    xin = xin + poisson_drive(a, lane, nidx);
+    xin = xin + drive;
     const float I = a.psc ? psc + xin : xin;
     const float tau = prm(a.tau, a.tau_s, nidx);
     const float vth = prm(a.vth, a.vth_s, nidx);
@@ -486,14 +499,17 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
         for (int j = 0; j < (GLIF ? kMaxAsc : 1); ++j)
             ia[k][j] = (GLIF && j < a.n_asc && n0 + k < a.n)
                            ? a.iasc[(i0 + k) * a.n_asc + j] : 0.f;
+    const float r0[4] = {r[0], r[1], r[2], r[3]};
     float wz[4] = {0.f, 0.f, 0.f, 0.f};
     if (a.prev) reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz);
+    float drv[4] = {0.f, 0.f, 0.f, 0.f};
+    poisson_drive4(a, lane, n0, drv);
     uint32_t nib = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (n0 + k < a.n) {
             const float s = lif_exp_neuron<GLIF>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k],
-                                                  v[k], r[k], ia[k]);
+                                                  drv[k], v[k], r[k], ia[k]);
             nib |= (uint32_t)(s != 0.f) << k;
         }
     }
@@ -506,7 +522,9 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     }
     if (a.vec4 && full) {
         store4(a.v + i0, v);
-        if (a.use_ref) store4(a.ref + i0, r);
+        // the refractory counter mostly stays 0: write it only where it moved
+        if (a.use_ref && (r[0] != r0[0] || r[1] != r0[1] || r[2] != r0[2] || r[3] != r0[3]))
+            store4(a.ref + i0, r);
         if (a.psc) store4(a.psc + i0, p);
     } else {
 #pragma unroll

@@ -179,10 +179,13 @@ typedef struct btx_rsnn_plan {
     /* external input x[t, lane, n], NULL = none; x_step_stride elements per step */
     const float* x;       int64_t x_step_stride;
     /* in-kernel Poisson drive, added to x: neuron n of lane b receives in_kick
-     * at its local step s = in_t0[b] + t when the counter-based hash of
-     * (in_seed[b], s, n) falls below in_prob[b]; NULL in_prob = none.  The
-     * drive of a lane depends only on its own seed and clock, so a job gives
-     * the same result alone or packed with others. */
+     * at its local step s = in_t0[b] + t when its uniform falls below
+     * in_prob[b]; NULL in_prob = none.  Uniforms come from a SplitMix64
+     * stream per (lane, step): stream = mix64(in_seed[b] ^ s*0x100000001b3),
+     * h_q = mix64(stream + (q+1)*0x9e3779b97f4a7c15), neuron 2q takes bits
+     * 40..63 of h_q and neuron 2q+1 bits 16..39, each / 2^24.  The drive of a
+     * lane depends only on its own seed and clock, so a job gives the same
+     * result alone or packed with others. */
     const uint64_t* in_seed;
     const float* in_prob;
     const int64_t* in_t0;

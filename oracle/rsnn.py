@@ -423,23 +423,30 @@ def mix64(z: np.ndarray) -> np.ndarray:
 
 
 def poisson_drive(seed, prob, t0, T, n, kick) -> torch.Tensor:
-    """(T, B, n) float32 drive: ``kick`` where the hash of (lane seed, local
-    step t0+t, neuron) falls below the lane's probability, else 0.
+    """(T, B, n) float32 drive: ``kick`` where neuron j's uniform at the
+    lane's local step s = t0 + t falls below the lane's probability, else 0.
 
-    u = top 24 bits of mix64(seed ^ mix64(step * 0x100000001b3 + neuron)) / 2^24.
+    stream = mix64(seed ^ s*0x100000001b3); h_q = mix64(stream + (q+1)*gamma)
+    (SplitMix64 gamma 0x9e3779b97f4a7c15); neuron 2q takes bits 40..63 of h_q,
+    neuron 2q+1 bits 16..39, each divided by 2^24.
     """
     seed = np.asarray(seed, dtype=np.int64).astype(np.uint64)
     prob = np.asarray(prob, dtype=np.float32)
     t0 = np.asarray(t0, dtype=np.int64)
     B = seed.size
     out = np.zeros((T, B, n), dtype=np.float32)
-    nid = np.arange(n, dtype=np.uint64)
+    nq = (n + 1) // 2
+    q1 = np.arange(1, nq + 1, dtype=np.uint64)
+    gamma = np.uint64(0x9E3779B97F4A7C15)
+    scale = np.float32(1.0 / 16777216.0)
     for t in range(T):
         for b in range(B):
             s = np.uint64(int(t0[b]) + t)
             with np.errstate(over="ignore"):
-                inner = mix64(s * np.uint64(0x100000001B3) + nid)
-            h = mix64(seed[b] ^ inner)
-            u = (h >> np.uint64(40)).astype(np.float32) * np.float32(1.0 / 16777216.0)
-            out[t, b] = np.where(u < prob[b], np.float32(kick), np.float32(0.0))
+                stream = mix64(seed[b] ^ (s * np.uint64(0x100000001B3)))
+                h = mix64(stream + q1 * gamma)
+            u = np.empty(2 * nq, dtype=np.float32)
+            u[0::2] = (h >> np.uint64(40)).astype(np.float32) * scale
+            u[1::2] = ((h >> np.uint64(16)) & np.uint64(0xFFFFFF)).astype(np.float32) * scale
+            out[t, b] = np.where(u[:n] < prob[b], np.float32(kick), np.float32(0.0))
     return torch.from_numpy(out)
