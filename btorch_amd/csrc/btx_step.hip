@@ -165,6 +165,9 @@ struct PushStep {
     const int64_t* lane_t_end;   // lanes past their trial end push nothing
     const int64_t* in_t0;
     int64_t t;
+    int affine;                  // bin j is (delay0 + j, sub0 + j)
+    int delay0;
+    int sub0;
 };
 
 // LDS of one workgroup, shared by the two roles.
@@ -196,8 +199,27 @@ union StepLds {
 // Neuron update
 // ---------------------------------------------------------------------------
 
+// Parameter of neuron n: per-neuron array (stride 1) or one shared value
+// (stride 0); the stride is uniform, so this is a select, not a multiply.
 __device__ __forceinline__ float prm(const float* p, int64_t stride, int64_t n) {
-    return p[stride * n];
+    return stride ? p[n] : p[0];
+}
+
+// x / c, exactly as IEEE division, without dividing when c == 1.
+__device__ __forceinline__ float div_by(float x, float c) {
+    return c == 1.f ? x : x / c;
+}
+
+// (x >= 0) of the quotient u = a / d (d = v_th - v_reset) as the
+// reference's surrogate Heaviside sees it (surrogate/base.py:6-8).  For
+// finite d > 0 the sign of u is the sign of a, except that a tiny negative a
+// can round to -0.0, which also counts; only then is the division made.
+__device__ __forceinline__ bool fires(float a, float d) {
+    if (d > 0.f && d <= 3.4028234e38f) {
+        if (a >= 0.f) return true;
+        if (!(a > -d * 1e-43f)) return false;  // |a/d| stays above the underflow range
+    }
+    return (a / d) >= 0.f;
 }
 
 // 16-byte state accesses marked non-temporal: the state of a step is touched
@@ -271,7 +293,7 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
 #pragma unroll
         for (int j = 0; j < kMaxAsc; ++j)
             if (j < a.n_asc) isum = isum + ia[j];
-        const float dv = (-(v - prm(a.vrest, a.vrest_s, nidx))) / tau + (I + isum) / cm;
+        const float dv = (-(v - prm(a.vrest, a.vrest_s, nidx))) / tau + div_by(I + isum, cm);
         v1 = v + prm(a.kv, a.kv_s, nidx) * dv;
         // adaptation
         const int64_t q = nidx * a.n_asc;
@@ -279,12 +301,11 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         for (int j = 0; j < kMaxAsc; ++j)
             if (j < a.n_asc) ia[j] = ia[j] + a.asc_kk[q + j] * ((-a.asc_k[q + j]) * ia[j]);
     } else {
-        const float dv = (-(v - vr)) / tau + I / cm;
+        const float dv = (-(v - vr)) / tau + div_by(I, cm);
         v1 = v + a.dt * dv;
     }
     if (a.rec_v_pre) a.rec_v_pre[i] = v1;
-    const float u = (v1 - vth) / (vth - vr);
-    float s = (u >= 0.f) ? 1.f : 0.f;
+    float s = fires(v1 - vth, vth - vr) ? 1.f : 0.f;
     if (a.use_ref) {
         s = s * ((ref == 0.f) ? 1.f : 0.f);
         float r2 = (ref + s * prm(a.tref, a.tref_s, nidx)) - a.dt;
@@ -370,6 +391,12 @@ __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int
         return;
     }
     int* cntp = a.bin_cnt + lane * nchunks + seg;
+    const int64_t q0 = (lane * nchunks + seg) * (int64_t)kBinCap;
+    // the first kThreads terms are requested together with the count (the
+    // slots past the count hold stale terms and are ignored)
+    const int d_pre = a.ev_d[q0 + tid];
+    const int r_pre = a.ev_r[q0 + tid];
+    const float w_pre = a.ev_w[q0 + tid];
     const int c = *cntp;
     if (c == 0) return;  // uniform across the workgroup
 #pragma unroll
@@ -383,12 +410,12 @@ __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int
             if (n0 + k < n) s4[k] = pull_sum(a.pull, lane, n0 + k);
         return;
     }
-    const int64_t q0 = (lane * nchunks + seg) * (int64_t)kBinCap;
     for (int i = tid; i < c; i += kThreads) {
-        const int d = a.ev_d[q0 + i] & (kChunk - 1);
+        const bool pre = i == tid;
+        const int d = (pre ? d_pre : (int)a.ev_d[q0 + i]) & (kChunk - 1);
         L.d[i] = (uint16_t)d;
-        L.r[i] = a.ev_r[q0 + i];
-        L.w[i] = a.ev_w[q0 + i];
+        L.r[i] = pre ? r_pre : a.ev_r[q0 + i];
+        L.w[i] = pre ? w_pre : a.ev_w[q0 + i];
         atomicAdd(&L.cnt[d], 1);
     }
     __syncthreads();
@@ -574,8 +601,10 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
     const int ncb = (int)(a.nchunks - c0 < a.cpb ? a.nchunks - c0 : a.cpb);
     const int tid = threadIdx.x;
     if (tid < a.nbins) {
-        L.slot[tid] = ring_slot(a.clock, a.ctl->bin_delay[tid], a.R);
-        L.sub[tid] = a.ctl->bin_sub[tid];
+        // bins (delay0 + j, sub0 + j) need no table
+        const int dl = a.affine ? a.delay0 + tid : a.ctl->bin_delay[tid];
+        L.slot[tid] = ring_slot(a.clock, dl, a.R);
+        L.sub[tid] = a.affine ? a.sub0 + tid : a.ctl->bin_sub[tid];
     }
     if (ORDERED) {
         for (int t = tid; t < a.nchunks; t += kThreads) L.tile[t] = 0;
@@ -1087,6 +1116,11 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ps.cap = cap;
     ps.lane_t_end = p->lane_t_end;
     ps.in_t0 = p->in_t0;
+    ps.affine = 1;
+    ps.delay0 = p->bin_delay[0];
+    ps.sub0 = p->bin_sub[0];
+    for (int j = 1; j < p->nbins; ++j)
+        if (p->bin_delay[j] != ps.delay0 + j || p->bin_sub[j] != ps.sub0 + j) ps.affine = 0;
     // segments per P workgroup: about 64 workgroups per lane, at most
     // kThreads segments (delay bins x segments) each
     int cpb = (int)(nchunks / 64);
