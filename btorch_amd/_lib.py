@@ -21,6 +21,12 @@ RING_CHUNK = 1024
 
 NEURON_LIF = 0
 NEURON_GLIF3 = 1
+NEURON_TWO_COMPARTMENT = 2
+
+# indices into RsnnPlan.tc (BTX_TC_*)
+TC_DA, TC_DA1, TC_DS, TC_DS1, TC_DTH1, TC_R_S, TC_E_L, TC_DT, TC_DTC = range(9)
+TC_W_CA, TC_THETA_CA, TC_W_SA, TC_W_AS, TC_DELTA_TH = range(9, 14)
+TC_NPARAM = 14
 
 SURR_SIGMOID = 0
 SURR_SUPERSPIKE = 1
@@ -96,6 +102,15 @@ class RsnnPlan(ctypes.Structure):
         ("iasc", _vp),
         ("ring_rebuild", _c_int),
         ("bin_cap", _c_int),
+        ("x_apical", _vp), ("x_apical_step_stride", _c_i64),
+        ("i_a", _vp),
+        ("i_bap", _vp),
+        ("theta_th", _vp),
+        ("tc", _vp * TC_NPARAM),
+        ("tc_stride", _c_i64 * TC_NPARAM),
+        ("rec_i_a", _vp),
+        ("rec_i_bap", _vp),
+        ("rec_theta_th", _vp),
         ("workspace", _vp),
         ("workspace_bytes", ctypes.c_size_t),
     ]

@@ -36,6 +36,10 @@
 //   adaptation  psc + W^T z, W^T z summed per destination from +0.0f in
 //               ascending source row order (torch.sparse.mm on the coalesced
 //               (dst, src) matrix, linear.py:474-483)    synapse.py:82-87
+// GLIF3 (glif.py:264-342) and the two-compartment neuron
+// (two_compartment.py:208-292) replace the LIF update with their own, in
+// their reference op order; their transcendental coefficients come from the
+// host.
 
 #include <hip/hip_runtime.h>
 
@@ -89,6 +93,11 @@ struct ReduceIn {
 
 constexpr int kMaxAsc = 4;
 
+// Neuron kinds of the F role.
+constexpr int kLif = 0;
+constexpr int kGlif3 = 1;
+constexpr int kTwoComp = 2;
+
 struct StepArgs {
     int64_t n;
     int64_t nwords;
@@ -96,7 +105,7 @@ struct StepArgs {
     float dt;
     int hard;
     int use_ref;
-    int glif;              // 1 = GLIF3 membrane with after-spike currents
+    int kind;              // kLif, kGlif3 or kTwoComp
     int n_asc;
     const float* vrest;    int64_t vrest_s;
     const float* kv;       int64_t kv_s;
@@ -138,6 +147,16 @@ struct StepArgs {
     int64_t n_sample;
     const int64_t* lane_t_end;
     int vec4;              // 16-byte aligned state and n % 4 == 0
+    // two-compartment neuron
+    const float* xa;       // step t apical input (lanes, n) or NULL
+    float* tia;            // state i_a, i_bap, theta_th (lanes, n)
+    float* tibap;
+    float* tth;
+    const float* tcp[BTX_TC_NPARAM];
+    int64_t tcs[BTX_TC_NPARAM];
+    float* rec_ia;         // step t or NULL
+    float* rec_ibap;
+    float* rec_th;
     ReduceIn red;          // the previous step's push
 };
 
@@ -267,27 +286,49 @@ __device__ __forceinline__ void poisson_drive4(const StepArgs& a, int64_t lane, 
     }
 }
 
-// One neuron of the fused step.  `psc` enters as the stored psc (decayed
-// current of the previous step) and `wz` as the previous step's summed push;
-// leaves as this step's decayed current.  `ia` holds the neuron's after-spike
-// currents (GLIF3).  Returns the spike (0/1).
-template <bool GLIF>
-__device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
-                                                int64_t i, float& psc, float wz, float xin,
-                                                float drive, float& v, float& ref, float* ia) {
+// Somatic input of a neuron: the stored psc (decayed current of the previous
+// step) plus the previous step's summed push `wz`, plus x and the drive.
+__device__ __forceinline__ float soma_input(const StepArgs& a, int64_t i, float& psc, float wz,
+                                           float xin, float drive) {
     if (a.prev) {
         psc = psc + wz;
         if (a.rec_psc_prev) a.rec_psc_prev[i] = psc;
     }
     xin = xin + drive;
-    const float I = a.psc ? psc + xin : xin;
+    return a.psc ? psc + xin : xin;
+}
+
+// Records, spike counts and membrane samples of a neuron after its update,
+// then the synapse's exponential decay of psc.
+__device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, int64_t nidx,
+                                             int64_t i, float s, float v, float& psc) {
+    if (a.rec_v) a.rec_v[i] = v;
+    const bool live = a.lane_t_end == nullptr || a.in_t0[lane] + a.t < a.lane_t_end[lane];
+    if (a.counts && s != 0.f && live) a.counts[i] += 1;
+    if (a.rec_sample && nidx < a.n_sample && live) a.rec_sample[lane * a.n_sample + nidx] = v;
+    if (a.rec_z) a.rec_z[i] = s;
+    if (a.psc) {
+        const float tsyn = prm(a.tsyn, a.tsyn_s, nidx);
+        const float k = prm(a.ksyn, a.ksyn_s, nidx);
+        psc = psc + k * ((-psc) / tsyn);
+    }
+}
+
+// One LIF or GLIF3 neuron of the fused step.  `psc` enters as the stored psc
+// and leaves as this step's decayed current.  `ia` holds the neuron's
+// after-spike currents (GLIF3).  Returns the spike (0/1).
+template <int KIND>
+__device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
+                                                int64_t i, float& psc, float wz, float xin,
+                                                float drive, float& v, float& ref, float* ia) {
+    const float I = soma_input(a, i, psc, wz, xin, drive);
     const float tau = prm(a.tau, a.tau_s, nidx);
     const float vth = prm(a.vth, a.vth_s, nidx);
     const float vr = prm(a.vr, a.vr_s, nidx);
     const float cm = prm(a.cm, a.cm_s, nidx);
     if (a.rec_v_prev) a.rec_v_prev[i] = v;
     float v1;
-    if (GLIF) {
+    if (KIND == kGlif3) {
         // charge (exponential Euler, Iasc of the previous step)
         float isum = 0.f;
 #pragma unroll
@@ -313,22 +354,60 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         if (a.rec_ref) a.rec_ref[i] = ref;
     }
     v = a.hard ? (v1 - (v1 - vr) * s) : (v1 - (vth - vr) * s);
-    if (GLIF) {
+    if (KIND == kGlif3) {
         const int64_t q = nidx * a.n_asc;
 #pragma unroll
         for (int j = 0; j < kMaxAsc; ++j)
             if (j < a.n_asc) ia[j] = ia[j] + a.asc_amp[q + j] * s;
     }
-    if (a.rec_v) a.rec_v[i] = v;
-    const bool live = a.lane_t_end == nullptr || a.in_t0[lane] + a.t < a.lane_t_end[lane];
-    if (a.counts && s != 0.f && live) a.counts[i] += 1;
-    if (a.rec_sample && nidx < a.n_sample && live) a.rec_sample[lane * a.n_sample + nidx] = v;
-    if (a.rec_z) a.rec_z[i] = s;
-    if (a.psc) {
-        const float tsyn = prm(a.tsyn, a.tsyn_s, nidx);
-        const float k = prm(a.ksyn, a.ksyn_s, nidx);
-        psc = psc + k * ((-psc) / tsyn);
+    step_outputs(a, lane, nidx, i, s, v, psc);
+    return s;
+}
+
+// One two-compartment neuron (two_compartment.py:208-292): semi-implicit
+// apical current with the calcium plateau, semi-implicit soma with the
+// exponential initiation term, Heaviside spike at the adaptive threshold,
+// hard reset, bAP current for the next step, threshold decay and increment.
+// `xa` is the apical input.  Returns the spike (0/1).
+__device__ __forceinline__ float two_comp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
+                                                 int64_t i, float& psc, float wz, float xin,
+                                                 float drive, float xa, float& v, float& ia,
+                                                 float& ibap, float& th) {
+#define TCP(k) prm(a.tcp[k], a.tcs[k], nidx)
+    const float I = soma_input(a, i, psc, wz, xin, drive);
+    if (a.rec_v_prev) a.rec_v_prev[i] = v;
+    // apical current: w_Ca * sigmoid(i_a - theta_Ca) is +-0 for w_Ca == 0
+    // whenever the sigmoid is a number, so it is only evaluated otherwise
+    const float wca = TCP(BTX_TC_W_CA);
+    const float thca = TCP(BTX_TC_THETA_CA);
+    const float z = ia - thca;
+    const float plateau = (wca == 0.f && z == z) ? wca * 0.5f
+                                                 : wca * (1.f / (1.f + expf(-z)));
+    const float ia1 = (ia + TCP(BTX_TC_DA) * ((xa + plateau) + ibap)) / TCP(BTX_TC_DA1);
+    // soma
+    const float thr = prm(a.vth, a.vth_s, nidx) + th;
+    const float total = I + TCP(BTX_TC_W_AS) * ia1;
+    const float dT = TCP(BTX_TC_DT);
+    float ex = 0.f;
+    if (dT > 0.f) {
+        float e = (v - thr) / TCP(BTX_TC_DTC);
+        e = e > 10.f ? 10.f : e;  // keeps a NaN, as torch.clamp does
+        ex = dT * expf(e);
     }
+    const float vp =
+        (v + TCP(BTX_TC_DS) * ((TCP(BTX_TC_E_L) + TCP(BTX_TC_R_S) * total) + ex)) /
+        TCP(BTX_TC_DS1);
+    if (a.rec_v_pre) a.rec_v_pre[i] = vp;
+    const float s = (vp - thr) >= 0.f ? 1.f : 0.f;
+    ia = ia1;
+    v = vp - (vp - prm(a.vr, a.vr_s, nidx)) * s;
+    ibap = TCP(BTX_TC_W_SA) * s;
+    th = th / TCP(BTX_TC_DTH1) + TCP(BTX_TC_DELTA_TH) * s;
+#undef TCP
+    if (a.rec_ia) a.rec_ia[i] = ia;
+    if (a.rec_ibap) a.rec_ibap[i] = ibap;
+    if (a.rec_th) a.rec_th[i] = th;
+    step_outputs(a, lane, nidx, i, s, v, psc);
     return s;
 }
 
@@ -484,7 +563,38 @@ __device__ __forceinline__ void write_segment(uint32_t bits, int64_t seg_base,
     if (threadIdx.x == 0) *cnt = total;
 }
 
-template <bool GLIF>
+// State of neurons n0 .. n0+3 of a lane (`room` of them exist): one 16-byte
+// access when `vec`, else per neuron; neurons past n read as 0.
+__device__ __forceinline__ void load_lane4(const float* p, int64_t i0, int64_t room, bool vec,
+                                           float* o) {
+    if (vec && room >= 4) {
+        load4(p + i0, o);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = k < room ? p[i0 + k] : 0.f;
+}
+
+__device__ __forceinline__ void store_lane4(float* p, int64_t i0, int64_t room, bool vec,
+                                            const float* v) {
+    if (vec && room >= 4) {
+        store4(p + i0, v);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k < room) p[i0 + k] = v[k];
+}
+
+// Whether any of four values changed bit pattern (a sign of zero counts).
+__device__ __forceinline__ bool moved4(const float* a, const float* b) {
+    uint32_t d = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d |= __float_as_uint(a[k]) ^ __float_as_uint(b[k]);
+    return d != 0;
+}
+
+template <int KIND>
 __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lane) {
     if (a.lane_t_end != nullptr && a.in_t0[lane] + a.t >= a.lane_t_end[lane]) {
         // no trial on this lane: consume what the previous push left and keep
@@ -495,28 +605,29 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
         }
         return;
     }
+    constexpr bool GLIF = KIND == kGlif3;
+    constexpr bool TC = KIND == kTwoComp;
     const int tid = threadIdx.x;
     const int64_t seg_base = seg * kChunk;
     const int64_t n0 = seg_base + tid * 4;
-    const bool full = n0 + 4 <= a.n;
+    const int64_t room = a.n - n0;
+    const bool vec = a.vec4 != 0;
     const int64_t i0 = lane * a.n + n0;
     // state loads first, so they are in flight during the reduction
     float v[4] = {0.f, 0.f, 0.f, 0.f}, p[4] = {0.f, 0.f, 0.f, 0.f};
     float r[4] = {0.f, 0.f, 0.f, 0.f}, x[4] = {0.f, 0.f, 0.f, 0.f};
-    if (a.vec4 && full) {
-        load4(a.v + i0, v);
-        if (a.psc) load4(a.psc + i0, p);
-        if (a.use_ref) load4(a.ref + i0, r);
-        if (a.x) load4(a.x + i0, x);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (n0 + k < a.n) {
-                v[k] = a.v[i0 + k];
-                if (a.psc) p[k] = a.psc[i0 + k];
-                if (a.use_ref) r[k] = a.ref[i0 + k];
-                if (a.x) x[k] = a.x[i0 + k];
-            }
+    float ta[4] = {0.f, 0.f, 0.f, 0.f}, tb[4] = {0.f, 0.f, 0.f, 0.f};
+    float tt[4] = {0.f, 0.f, 0.f, 0.f}, xa[4] = {0.f, 0.f, 0.f, 0.f};
+    if (room > 0) {
+        load_lane4(a.v, i0, room, vec, v);
+        if (a.psc) load_lane4(a.psc, i0, room, vec, p);
+        if (a.use_ref) load_lane4(a.ref, i0, room, vec, r);
+        if (a.x) load_lane4(a.x, i0, room, vec, x);
+        if (TC) {
+            load_lane4(a.tia, i0, room, vec, ta);
+            load_lane4(a.tibap, i0, room, vec, tb);
+            load_lane4(a.tth, i0, room, vec, tt);
+            if (a.xa) load_lane4(a.xa, i0, room, vec, xa);
         }
     }
     float ia[4][GLIF ? kMaxAsc : 1];
@@ -524,9 +635,9 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int j = 0; j < (GLIF ? kMaxAsc : 1); ++j)
-            ia[k][j] = (GLIF && j < a.n_asc && n0 + k < a.n)
-                           ? a.iasc[(i0 + k) * a.n_asc + j] : 0.f;
+            ia[k][j] = (GLIF && j < a.n_asc && k < room) ? a.iasc[(i0 + k) * a.n_asc + j] : 0.f;
     const float r0[4] = {r[0], r[1], r[2], r[3]};
+    const float b0[4] = {tb[0], tb[1], tb[2], tb[3]};
     float wz[4] = {0.f, 0.f, 0.f, 0.f};
     if (a.prev) reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz);
     float drv[4] = {0.f, 0.f, 0.f, 0.f};
@@ -534,9 +645,14 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     uint32_t nib = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (n0 + k < a.n) {
-            const float s = lif_exp_neuron<GLIF>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k],
-                                                  drv[k], v[k], r[k], ia[k]);
+        if (k < room) {
+            float s;
+            if (TC)
+                s = two_comp_neuron(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv[k], xa[k],
+                                    v[k], ta[k], tb[k], tt[k]);
+            else
+                s = lif_exp_neuron<KIND>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv[k],
+                                         v[k], r[k], ia[k]);
             nib |= (uint32_t)(s != 0.f) << k;
         }
     }
@@ -545,22 +661,18 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
         for (int k = 0; k < 4; ++k)
 #pragma unroll
             for (int j = 0; j < (GLIF ? kMaxAsc : 1); ++j)
-                if (j < a.n_asc && n0 + k < a.n) a.iasc[(i0 + k) * a.n_asc + j] = ia[k][j];
+                if (j < a.n_asc && k < room) a.iasc[(i0 + k) * a.n_asc + j] = ia[k][j];
     }
-    if (a.vec4 && full) {
-        store4(a.v + i0, v);
-        // the refractory counter mostly stays 0: write it only where it moved
-        if (a.use_ref && (r[0] != r0[0] || r[1] != r0[1] || r[2] != r0[2] || r[3] != r0[3]))
-            store4(a.ref + i0, r);
-        if (a.psc) store4(a.psc + i0, p);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (n0 + k < a.n) {
-                a.v[i0 + k] = v[k];
-                if (a.use_ref) a.ref[i0 + k] = r[k];
-                if (a.psc) a.psc[i0 + k] = p[k];
-            }
+    if (room > 0) {
+        store_lane4(a.v, i0, room, vec, v);
+        // the refractory counter and the bAP current mostly stay 0: they are
+        // written only where they moved
+        if (a.use_ref && moved4(r, r0)) store_lane4(a.ref, i0, room, vec, r);
+        if (a.psc) store_lane4(a.psc, i0, room, vec, p);
+        if (TC) {
+            store_lane4(a.tia, i0, room, vec, ta);
+            if (moved4(tb, b0)) store_lane4(a.tibap, i0, room, vec, tb);
+            store_lane4(a.tth, i0, room, vec, tt);
         }
     }
     // spike word of 16 threads (64 neurons), by an OR across them
@@ -711,12 +823,12 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 
 // Workgroups [0, np) of a row play P (the longer chain, dispatched first),
 // [np, gridDim.x) play F.
-template <int G, bool ORDERED, bool GLIF>
+template <int G, bool ORDERED, int KIND>
 __global__ __launch_bounds__(kThreads) void k_step(StepArgs a, PushStep p, int np) {
     __shared__ StepLds lds;
     const int64_t lane = blockIdx.y;
     if ((int)blockIdx.x < np) p_role<G, ORDERED>(p, lds.push, blockIdx.x, lane);
-    else f_role<GLIF>(a, lds, (int64_t)blockIdx.x - np, lane);
+    else f_role<KIND>(a, lds, (int64_t)blockIdx.x - np, lane);
 }
 
 // FIN: the last push reduced into psc (and recorded).
@@ -848,8 +960,19 @@ static StepArgs neuron_args(const btx_rsnn_plan* p) {
     a.ksyn = p->k_syn; a.ksyn_s = p->k_syn_stride;
     a.v = p->v;
     a.ref = p->refractory;
-    a.glif = p->neuron_model == BTX_NEURON_GLIF3;
-    if (a.glif) {
+    a.kind = p->neuron_model == BTX_NEURON_GLIF3             ? kGlif3
+             : p->neuron_model == BTX_NEURON_TWO_COMPARTMENT ? kTwoComp
+                                                             : kLif;
+    if (a.kind == kTwoComp) {
+        a.tia = p->i_a;
+        a.tibap = p->i_bap;
+        a.tth = p->theta_th;
+        for (int k = 0; k < BTX_TC_NPARAM; ++k) {
+            a.tcp[k] = p->tc[k];
+            a.tcs[k] = p->tc_stride[k];
+        }
+    }
+    if (a.kind == kGlif3) {
         a.n_asc = p->n_asc;
         a.vrest = p->v_rest; a.vrest_s = p->v_rest_stride;
         a.kv = p->k_v; a.kv_s = p->k_v_stride;
@@ -861,9 +984,20 @@ static StepArgs neuron_args(const btx_rsnn_plan* p) {
     return a;
 }
 
+static bool aligned16(const void* q) { return ((uintptr_t)q & 15) == 0; }
+
 static int check_neuron(const btx_rsnn_plan* p) {
-    if (p->neuron_model != BTX_NEURON_LIF && p->neuron_model != BTX_NEURON_GLIF3)
+    if (p->neuron_model != BTX_NEURON_LIF && p->neuron_model != BTX_NEURON_GLIF3 &&
+        p->neuron_model != BTX_NEURON_TWO_COMPARTMENT)
         return fail("unsupported neuron model");
+    if (p->neuron_model == BTX_NEURON_TWO_COMPARTMENT) {
+        if (p->i_a == nullptr || p->i_bap == nullptr || p->theta_th == nullptr)
+            return fail("two-compartment neuron needs i_a, i_bap and theta_th state");
+        for (int k = 0; k < BTX_TC_NPARAM; ++k)
+            if (p->tc[k] == nullptr || p->tc_stride[k] < 0 || p->tc_stride[k] > 1)
+                return fail("two-compartment neuron needs every coefficient (stride 0 or 1)");
+        if (p->use_refractory) return fail("two-compartment neuron has no refractory state");
+    }
     if (p->neuron_model == BTX_NEURON_GLIF3 &&
         (p->n_asc < 1 || p->n_asc > kMaxAsc || p->v_rest == nullptr || p->k_v == nullptr ||
          p->asc_k == nullptr || p->asc_kk == nullptr || p->asc_amp == nullptr ||
@@ -875,7 +1009,23 @@ static int check_neuron(const btx_rsnn_plan* p) {
     return 0;
 }
 
-static bool aligned16(const void* q) { return ((uintptr_t)q & 15) == 0; }
+// 16-byte accesses are possible when every state array is aligned and n % 4 == 0.
+static int vec4_ok(const btx_rsnn_plan* p, const float* x) {
+    bool ok = (p->n % 4 == 0) && aligned16(p->v) && (!p->psc || aligned16(p->psc)) &&
+              (!p->use_refractory || aligned16(p->refractory)) && (!x || aligned16(x));
+    if (p->neuron_model == BTX_NEURON_TWO_COMPARTMENT)
+        ok = ok && aligned16(p->i_a) && aligned16(p->i_bap) && aligned16(p->theta_th) &&
+             (!p->x_apical || aligned16(p->x_apical)) &&
+             (p->x_apical_step_stride % 4 == 0);
+    ok = ok && (p->x_step_stride % 4 == 0);
+    return ok ? 1 : 0;
+}
+
+template <int KIND>
+static void launch_neuron_only(const StepArgs& a, dim3 grid, hipStream_t s) {
+    PushStep none{};
+    hipLaunchKernelGGL((k_step<16, true, KIND>), grid, dim3(kThreads), 0, s, a, none, 0);
+}
 
 BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float* v_pre,
                          void* stream) {
@@ -886,16 +1036,13 @@ BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float
     a.x = x;
     a.rec_z = z;
     a.rec_v_pre = v_pre;
-    a.vec4 = (p->n % 4 == 0) && aligned16(p->v) &&
-             (!p->use_refractory || aligned16(p->refractory)) && (!x || aligned16(x));
-    PushStep none{};
+    a.xa = a.kind == kTwoComp ? p->x_apical : nullptr;
+    a.vec4 = vec4_ok(p, x);
     const dim3 grid((unsigned)a.nchunks, (unsigned)p->lanes);
-    if (a.glif)
-        hipLaunchKernelGGL((k_step<16, true, true>), grid, dim3(kThreads), 0, (hipStream_t)stream,
-                           a, none, 0);
-    else
-        hipLaunchKernelGGL((k_step<16, true, false>), grid, dim3(kThreads), 0, (hipStream_t)stream,
-                           a, none, 0);
+    const hipStream_t s = (hipStream_t)stream;
+    if (a.kind == kGlif3) launch_neuron_only<kGlif3>(a, grid, s);
+    else if (a.kind == kTwoComp) launch_neuron_only<kTwoComp>(a, grid, s);
+    else launch_neuron_only<kLif>(a, grid, s);
     return check_launch("btx_lif_step");
 }
 
@@ -959,26 +1106,29 @@ struct KernelTimer {
 
 static int rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s, KernelTimer& kt);
 
-template <bool ORDERED, bool GLIF>
+template <bool ORDERED, int KIND>
 static void launch_step_g(const StepArgs& a, const PushStep& ps, int np, int group, dim3 grid,
                           hipStream_t s) {
     switch (group) {
-        case 8: hipLaunchKernelGGL((k_step<8, ORDERED, GLIF>), grid, dim3(kThreads), 0, s, a, ps, np); break;
-        case 16: hipLaunchKernelGGL((k_step<16, ORDERED, GLIF>), grid, dim3(kThreads), 0, s, a, ps, np); break;
-        case 32: hipLaunchKernelGGL((k_step<32, ORDERED, GLIF>), grid, dim3(kThreads), 0, s, a, ps, np); break;
-        default: hipLaunchKernelGGL((k_step<64, ORDERED, GLIF>), grid, dim3(kThreads), 0, s, a, ps, np); break;
+        case 8: hipLaunchKernelGGL((k_step<8, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np); break;
+        case 16: hipLaunchKernelGGL((k_step<16, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np); break;
+        case 32: hipLaunchKernelGGL((k_step<32, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np); break;
+        default: hipLaunchKernelGGL((k_step<64, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np); break;
     }
+}
+
+template <bool ORDERED>
+static void launch_step_k(const StepArgs& a, const PushStep& ps, int np, int group, dim3 grid,
+                          hipStream_t s) {
+    if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3>(a, ps, np, group, grid, s);
+    else if (a.kind == kTwoComp) launch_step_g<ORDERED, kTwoComp>(a, ps, np, group, grid, s);
+    else launch_step_g<ORDERED, kLif>(a, ps, np, group, grid, s);
 }
 
 static void launch_step(int ordered, const StepArgs& a, const PushStep& ps, int np, int group,
                         dim3 grid, hipStream_t s) {
-    if (ordered) {
-        if (a.glif) launch_step_g<true, true>(a, ps, np, group, grid, s);
-        else launch_step_g<true, false>(a, ps, np, group, grid, s);
-    } else {
-        if (a.glif) launch_step_g<false, true>(a, ps, np, group, grid, s);
-        else launch_step_g<false, false>(a, ps, np, group, grid, s);
-    }
+    if (ordered) launch_step_k<true>(a, ps, np, group, grid, s);
+    else launch_step_k<false>(a, ps, np, group, grid, s);
 }
 
 }  // namespace btx
@@ -1088,8 +1238,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     a.counts = p->counts;
     a.n_sample = p->n_sample;
     a.lane_t_end = p->lane_t_end;
-    a.vec4 = (n % 4 == 0) && aligned16(p->v) && aligned16(p->psc) &&
-             (!p->use_refractory || aligned16(p->refractory)) && (!p->x || aligned16(p->x));
+    a.vec4 = vec4_ok(p, p->x);
     a.red.pull.csc_ptr = p->csc_ptr;
     a.red.pull.csc_src = p->csc_src;
     a.red.pull.csc_w = p->csc_w;
@@ -1141,6 +1290,8 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
             a.red.pull.clock = p->ring_cursor + t - 1;
         }
         a.x = p->x ? p->x + t * p->x_step_stride : nullptr;
+        a.xa = (a.kind == kTwoComp && p->x_apical) ? p->x_apical + t * p->x_apical_step_stride
+                                                   : nullptr;
         a.t = t;
         a.rec_sample = p->rec_sample ? p->rec_sample + t * lanes * p->n_sample : nullptr;
         a.raster = p->raster + t * lanes * nwords;
@@ -1154,6 +1305,11 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         a.rec_ref = p->rec_refractory ? p->rec_refractory + t * rec_step : nullptr;
         a.rec_v_pre = p->rec_v_pre ? p->rec_v_pre + t * rec_step : nullptr;
         a.rec_v_prev = p->rec_v_prev ? p->rec_v_prev + t * rec_step : nullptr;
+        if (a.kind == kTwoComp) {
+            a.rec_ia = p->rec_i_a ? p->rec_i_a + t * rec_step : nullptr;
+            a.rec_ibap = p->rec_i_bap ? p->rec_i_bap + t * rec_step : nullptr;
+            a.rec_th = p->rec_theta_th ? p->rec_theta_th + t * rec_step : nullptr;
+        }
         a.rec_psc_prev = (p->rec_psc && t > 0) ? p->rec_psc + (t - 1) * rec_step : nullptr;
         ps.clock = p->ring_cursor + t;
         ps.t = t;

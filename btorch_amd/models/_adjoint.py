@@ -34,8 +34,11 @@ def needs_grad(rnn, x: torch.Tensor) -> bool:
         return True
     if any(p.requires_grad for p in rnn.parameters()):
         return True
-    n = rnn.neuron
-    return bool(n.v is not None and n.v.requires_grad)
+    for m in rnn.modules():
+        v = getattr(m, "v", None)
+        if torch.is_tensor(v) and v.requires_grad:
+            return True
+    return False
 
 
 def _surrogate_code(fn, require_spiking: bool = True) -> tuple[int, float, float]:
@@ -54,8 +57,9 @@ def _surrogate_code(fn, require_spiking: bool = True) -> tuple[int, float, float
 
 def _check_supported(spec, lanes):
     neuron = spec.neuron
-    if hasattr(neuron, "Iasc"):
-        raise NotImplementedError("the fused adjoint covers LIF neurons (GLIF3 is forward-only)")
+    if hasattr(neuron, "Iasc") or hasattr(neuron, "i_bap"):
+        raise NotImplementedError("the fused adjoint covers LIF neurons (GLIF3 and the "
+                                  "two-compartment neuron are forward-only)")
     for name in ("v_threshold", "v_reset", "c_m", "tau_ref"):
         t = getattr(neuron, name, None)
         if isinstance(t, torch.nn.Parameter) and t.requires_grad:

@@ -4,7 +4,11 @@ from C, with no per-step Python and no (B, N*D) history matrix.
 
 Supported cells (everything else runs the generic per-step loop):
   neuron   LIF / IF (soft or hard reset, optional refractory, scalar or
-           per-neuron parameters); GLIF3 with 1..4 after-spike currents
+           per-neuron parameters); GLIF3 with 1..4 after-spike currents;
+           TwoCompartmentGLIF, with an apical input sequence, in a
+           MixedNeuronPopulation of that one group (how the reference's
+           ApicalRecurrentNN drives it); a one-group population of the
+           other neurons too
   synapse  ExponentialPSC(linear=SparseConn(N, N))
            DelayedPSC(ExponentialPSC(SparseConn(N, N)), max_delay_steps=m)
            HeterSynapsePSC(base_psc=ExponentialPSC, n_receptor=1,
@@ -29,12 +33,14 @@ from .history import SpikeHistory
 from .linear import BaseSparseConn
 from .neurons.glif import GLIF3
 from .neurons.lif import LIF
+from .neurons.mixed import MixedNeuronPopulation
+from .neurons.two_compartment import TwoCompartmentGLIF
 from .synapse import DelayedPSC, ExponentialPSC, HeterSynapsePSC
 
 
 @dataclass
 class CellSpec:
-    neuron: LIF | GLIF3
+    neuron: LIF | GLIF3 | TwoCompartmentGLIF
     exp_psc: ExponentialPSC          # module whose psc decays and receives W z
     conn: BaseSparseConn
     history: SpikeHistory | None
@@ -43,6 +49,7 @@ class CellSpec:
     bins: list[tuple[int, int]]      # (ring age, virtual-row offset)
     psc_keys: list[str]              # dotted names whose state equals psc
     psc_modules: list = field(default_factory=list)
+    neuron_key: str = "neuron."      # dotted prefix of the neuron's memories
 
 
 def _bin_nonempty(conn: BaseSparseConn, csr_d: int) -> list[bool]:
@@ -54,7 +61,18 @@ def _bin_nonempty(conn: BaseSparseConn, csr_d: int) -> list[bool]:
 def match_cell(rnn) -> CellSpec | None:
     """CellSpec when ``rnn`` is a cell the engine runs, else None."""
     neuron, syn = getattr(rnn, "neuron", None), getattr(rnn, "synapse", None)
-    if not isinstance(neuron, (LIF, GLIF3)):
+    neuron_key = "neuron."
+    if isinstance(neuron, MixedNeuronPopulation):
+        # one group spans the whole neuron axis: the population is that neuron
+        if len(neuron.counts) != 1:
+            return None
+        neuron_key = f"neuron.{neuron._group_names[0]}."
+        neuron = neuron.groups()[0]
+    elif isinstance(neuron, TwoCompartmentGLIF):
+        return None  # its (spike, v) output is not a synapse input
+    if getattr(rnn, "synapse_apical", None) is not None:
+        return None
+    if not isinstance(neuron, (LIF, GLIF3, TwoCompartmentGLIF)):
         return None
     if isinstance(neuron, GLIF3) and not 1 <= neuron.n_Iasc <= 4:
         return None
@@ -98,6 +116,7 @@ def match_cell(rnn) -> CellSpec | None:
         spec.bins = [(d, d) for d in range(spec.csr_d) if ne[d]] or [(0, 0)]
     if len(spec.bins) > _lib.MAX_BINS:
         return None
+    spec.neuron_key = neuron_key
     return spec
 
 
@@ -143,7 +162,7 @@ class Engine:
         dev = neuron.v.device
         _lib.require_hip(neuron.v)
         cfg = conn.sparse_config
-        p = _plan.lif_plan(neuron, lanes, dev, keep)
+        p = _plan.neuron_plan(neuron, lanes, dev, keep)
         p.ordered = int(cfg.ordered)
         p.slots = int(cfg.slots)
         p.bin_cap = int(getattr(cfg, "bin_cap", 0))
@@ -162,7 +181,7 @@ class Engine:
         p.v = neuron.v.data_ptr()
         p.psc = syn.psc.data_ptr()
         keep += [neuron.v, syn.psc]
-        if neuron._use_refractory:
+        if getattr(neuron, "_use_refractory", False):
             if not neuron.refractory.is_contiguous():
                 neuron.refractory = neuron.refractory.contiguous()
             p.refractory = neuron.refractory.data_ptr()
@@ -230,8 +249,9 @@ class Engine:
     def run(self, x: torch.Tensor | None, T: int, lanes: int, record: set[str],
             teacher: torch.Tensor | None = None, timings: dict | None = None,
             drive: dict | None = None, counts: torch.Tensor | None = None,
-            sample: torch.Tensor | None = None):
+            sample: torch.Tensor | None = None, x_apical: torch.Tensor | None = None):
         """x: (T, lanes, n) fp32 contiguous or None.  Returns (raster, records).
+        x_apical: (T, lanes, n) apical input of a two-compartment neuron.
 
         drive: {"seed": (lanes,) int64, "prob": (lanes,) f32, "t0": (lanes,) int64,
         "kick": float, optional "t_end": (lanes,) int64} in-kernel Poisson input;
@@ -245,6 +265,14 @@ class Engine:
                 raise ValueError("engine input must be contiguous float32 (T, lanes, n)")
             p.x = x.data_ptr()
             p.x_step_stride = lanes * n
+        if x_apical is not None:
+            if p.neuron_model != _lib.NEURON_TWO_COMPARTMENT:
+                raise ValueError("an apical input needs a two-compartment neuron")
+            if x_apical.shape != (T, lanes, n) or not x_apical.is_contiguous() \
+                    or x_apical.dtype != torch.float32:
+                raise ValueError("apical input must be contiguous float32 (T, lanes, n)")
+            p.x_apical = x_apical.data_ptr()
+            p.x_apical_step_stride = lanes * n
         # delay ring (ring_d + 1 slots): the history memory's spike words go
         # into slots 0..ring_d-1 (slot ring_d - k holds the step k steps
         # before the run) and the run starts at slot ring_d; the lists are
@@ -285,9 +313,15 @@ class Engine:
                 raise ValueError("teacher must be contiguous int64 (T, lanes, ceil(n/64))")
             p.teacher = teacher.data_ptr()
         recs = {}
-        for key in ("z", "v", "refractory", "psc", "v_pre", "v_prev"):
+        for key in ("z", "v", "refractory", "psc", "v_pre", "v_prev", "i_a", "i_bap",
+                    "theta_th"):
             if key in record:
                 recs[key] = torch.empty((T, lanes, n), dtype=torch.float32, device=dev)
+        for key in ("i_a", "i_bap", "theta_th"):
+            if key in recs:
+                if p.neuron_model != _lib.NEURON_TWO_COMPARTMENT:
+                    raise ValueError(f"'{key}' is a two-compartment state")
+                setattr(p, "rec_" + key, recs[key].data_ptr())
         p.rec_z = recs["z"].data_ptr() if "z" in recs else None
         p.rec_v = recs["v"].data_ptr() if "v" in recs else None
         p.rec_refractory = recs["refractory"].data_ptr() if "refractory" in recs else None
@@ -385,7 +419,10 @@ class ResidentRun:
         # the state this run owns (the module may later be given other tensors)
         neuron = sp.neuron
         self.v = neuron.v
-        self.refractory = neuron.refractory if neuron._use_refractory else None
+        self.refractory = (neuron.refractory if getattr(neuron, "_use_refractory", False)
+                           else None)
+        self.tc_state = ([neuron.i_a, neuron.i_bap, neuron.theta_th]
+                         if isinstance(neuron, TwoCompartmentGLIF) else [])
         self.psc = sp.exp_psc.psc
         self.iasc = getattr(neuron, "Iasc", None)
         self.v_reset = neuron.v_reset
@@ -418,5 +455,7 @@ class ResidentRun:
         self.psc.view(self.lanes, -1)[sl] = 0.0
         if self.iasc is not None:
             self.iasc.view(self.lanes, -1)[sl] = 0.0
+        for t in self.tc_state:
+            t.view(self.lanes, -1)[sl] = 0.0
         self.cnt[:, sl] = 0
         self.words[:, sl] = 0

@@ -85,3 +85,71 @@ def glif3_fields(p, neuron, lanes: int, device, keep: list) -> None:
         neuron.Iasc = ia = ia.contiguous()
     keep.append(ia)
     p.iasc = ia.data_ptr()
+
+
+def _cpu32(t) -> torch.Tensor:
+    return torch.as_tensor(t).detach().to("cpu", torch.float32)
+
+
+def tc_coefficients(neuron, dt: float) -> list[torch.Tensor]:
+    """Coefficients BTX_TC_* of a TwoCompartmentGLIF, evaluated by torch on
+    the CPU in fp32 with the reference's ops (two_compartment.py:204-259:
+    _positive clamps, ``dt / tau``, ``1.0 + decay``)."""
+    c = [None] * _lib.TC_NPARAM
+    da = dt / torch.clamp(_cpu32(neuron.tau_a), min=1e-4)
+    ds = dt / torch.clamp(_cpu32(neuron.tau_s), min=1e-4)
+    dth = dt / torch.clamp(_cpu32(neuron.tau_th), min=1e-4)
+    delta_T = torch.clamp(_cpu32(neuron.delta_T), min=0.0)
+    c[_lib.TC_DA], c[_lib.TC_DA1] = da, 1.0 + da
+    c[_lib.TC_DS], c[_lib.TC_DS1] = ds, 1.0 + ds
+    c[_lib.TC_DTH1] = 1.0 + dth
+    c[_lib.TC_R_S] = torch.clamp(_cpu32(neuron.R_s), min=1e-4)
+    c[_lib.TC_E_L] = _cpu32(neuron.E_L)
+    c[_lib.TC_DT] = delta_T
+    c[_lib.TC_DTC] = torch.clamp(delta_T, min=1e-4)
+    c[_lib.TC_W_CA] = _cpu32(neuron.w_Ca)
+    c[_lib.TC_THETA_CA] = _cpu32(neuron.theta_Ca)
+    c[_lib.TC_W_SA] = _cpu32(neuron.w_sa)
+    c[_lib.TC_W_AS] = _cpu32(neuron.w_as)
+    c[_lib.TC_DELTA_TH] = _cpu32(neuron.delta_th)
+    return c
+
+
+def tc_plan(neuron, lanes: int, device, keep: list) -> _lib.RsnnPlan:
+    """Neuron fields of a plan for a TwoCompartmentGLIF (state initialised)."""
+    p = _lib.RsnnPlan()
+    n, shape = neuron.size, tuple(neuron.n_neuron)
+    p.n = n
+    p.lanes = lanes
+    p.neuron_model = _lib.NEURON_TWO_COMPARTMENT
+    p.hard_reset = 1
+    p.use_refractory = 0
+    p.dt = float(environ.get("dt"))
+    for field, value in (("v_th", neuron.v_threshold), ("v_reset", neuron.v_reset)):
+        vec, stride = param_vec(value, n, shape, device)
+        keep.append(vec)
+        setattr(p, field, vec.data_ptr())
+        setattr(p, field + "_stride", stride)
+    for k, t in enumerate(tc_coefficients(neuron, p.dt)):
+        vec, stride = param_vec(t, n, shape, device)
+        keep.append(vec)
+        p.tc[k] = vec.data_ptr()
+        p.tc_stride[k] = stride
+    for name in ("i_a", "i_bap", "theta_th"):
+        t = getattr(neuron, name)
+        if t is None or t.numel() != lanes * n or t.dtype != torch.float32:
+            raise RuntimeError(f"TwoCompartmentGLIF.{name} is not initialised for "
+                               f"{lanes} lanes x {n} neurons")
+        if not t.is_contiguous():
+            setattr(neuron, name, t.contiguous())
+            t = getattr(neuron, name)
+        keep.append(t)
+        setattr(p, name, t.data_ptr())
+    return p
+
+
+def neuron_plan(neuron, lanes: int, device, keep: list) -> _lib.RsnnPlan:
+    """Neuron fields of a plan for any neuron the HIP step runs."""
+    if hasattr(neuron, "i_bap"):
+        return tc_plan(neuron, lanes, device, keep)
+    return lif_plan(neuron, lanes, device, keep)

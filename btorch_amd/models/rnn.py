@@ -188,36 +188,46 @@ class RecurrentNN(RecurrentNNAbstract):
         x = x.reshape(T, -1, n) if x.numel() else x
         return x.to(torch.float32).contiguous(), lead
 
-    def _fused_forward(self, eng: Engine, x: torch.Tensor):
+    def _fused_forward(self, eng: Engine, x: torch.Tensor, x_apical: torch.Tensor | None = None):
         sp = eng.spec
         n = sp.neuron.size
         T = x.shape[0]
         xs, _ = self._input_sequence(x, n)
         lanes = xs.shape[1]
+        xa = None
+        if x_apical is not None:
+            xa = torch.broadcast_to(x_apical, x.shape).reshape(T, lanes, n)
+            xa = xa.to(torch.float32).contiguous()
         names = self._record_names()
         want = self._wanted_records(names, sp)
         hist0 = sp.history.history.clone() if (sp.history is not None and
                                                "history" in want) else None
         cursor0 = sp.history.cursor if sp.history is not None else 0
-        raster, recs = eng.run(xs, T, lanes, want)
+        raster, recs = eng.run(xs, T, lanes, want, x_apical=xa)
         return self._assemble(eng, names, recs, raster, hist0, cursor0, T, lanes, n)
 
-    @staticmethod
-    def _wanted_records(names, sp) -> set[str]:
+    # neuron memories the engine records, by name within the neuron module
+    _NEURON_RECORDS = {"v": "v", "refractory": "refractory", "v_pre_spike": "v_pre",
+                       "i_a": "i_a", "i_bap": "i_bap", "theta_th": "theta_th"}
+
+    @classmethod
+    def _record_of(cls, k: str, sp) -> str | None:
+        if k.startswith(sp.neuron_key):
+            return cls._NEURON_RECORDS.get(k[len(sp.neuron_key):])
+        if k in sp.psc_keys:
+            return "psc"
+        if k.endswith("history.history"):
+            return "history"
+        return None
+
+    @classmethod
+    def _wanted_records(cls, names, sp) -> set[str]:
         want = {"z"}
         for k in names:
-            if k == "neuron.v":
-                want.add("v")
-            elif k == "neuron.refractory":
-                want.add("refractory")
-            elif k == "neuron.v_pre_spike":
-                want.add("v_pre")
-            elif k in sp.psc_keys:
-                want.add("psc")
-            elif k.endswith("history.history"):
-                want.add("history")
-            else:
+            rec = cls._record_of(k, sp)
+            if rec is None:
                 raise NotImplementedError(f"recording '{k}' is not supported by the fused engine")
+            want.add(rec)
         return want
 
     def _assemble(self, eng, names, recs, raster, hist0, cursor0, T, lanes, n):
@@ -226,18 +236,13 @@ class RecurrentNN(RecurrentNNAbstract):
         z = recs["z"].reshape(T, *shape)
         states = {}
         for k in names:
-            if k == "neuron.v":
-                states[k] = recs["v"].reshape(T, *shape)
-            elif k == "neuron.refractory":
-                states[k] = recs["refractory"].reshape(T, *shape)
-            elif k == "neuron.v_pre_spike":
-                states[k] = recs["v_pre"].reshape(T, *shape)
-            elif k in sp.psc_keys:
-                states[k] = recs["psc"].reshape(T, *shape)
-            elif k.endswith("history.history"):
+            rec = self._record_of(k, sp)
+            if rec == "history":
                 seq = eng.history_sequence(raster, hist0, T, lanes, n, cursor0)
                 states[k] = seq.reshape(T, *sp.history.history.shape)
-        if sp.neuron.pre_spike_v and "v_pre" in recs:
+            else:
+                states[k] = recs[rec].reshape(T, *shape)
+        if getattr(sp.neuron, "pre_spike_v", False) and "v_pre" in recs:
             sp.neuron.v_pre_spike = recs["v_pre"][-1].reshape(shape).clone()
         # HeterSynapsePSC keeps its own receptor-summed copy of psc
         for m in sp.psc_modules:
@@ -249,4 +254,83 @@ class RecurrentNN(RecurrentNNAbstract):
         return z, states
 
 
-__all__ = ["RecurrentNNAbstract", "RecurrentNN", "make_rnn"]
+class ApicalRecurrentNN(RecurrentNN):
+    """RecurrentNN with an apical / top-down input for multi-port neurons
+    (btorch/models/rnn.py:530-649): z_t = neuron(synapse.psc + x_t, A_t)
+    with A_t = x_apical_t (+ synapse_apical.psc); both synapses receive z_t.
+    Call it as ``model(x, None, x_apical)`` so the loop slices the apical
+    sequence.  A one-group two-compartment population without
+    ``synapse_apical`` runs in the fused engine."""
+
+    def __init__(self, neuron: nn.Module, synapse: nn.Module, synapse_apical=None,
+                 syn_inp_module=None, neuron_inp_module=None, *, update_state_names=None,
+                 unroll=8, chunk_size=None, cpu_offload=False, grad_checkpoint=False,
+                 allow_buffer=False, **kwargs):
+        super().__init__(neuron=neuron, synapse=synapse, syn_inp_module=syn_inp_module,
+                         neuron_inp_module=neuron_inp_module,
+                         update_state_names=update_state_names, unroll=unroll,
+                         chunk_size=chunk_size, cpu_offload=cpu_offload,
+                         grad_checkpoint=grad_checkpoint, allow_buffer=allow_buffer, **kwargs)
+        self.synapse_apical = synapse_apical
+
+    def single_step_forward(self, x, x_syn=None, x_apical=None):
+        if self.neuron_inp_module is not None:
+            x = self.neuron_inp_module(x)
+        if self.syn_inp_module is not None:
+            x_syn = self.syn_inp_module(x_syn)
+        total_input = self.synapse.psc + x
+        apical_input = x_apical
+        if self.synapse_apical is not None:
+            apical_input = (self.synapse_apical.psc if apical_input is None
+                            else self.synapse_apical.psc + apical_input)
+        z = (self.neuron(total_input) if apical_input is None
+             else self.neuron(total_input, apical_input))
+        self.synapse(z if x_syn is None else z + x_syn)
+        if self.synapse_apical is not None:
+            self.synapse_apical(z if x_syn is None else z + x_syn)
+        return z, filter_hidden_states(self, self.update_state_names,
+                                       allow_buffer=self.allow_buffer)
+
+    def fused_engine(self) -> Engine | None:
+        if self.synapse_apical is not None:
+            return None
+        return super().fused_engine()
+
+    def _multi_step_loop(self, *args, loop_args=None, **kwargs):
+        eng = self.fused_engine()
+        x = args[0] if args else None
+        x_syn = args[1] if len(args) > 1 else None
+        x_apical = args[2] if len(args) > 2 else None
+        fusable = (eng is not None and not kwargs and len(args) <= 3 and torch.is_tensor(x)
+                   and x.device.type == "cuda" and x_syn is None
+                   and (x_apical is None or (torch.is_tensor(x_apical)
+                                             and x_apical.shape[0] == x.shape[0])))
+        if not fusable:
+            return RecurrentNNAbstract._multi_step_loop(self, *args, loop_args=loop_args,
+                                                        **kwargs)
+        from ._adjoint import needs_grad, run_with_grad
+
+        if needs_grad(self, x) or (x_apical is not None and x_apical.requires_grad
+                                   and torch.is_grad_enabled()):
+            if x_apical is not None:
+                raise NotImplementedError("the fused adjoint has no apical input")
+            return run_with_grad(self, eng, x)
+        with torch.no_grad():
+            return self._fused_forward(eng, x, x_apical)
+
+
+class SomaApicalRecurrentNN(ApicalRecurrentNN):
+    """ApicalRecurrentNN with both a somatic and an apical synapse
+    (btorch/models/rnn.py:652-712)."""
+
+    def __init__(self, neuron: nn.Module, synapse_soma: nn.Module, synapse_apical: nn.Module,
+                 syn_inp_module=None, neuron_inp_module=None, *, update_state_names=None,
+                 **kwargs):
+        super().__init__(neuron=neuron, synapse=synapse_soma, synapse_apical=synapse_apical,
+                         syn_inp_module=syn_inp_module, neuron_inp_module=neuron_inp_module,
+                         update_state_names=update_state_names, **kwargs)
+        self.synapse_soma = synapse_soma
+
+
+__all__ = ["RecurrentNNAbstract", "RecurrentNN", "ApicalRecurrentNN", "SomaApicalRecurrentNN",
+           "make_rnn"]

@@ -107,4 +107,19 @@ class Erf(SurrogateFunctionBase):
         return grad_output * (damping_factor * torch.exp(-LN2 * (self.alpha * x) ** 2))
 
 
-__all__ = ["SurrogateFunctionBase", "Sigmoid", "SuperSpike", "Erf"]
+class ATan(SurrogateFunctionBase):
+    """g(v) = 1/(1 + (alpha v)^2), the Cauchy kernel (btorch/models/surrogate/atan.py:49-72)."""
+
+    kernel_id = 3
+
+    def __init__(self, alpha: float = 2.0, damping_factor: float = 1.0, spiking: bool = True):
+        super().__init__(alpha, damping_factor, spiking)
+
+    def primitive(self, x):
+        return 0.5 + torch.atan(self.alpha * x) / math.pi
+
+    def derivative(self, x, grad_output, damping_factor: float = 1.0):
+        return grad_output * (damping_factor / (1.0 + (self.alpha * x) ** 2))
+
+
+__all__ = ["SurrogateFunctionBase", "Sigmoid", "SuperSpike", "Erf", "ATan"]

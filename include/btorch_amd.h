@@ -127,6 +127,24 @@ int btx_spmv_weight_grad(const int64_t* row_ptr, const int32_t* col, int64_t n_s
 /* Neuron models of the fused step. */
 #define BTX_NEURON_LIF 0      /* btorch/models/neurons/lif.py:144-187 */
 #define BTX_NEURON_GLIF3 1    /* btorch/models/neurons/glif.py:264-342 */
+#define BTX_NEURON_TWO_COMPARTMENT 2  /* btorch/models/neurons/two_compartment.py:208-292 */
+
+/* Coefficients of the two-compartment neuron, indices into plan.tc[]. */
+#define BTX_TC_DA 0        /* dt / max(tau_a, 1e-4) */
+#define BTX_TC_DA1 1       /* 1 + DA */
+#define BTX_TC_DS 2        /* dt / max(tau_s, 1e-4) */
+#define BTX_TC_DS1 3       /* 1 + DS */
+#define BTX_TC_DTH1 4      /* 1 + dt / max(tau_th, 1e-4) */
+#define BTX_TC_R_S 5       /* max(R_s, 1e-4) */
+#define BTX_TC_E_L 6       /* E_L */
+#define BTX_TC_DT 7        /* max(delta_T, 0) */
+#define BTX_TC_DTC 8       /* max(DT, 1e-4) */
+#define BTX_TC_W_CA 9      /* w_Ca */
+#define BTX_TC_THETA_CA 10 /* theta_Ca */
+#define BTX_TC_W_SA 11     /* w_sa */
+#define BTX_TC_W_AS 12     /* w_as */
+#define BTX_TC_DELTA_TH 13 /* delta_th */
+#define BTX_TC_NPARAM 14
 
 /* Surrogate functions (forward Heaviside, backward derivative). */
 #define BTX_SURR_SIGMOID 0    /* btorch/models/surrogate/sigmoid.py:16-29 */
@@ -255,9 +273,31 @@ typedef struct btx_rsnn_plan {
      * 0 = the ring lists are already current (a resident ring carried over
      * from the previous run) */
     int ring_rebuild;
-    /* ordered mode: terms per (lane, destination tile) bin, 0 = the maximum
-     * (6144); a bin that receives more is reduced by pulling */
+    /* ordered mode: terms per (lane, destination segment) bin, 0 = the
+     * maximum (1024); a bin that receives more is reduced by pulling */
     int bin_cap;
+    /* two-compartment GLIF (neuron_model == BTX_NEURON_TWO_COMPARTMENT,
+     * TwoCompartmentGLIF.single_step_forward, two_compartment.py:208-292),
+     * somatic input I = psc + x, apical input A = x_apical (0 when NULL):
+     *   i_a'  = (i_a + DA*((A + W_CA*sigmoid(i_a - THETA_CA)) + i_bap)) / DA1
+     *   thr   = v_th + theta_th
+     *   ex    = DT > 0 ? DT*exp(min((v - thr)/DTC, 10)) : 0
+     *   v_pre = (v + DS*((E_L + R_S*(I + W_AS*i_a')) + ex)) / DS1
+     *   s     = (v_pre - thr) >= 0 ; v = v_pre - (v_pre - v_reset)*s
+     *   i_bap = W_SA*s ; theta_th = theta_th/DTH1 + DELTA_TH*s
+     * with the coefficients BTX_TC_* evaluated by the caller in the
+     * reference's ops.  exp and the sigmoid are evaluated on the device, so
+     * a neuron with W_CA != 0 or DT > 0 agrees with torch to rounding, every
+     * other neuron bit for bit.  use_refractory must be 0. */
+    const float* x_apical;  int64_t x_apical_step_stride;
+    float* i_a;           /* state (lanes, n), updated in place */
+    float* i_bap;
+    float* theta_th;
+    const float* tc[BTX_TC_NPARAM];      /* (n,) or one value (stride 0) */
+    int64_t tc_stride[BTX_TC_NPARAM];
+    float* rec_i_a;       /* [T][lanes][n] or NULL */
+    float* rec_i_bap;     /* [T][lanes][n] or NULL */
+    float* rec_theta_th;  /* [T][lanes][n] or NULL */
     /* workspace from btx_rsnn_workspace_bytes(), zero-filled once */
     void* workspace;      size_t workspace_bytes;
 } btx_rsnn_plan;
@@ -269,10 +309,11 @@ size_t btx_rsnn_workspace_bytes(int64_t lanes, int64_t n, int slots, int ordered
 int btx_rsnn_run(const btx_rsnn_plan* plan, int64_t n_steps, void* stream);
 
 /* One neuron step without synapse (LIF.single_step_forward, lif.py:144-187,
- * base.py:1108-1116): reads the plan's neuron parameters and state v /
- * refractory, input x[lanes, n]; writes spikes z[lanes, n] (fp32 0/1) and,
- * if v_pre is not NULL, the membrane before reset.  Synapse, ring and
- * connectivity fields of the plan are ignored. */
+ * base.py:1108-1116; GLIF3 and the two-compartment neuron likewise): reads
+ * the plan's neuron parameters and state, input x[lanes, n] (and, for the
+ * two-compartment neuron, x_apical[lanes, n] of the plan); writes spikes
+ * z[lanes, n] (fp32 0/1) and, if v_pre is not NULL, the membrane before
+ * reset.  Synapse, ring and connectivity fields of the plan are ignored. */
 int btx_lif_step(const btx_rsnn_plan* plan, const float* x, float* z, float* v_pre,
                  void* stream);
 

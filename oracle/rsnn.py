@@ -187,6 +187,68 @@ def glif3_step(v, iasc, ref, x, p: GLIF3Params, dt: float):
     return spike, v, iasc, ref, v_pre
 
 
+@dataclass
+class TwoCompParams:
+    """TwoCompartmentGLIF parameters (two_compartment.py:127-182): 0-d or
+    (N,) tensors, defaults as in the reference constructor."""
+
+    v_threshold: torch.Tensor
+    v_reset: torch.Tensor
+    tau_s: torch.Tensor
+    R_s: torch.Tensor
+    E_L: torch.Tensor
+    tau_a: torch.Tensor
+    tau_th: torch.Tensor
+    delta_th: torch.Tensor
+    delta_T: torch.Tensor
+    w_Ca: torch.Tensor
+    theta_Ca: torch.Tensor
+    w_sa: torch.Tensor
+    w_as: torch.Tensor
+
+    @staticmethod
+    def make(v_threshold=-50.0, v_reset=-65.0, tau_s=20.0, R_s=1.0, E_L=-70.0, tau_a=100.0,
+             tau_th=50.0, delta_th=0.0, delta_T=0.0, w_Ca=0.0, theta_Ca=1.0, w_sa=0.5,
+             w_as=1.0, dtype=torch.float32):
+        vals = (v_threshold, v_reset, tau_s, R_s, E_L, tau_a, tau_th, delta_th, delta_T, w_Ca,
+                theta_Ca, w_sa, w_as)
+        return TwoCompParams(*(torch.as_tensor(v, dtype=dtype) for v in vals))
+
+    def to(self, dtype):
+        return TwoCompParams(*(getattr(self, f).to(dtype) for f in self.__dataclass_fields__))
+
+
+def tc_step(v, i_a, i_bap, theta_th, i_soma, i_apical, p: TwoCompParams, dt: float):
+    """One TwoCompartmentGLIF.single_step_forward (two_compartment.py:235-292):
+    semi-implicit apical current (:208-217), somatic voltage with the
+    exponential initiation term (:219-241), surrogate Heaviside spike, hard
+    reset (:243-249), bAP current and adaptive threshold (:251-259).
+
+    Returns (spike, v, i_a, i_bap, theta_th, v_pre)."""
+    # _update_apical_current, :212-217
+    tau_a = torch.clamp(p.tau_a, min=1e-4)
+    drive = i_apical + p.w_Ca * torch.sigmoid(i_a - p.theta_Ca) + i_bap
+    decay = dt / tau_a
+    i_a_next = (i_a + decay * drive) / (1.0 + decay)
+    threshold_eff = p.v_threshold + theta_th
+    # _update_somatic_voltage, :219-241
+    tau_s = torch.clamp(p.tau_s, min=1e-4)
+    R_s = torch.clamp(p.R_s, min=1e-4)
+    total_current = i_soma + p.w_as * i_a_next
+    delta_T = torch.clamp(p.delta_T, min=0.0)
+    exp_arg = torch.clamp((v - threshold_eff) / torch.clamp(delta_T, min=1e-4), max=10.0)
+    exp_drive = torch.where(delta_T > 0.0, delta_T * torch.exp(exp_arg), torch.zeros_like(v))
+    decay = dt / tau_s
+    v_pre = (v + decay * (p.E_L + R_s * total_current + exp_drive)) / (1.0 + decay)
+    spike = heaviside(v_pre - threshold_eff)
+    # state updates, :281-284
+    v = v_pre - (v_pre - p.v_reset) * spike
+    i_bap = p.w_sa * spike
+    tau_th = torch.clamp(p.tau_th, min=1e-4)
+    theta_th = theta_th / (1.0 + dt / tau_th) + p.delta_th * spike
+    return spike, v, i_a_next, i_bap, theta_th, v_pre
+
+
 def exp_psc_decay(psc, tau_syn, dt: float):
     """ExponentialPSC.conductance_charge (synapse.py:171-178) through
     exp_euler_step (ode.py:20-50): x + expm1(dt*A)/A * f(x)."""
@@ -238,14 +300,20 @@ class Net:
     circular: bool = False
 
 
-def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, teacher=None):
+def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, teacher=None,
+             x_apical=None):
     """Run T steps; x_seq (T, B, N).  ``teacher`` (T, B, N) forces the spikes
     fed to the synapse (teacher forcing) while still reporting the produced
-    raster.  Returns dict of stacked z, v, refractory, psc, v_pre (T, B, N)
-    and the final state."""
+    raster.  With TwoCompParams the cell is ApicalRecurrentNN over a
+    TwoCompartmentGLIF population (rnn.py:599-649) and ``x_apical`` (T, B, N)
+    its apical input (None = zeros).  Returns dict of stacked z, v,
+    refractory, psc, v_pre (T, B, N) and the final state."""
     T, B, n = x_seq.shape
     glif = isinstance(net.lif, GLIF3Params)
-    if glif:
+    tc = isinstance(net.lif, TwoCompParams)
+    if tc:
+        p = net.lif.to(dtype)
+    elif glif:
         g = net.lif
         p = GLIF3Params(*(None if t is None else t.to(dtype) for t in
                           (g.tau, g.v_threshold, g.v_reset, g.v_rest, g.c_m, g.tau_ref, g.k,
@@ -265,6 +333,9 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
     psc = st.get("psc", torch.zeros(B, n, dtype=dtype)).to(dtype).clone()
     iasc = (st.get("iasc", torch.zeros(B, n, n_asc, dtype=dtype)).to(dtype).clone()
             if glif else None)
+    if tc:
+        zero = torch.zeros(B, n, dtype=dtype)
+        i_a, i_bap, th = (st.get(k, zero).to(dtype).clone() for k in ("i_a", "i_bap", "theta_th"))
     ring = None
     if net.delay_mode == "hetero" and net.max_delay_steps > 1:
         ring = Ring(B, n, net.max_delay_steps, net.circular, dtype, st.get("history"),
@@ -272,10 +343,14 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
     elif net.delay_mode == "fixed" and net.max_delay_steps > 1:
         ring = Ring(B, n, net.max_delay_steps + 1, net.circular, dtype, st.get("history"),
                     st.get("cursor", 0))
-    rec = {k: [] for k in ("z", "v", "refractory", "psc", "v_pre")}
+    rec = {k: [] for k in ("z", "v", "refractory", "psc", "v_pre", "i_a", "theta_th")}
     x_seq = x_seq.to(dtype)
     for t in range(T):
-        if glif:
+        if tc:
+            xa = x_apical[t].to(dtype) if x_apical is not None else torch.zeros_like(v)
+            z, v, i_a, i_bap, th, v_pre = tc_step(v, i_a, i_bap, th, psc + x_seq[t], xa, p,
+                                                  net.dt)
+        elif glif:
             z, v, iasc, ref, v_pre = glif3_step(v, iasc, ref, psc + x_seq[t], p, net.dt)
         else:
             z, v, ref, v_pre = lif_step(v, ref, psc + x_seq[t], p, net.dt)
@@ -292,10 +367,15 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
         psc = psc + mm(zf)
         for k, val in (("z", z), ("v", v), ("refractory", ref), ("psc", psc), ("v_pre", v_pre)):
             rec[k].append(val)
-    out = {k: torch.stack(vals) for k, vals in rec.items()}
+        if tc:
+            rec["i_a"].append(i_a)
+            rec["theta_th"].append(th)
+    out = {k: torch.stack(vals) for k, vals in rec.items() if vals}
     out["final"] = {"v": v, "refractory": ref, "psc": psc}
     if glif:
         out["final"]["iasc"] = iasc
+    if tc:
+        out["final"].update(i_a=i_a, i_bap=i_bap, theta_th=th)
     if ring is not None:
         out["final"]["history"] = ring.h
         out["final"]["cursor"] = ring.cursor

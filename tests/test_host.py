@@ -1,6 +1,7 @@
 """CPU tests of the host side: config registry, module state management,
 connection layout, delay expansion, and the C ABI library's exports."""
 
+import ctypes
 import re
 from pathlib import Path
 
@@ -32,6 +33,33 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     assert declared == set(_lib.declared_symbols())
     assert lib.btx_version() >= 100
+
+
+@pytest.mark.parametrize("cname,pyname", [("btx_rsnn_plan", "RsnnPlan"),
+                                           ("btx_adj_lif_args", "AdjLifArgs")])
+def test_ctypes_structs_match_the_c_layout(tmp_path, cname, pyname):
+    """Every field offset and the size of the ctypes mirrors equal what a C
+    compiler makes of include/btorch_amd.h."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    py = getattr(_lib, pyname)
+    lines = [f'printf("size %zu\\n", sizeof({cname}));']
+    for f, _ in py._fields_:
+        lines.append(f'printf("{f} %zu\\n", offsetof({cname}, {f}));')
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"btorch_amd.h\"\n"
+                   "int main(void) {\n" + "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = dict(line.split() for line in subprocess.run(
+        [str(exe)], check=True, capture_output=True, text=True).stdout.splitlines())
+    assert int(out["size"]) == ctypes.sizeof(py)
+    for f, _ in py._fields_:
+        assert int(out[f]) == getattr(py, f).offset, f
 
 
 def test_sparse_registry_and_errors():
@@ -181,3 +209,30 @@ def test_engine_matches_supported_cells():
     assert spec.bins == [(1, 1), (3, 3)] and spec.csr_d == 5 and spec.ring_d == 5
     # an unsupported cell is not matched
     assert match_cell(RecurrentNN(LIF(n), ExponentialPSC(n, 5.0, torch.nn.Identity()))) is None
+
+
+def test_engine_matches_two_compartment_cells():
+    from btorch_amd.models import (ApicalRecurrentNN, MixedNeuronPopulation,
+                                   TwoCompartmentGLIF)
+    from btorch_amd.models._engine import match_cell
+    n = 12
+    m = scipy.sparse.random(n, n, density=0.3, format="coo", random_state=3)
+
+    def syn():
+        return ExponentialPSC(n, 5.0, SparseConn(m))
+
+    one = MixedNeuronPopulation([(n, TwoCompartmentGLIF(n))])
+    spec = match_cell(ApicalRecurrentNN(one, syn()))
+    assert spec is not None and isinstance(spec.neuron, TwoCompartmentGLIF)
+    assert spec.neuron_key == "neuron.group_0."
+    assert match_cell(RecurrentNN(MixedNeuronPopulation([(n, LIF(n))]), syn())) is not None
+    # a bare two-compartment neuron returns (spike, v), which no synapse takes
+    assert match_cell(RecurrentNN(TwoCompartmentGLIF(n), syn())) is None
+    # two groups, or an apical synapse, run the per-step loop
+    two = MixedNeuronPopulation([(6, TwoCompartmentGLIF(6)), (6, LIF(6))])
+    assert match_cell(ApicalRecurrentNN(two, syn())) is None
+    assert match_cell(ApicalRecurrentNN(one, syn(), synapse_apical=syn())) is None
+    with pytest.raises(ValueError):
+        MixedNeuronPopulation([])
+    with pytest.raises(ValueError):
+        MixedNeuronPopulation([(0, LIF(1))])

@@ -259,3 +259,156 @@ def test_delayed_autograd_restatement_matches_forward_oracle():
     assert float(ref["z"].sum()) > 20
     assert torch.equal(z, ref["z"]) and torch.equal(v, ref["final"]["v"])
     assert torch.equal(psc, ref["final"]["psc"])
+
+
+# ---------------------------------------------------------------------------
+# Two-compartment neuron (two_compartment.py:208-292)
+# ---------------------------------------------------------------------------
+
+def _tc_run(p, T, soma, apical, dt, dtype, v0=None, ia0=0.0, th0=0.0):
+    shape = (1, 4)
+    v = torch.full(shape, float(p.v_reset) if v0 is None else v0, dtype=dtype)
+    ia = torch.full(shape, ia0, dtype=dtype)
+    ib = torch.zeros(shape, dtype=dtype)
+    th = torch.full(shape, th0, dtype=dtype)
+    pp = p.to(dtype)
+    out = []
+    for t in range(T):
+        s, v, ia, ib, th, vp = rsnn.tc_step(v, ia, ib, th, torch.full(shape, soma, dtype=dtype),
+                                            torch.full(shape, apical, dtype=dtype), pp, dt)
+        out.append((s, v, ia, ib, th, vp))
+    return out
+
+
+def test_tc_linear_regime_matches_closed_form():
+    # No spike (threshold out of reach), delta_T = 0, w_Ca = 0: both
+    # compartments are first-order implicit-Euler filters, so with q = 1/(1+da),
+    # r = 1/(1+ds):  i_a_t = A + (i_a0 - A) q^t  and, with u = v - v*,
+    # v* = E_L + R_s (I + w_as A):  u_t = r^t u_0 + (1-r) R_s w_as c q (r^t - q^t)/(r - q).
+    dt, tau_a, tau_s, R, EL, w_as, I, A = 0.5, 7.0, 3.0, 1.7, -70.0, 0.8, 4.0, 2.5
+    p = rsnn.TwoCompParams.make(v_threshold=1e6, tau_a=tau_a, tau_s=tau_s, R_s=R, E_L=EL,
+                                w_as=w_as, dtype=torch.float64)
+    ia0, v0 = 1.25, -60.0
+    out = _tc_run(p, 60, I, A, dt, torch.float64, v0=v0, ia0=ia0)
+    q, r = 1 / (1 + dt / tau_a), 1 / (1 + dt / tau_s)
+    c = ia0 - A
+    vstar = EL + R * (I + w_as * A)
+    for t, (s, v, ia, ib, th, _) in enumerate(out, start=1):
+        assert float(s.max()) == 0.0 and float(ib.abs().max()) == 0.0
+        ia_cf = A + c * q ** t
+        u_cf = r ** t * (v0 - vstar) + (1 - r) * R * w_as * c * q * (r ** t - q ** t) / (r - q)
+        assert abs(float(ia[0, 0]) - ia_cf) < 1e-11
+        assert abs(float(v[0, 0]) - (vstar + u_cf)) < 1e-10
+    out32 = _tc_run(p, 60, I, A, dt, torch.float32, v0=v0, ia0=ia0)
+    assert abs(float(out32[-1][1][0, 0]) - float(out[-1][1][0, 0])) < 1e-4
+
+
+def test_tc_plateau_fixed_point():
+    # with no spikes the apical current settles at the root of
+    # i = A + w_Ca * sigmoid(i - theta_Ca)
+    from scipy.optimize import brentq
+
+    A, wca, thca = -0.3, 1.5, 0.4
+    p = rsnn.TwoCompParams.make(v_threshold=1e6, tau_a=2.0, w_Ca=wca, theta_Ca=thca,
+                                dtype=torch.float64)
+    out = _tc_run(p, 400, 0.0, A, 1.0, torch.float64)
+    root = brentq(lambda i: A + wca / (1 + math.exp(-(i - thca))) - i, -10, 10, xtol=1e-14)
+    assert abs(float(out[-1][2][0, 0]) - root) < 1e-9
+    # w_Ca = 0 removes the plateau: the current settles at A exactly
+    p0 = rsnn.TwoCompParams.make(v_threshold=1e6, tau_a=2.0, w_Ca=0.0, theta_Ca=thca,
+                                 dtype=torch.float64)
+    assert abs(float(_tc_run(p0, 400, 0.0, A, 1.0, torch.float64)[-1][2][0, 0]) - A) < 1e-12
+
+
+def test_tc_exponential_initiation_fixed_point():
+    # subthreshold, the soma settles at the lower root of
+    # v = E_L + R_s I + delta_T exp((v - v_th)/delta_T)   (argument below the clamp at 10)
+    from scipy.optimize import brentq
+
+    EL, R, I, dT, vth = -70.0, 1.0, 12.0, 2.0, -50.0
+    p = rsnn.TwoCompParams.make(v_threshold=vth, E_L=EL, R_s=R, delta_T=dT, tau_s=5.0,
+                                w_as=0.0, v_reset=-65.0, dtype=torch.float64)
+    out = _tc_run(p, 600, I, 0.0, 1.0, torch.float64, v0=-70.0)
+    f = lambda v: EL + R * I + dT * math.exp((v - vth) / dT) - v  # noqa: E731
+    root = brentq(f, -80.0, -55.0, xtol=1e-14)
+    assert all(float(o[0].max()) == 0.0 for o in out)
+    assert abs(float(out[-1][1][0, 0]) - root) < 1e-9
+    assert root > EL + R * I  # the exponential term raises the resting level
+
+
+def test_tc_spike_reset_bap_and_threshold():
+    # a spike resets v to v_reset exactly, sets i_bap = w_sa, adds delta_th to
+    # the decayed threshold, and the bAP enters the next apical update exactly
+    # as the same amount of apical input would (two_compartment.py:208-259)
+    dt, tau_th, dth, wsa = 1.0, 20.0, 5.0, 0.7
+    p = rsnn.TwoCompParams.make(v_threshold=-55.0, v_reset=-70.0, tau_s=5.0, tau_th=tau_th,
+                                delta_th=dth, w_sa=wsa, tau_a=10.0, dtype=torch.float64)
+    d = torch.float64
+    shape = (1, 1)
+    z0 = torch.zeros(shape, dtype=d)
+    v0 = torch.full(shape, -70.0, dtype=d)
+    s1, v1, ia1, ib1, th1, vp1 = rsnn.tc_step(v0, z0, z0, torch.full(shape, 2.0, dtype=d),
+                                              torch.full(shape, 200.0, dtype=d), z0, p.to(d), dt)
+    assert float(s1) == 1.0 and float(v1) == -70.0 and float(ib1) == wsa
+    assert abs(float(th1) - (2.0 / (1 + dt / tau_th) + dth)) < 1e-12
+    s2, *_ = rsnn.tc_step(v1, ia1, ib1, th1, z0, z0, p.to(d), dt)
+    assert float(s2) == 0.0
+    # bAP versus explicit apical input of the same size
+    _, _, ia_a, _, _, _ = rsnn.tc_step(v1, ia1, ib1, th1, z0, z0, p.to(d), dt)
+    _, _, ia_b, _, _, _ = rsnn.tc_step(v1, ia1, z0, th1, z0, ib1, p.to(d), dt)
+    assert abs(float(ia_a) - float(ia_b)) < 1e-15
+    # threshold decay after the spike: theta_t = theta_1 (1 + dt/tau)^-(t-1)
+    th = th1
+    v, ia, ib = v1, ia1, ib1
+    for t in range(2, 30):
+        _, v, ia, ib, th, _ = rsnn.tc_step(v, ia, ib, th, z0, z0, p.to(d), dt)
+        assert abs(float(th) - float(th1) * (1 + dt / tau_th) ** -(t - 1)) < 1e-12
+
+
+def test_tc_reference_test_behaviours():
+    # the reference's own checks (tests/models/neurons/test_two_compartment.py:28-145)
+    d, dt = torch.float32, 1.0
+    p = rsnn.TwoCompParams.make(w_Ca=2.0, theta_Ca=-0.5, w_sa=0.7, delta_th=1.5)
+    v = torch.full((3, 2), -65.0)
+    z = torch.zeros(3, 2)
+    s, v1, ia, ib, th, vp = rsnn.tc_step(v, z, z, z, torch.full((3, 2), 50.0),
+                                         torch.full((3, 2), 1.5), p, dt)
+    assert torch.all(ia > 0.0) and torch.all(p.v_threshold + z >= p.v_threshold)
+    # exponential initiation raises the pre-spike voltage
+    kw = dict(tau_s=10.0, R_s=0.2, E_L=-70.0, v_threshold=-50.0, v_reset=-65.0)
+    base = rsnn.TwoCompParams.make(delta_T=0.0, **kw)
+    boost = rsnn.TwoCompParams.make(delta_T=3.0, **kw)
+    v = torch.full((1, 1), -51.0)
+    z = torch.zeros(1, 1)
+    vb = rsnn.tc_step(v, z, z, z, torch.full((1, 1), 10.0), z, base, dt)[5]
+    vx = rsnn.tc_step(v, z, z, z, torch.full((1, 1), 10.0), z, boost, dt)[5]
+    assert float(vx) > float(vb)
+    assert d == vx.dtype
+
+
+def test_tc_host_coefficients_equal_reference_expressions():
+    # the host-side coefficients are the reference's own torch expressions,
+    # evaluated per neuron (two_compartment.py:204-259)
+    from btorch_amd import _lib
+    from btorch_amd.models import TwoCompartmentGLIF
+    from btorch_amd.models._plan import tc_coefficients
+
+    n = 257
+    g = torch.Generator().manual_seed(3)
+    kw = dict(tau_s=torch.rand(n, generator=g) * 20 + 1e-5, tau_a=torch.rand(n, generator=g) * 90,
+              tau_th=torch.rand(n, generator=g) * 400 + 100, R_s=torch.rand(n, generator=g) - 0.2,
+              delta_T=torch.rand(n, generator=g) * 2 - 1.0)
+    neuron = TwoCompartmentGLIF(n, **kw)
+    for dt in (0.1, 1.0, 0.25):
+        c = tc_coefficients(neuron, dt)
+        full = lambda t: torch.broadcast_to(t, (3, n))  # noqa: E731
+        da = dt / full(torch.clamp(neuron.tau_a, min=1e-4))
+        ds = dt / full(torch.clamp(neuron.tau_s, min=1e-4))
+        assert torch.equal(full(c[_lib.TC_DA]), da) and torch.equal(full(c[_lib.TC_DA1]), 1.0 + da)
+        assert torch.equal(full(c[_lib.TC_DS]), ds) and torch.equal(full(c[_lib.TC_DS1]), 1.0 + ds)
+        dth = dt / full(torch.clamp(neuron.tau_th, min=1e-4))
+        assert torch.equal(full(c[_lib.TC_DTH1]), 1.0 + dth)
+        dT = torch.clamp(neuron.delta_T, min=0.0)
+        assert torch.equal(c[_lib.TC_DT], dT)
+        assert torch.equal(c[_lib.TC_DTC], torch.clamp(dT, min=1e-4))
+        assert torch.equal(c[_lib.TC_R_S], torch.clamp(neuron.R_s, min=1e-4))
