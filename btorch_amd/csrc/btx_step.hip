@@ -147,6 +147,7 @@ struct StepArgs {
     int64_t n_sample;
     const int64_t* lane_t_end;
     int vec4;              // 16-byte aligned state and n % 4 == 0
+    int rec_any;           // 1 = some record, count or sample pointer is set
     // two-compartment neuron
     const float* xa;       // step t apical input (lanes, n) or NULL
     float* tia;            // state i_a, i_bap, theta_th (lanes, n)
@@ -191,8 +192,7 @@ struct PushStep {
 
 // LDS of one workgroup, shared by the two roles.
 struct RedLds {
-    int cnt[kChunk];
-    int pos[kChunk];
+    int cnt[kChunk];             // terms per neuron, then (same words) their positions
     int r[kBinCap];
     float w[kBinCap];
     uint16_t d[kBinCap];
@@ -203,7 +203,7 @@ struct PushLds {
     int slot[BTX_MAX_BINS];
     int sub[BTX_MAX_BINS];
     int tile[kMaxChunks];
-    int base[kMaxChunks];
+    uint16_t base[kMaxChunks];   // reserved position, clamped to the bin capacity
     int nev;                     // terms met in the first pass
     int er[kStage];              // staged terms: virtual row, weight, destination
     float ew[kStage];
@@ -222,6 +222,17 @@ union StepLds {
 // (stride 0); the stride is uniform, so this is a select, not a multiply.
 __device__ __forceinline__ float prm(const float* p, int64_t stride, int64_t n) {
     return stride ? p[n] : p[0];
+}
+
+// Neuron parameters requested at the start of the F role, so that their
+// loads overlap the reduction: the shared value of each (used when its
+// stride is 0).
+struct PreParams {
+    float vth, vr, cm, tref, tsyn, ksyn;
+};
+
+__device__ __forceinline__ float prm_or(const float* p, int64_t stride, int64_t n, float shared) {
+    return stride ? p[n] : shared;
 }
 
 // x / c, exactly as IEEE division, without dividing when c == 1.
@@ -265,51 +276,59 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 }
 
 // In-kernel Poisson drive of neurons n0 .. n0+3 (n0 even) of `lane` at this
-// step: a SplitMix64 stream per (lane, local step), stream =
-// mix64(seed ^ step*0x100000001b3); output q = mix64(stream + (q+1)*gamma)
-// gives neurons 2q (bits 40..63) and 2q+1 (bits 16..39) a uniform in [0, 1)
-// each; a neuron gets the kick when its uniform falls below the lane's rate.
-__device__ __forceinline__ void poisson_drive4(const StepArgs& a, int64_t lane, int64_t n0,
-                                               float drv[4]) {
-    if (a.in_prob == nullptr) return;
+// step, as a 4-bit mask (bit k = neuron n0+k gets the kick): a SplitMix64
+// stream per (lane, local step), stream = mix64(seed ^ step*0x100000001b3);
+// output q = mix64(stream + (q+1)*gamma) gives neurons 2q (bits 40..63) and
+// 2q+1 (bits 16..39) a uniform in [0, 1) each; a neuron gets the kick when
+// its uniform falls below the lane's rate.
+__device__ __forceinline__ uint32_t poisson_drive4(const StepArgs& a, int64_t lane, int64_t n0) {
+    if (a.in_prob == nullptr) return 0u;
     const uint64_t step = (uint64_t)(a.in_t0[lane] + a.t);
     const uint64_t stream = mix64((uint64_t)a.in_seed[lane] ^ (step * 0x100000001b3ull));
     const float pr = a.in_prob[lane];
+    uint32_t m = 0;
 #pragma unroll
     for (int h2 = 0; h2 < 2; ++h2) {
         const uint64_t q = (uint64_t)((n0 >> 1) + h2);
         const uint64_t h = mix64(stream + (q + 1) * 0x9e3779b97f4a7c15ull);
         const float u0 = (float)(h >> 40) * (1.0f / 16777216.0f);
         const float u1 = (float)((h >> 16) & 0xffffffull) * (1.0f / 16777216.0f);
-        drv[2 * h2] = u0 < pr ? a.in_kick : 0.f;
-        drv[2 * h2 + 1] = u1 < pr ? a.in_kick : 0.f;
+        m |= (u0 < pr ? 1u : 0u) << (2 * h2);
+        m |= (u1 < pr ? 1u : 0u) << (2 * h2 + 1);
     }
+    return m;
 }
 
 // Somatic input of a neuron: the stored psc (decayed current of the previous
 // step) plus the previous step's summed push `wz`, plus x and the drive.
+template <bool REC>
 __device__ __forceinline__ float soma_input(const StepArgs& a, int64_t i, float& psc, float wz,
                                            float xin, float drive) {
     if (a.prev) {
         psc = psc + wz;
-        if (a.rec_psc_prev) a.rec_psc_prev[i] = psc;
+        if (REC && a.rec_psc_prev) a.rec_psc_prev[i] = psc;
     }
     xin = xin + drive;
     return a.psc ? psc + xin : xin;
 }
 
-// Records, spike counts and membrane samples of a neuron after its update,
-// then the synapse's exponential decay of psc.
+// Records, spike counts and membrane samples of a neuron after its update
+// (REC: the launch writes any of them), then the synapse's exponential decay
+// of psc.
+template <bool REC>
 __device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, int64_t nidx,
-                                             int64_t i, float s, float v, float& psc) {
-    if (a.rec_v) a.rec_v[i] = v;
-    const bool live = a.lane_t_end == nullptr || a.in_t0[lane] + a.t < a.lane_t_end[lane];
-    if (a.counts && s != 0.f && live) a.counts[i] += 1;
-    if (a.rec_sample && nidx < a.n_sample && live) a.rec_sample[lane * a.n_sample + nidx] = v;
-    if (a.rec_z) a.rec_z[i] = s;
+                                             int64_t i, float s, float v, float& psc,
+                                             const PreParams& q) {
+    if (REC) {
+        if (a.rec_v) a.rec_v[i] = v;
+        const bool live = a.lane_t_end == nullptr || a.in_t0[lane] + a.t < a.lane_t_end[lane];
+        if (a.counts && s != 0.f && live) a.counts[i] += 1;
+        if (a.rec_sample && nidx < a.n_sample && live) a.rec_sample[lane * a.n_sample + nidx] = v;
+        if (a.rec_z) a.rec_z[i] = s;
+    }
     if (a.psc) {
-        const float tsyn = prm(a.tsyn, a.tsyn_s, nidx);
-        const float k = prm(a.ksyn, a.ksyn_s, nidx);
+        const float tsyn = prm_or(a.tsyn, a.tsyn_s, nidx, q.tsyn);
+        const float k = prm_or(a.ksyn, a.ksyn_s, nidx, q.ksyn);
         psc = psc + k * ((-psc) / tsyn);
     }
 }
@@ -317,16 +336,17 @@ __device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, in
 // One LIF or GLIF3 neuron of the fused step.  `psc` enters as the stored psc
 // and leaves as this step's decayed current.  `ia` holds the neuron's
 // after-spike currents (GLIF3).  Returns the spike (0/1).
-template <int KIND>
+template <int KIND, bool REC>
 __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
                                                 int64_t i, float& psc, float wz, float xin,
-                                                float drive, float& v, float& ref, float* ia) {
-    const float I = soma_input(a, i, psc, wz, xin, drive);
+                                                float drive, float& v, float& ref, float* ia,
+                                                const PreParams& q) {
+    const float I = soma_input<REC>(a, i, psc, wz, xin, drive);
     const float tau = prm(a.tau, a.tau_s, nidx);
-    const float vth = prm(a.vth, a.vth_s, nidx);
-    const float vr = prm(a.vr, a.vr_s, nidx);
-    const float cm = prm(a.cm, a.cm_s, nidx);
-    if (a.rec_v_prev) a.rec_v_prev[i] = v;
+    const float vth = prm_or(a.vth, a.vth_s, nidx, q.vth);
+    const float vr = prm_or(a.vr, a.vr_s, nidx, q.vr);
+    const float cm = prm_or(a.cm, a.cm_s, nidx, q.cm);
+    if (REC && a.rec_v_prev) a.rec_v_prev[i] = v;
     float v1;
     if (KIND == kGlif3) {
         // charge (exponential Euler, Iasc of the previous step)
@@ -337,30 +357,30 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         const float dv = (-(v - prm(a.vrest, a.vrest_s, nidx))) / tau + div_by(I + isum, cm);
         v1 = v + prm(a.kv, a.kv_s, nidx) * dv;
         // adaptation
-        const int64_t q = nidx * a.n_asc;
+        const int64_t qa = nidx * a.n_asc;
 #pragma unroll
         for (int j = 0; j < kMaxAsc; ++j)
-            if (j < a.n_asc) ia[j] = ia[j] + a.asc_kk[q + j] * ((-a.asc_k[q + j]) * ia[j]);
+            if (j < a.n_asc) ia[j] = ia[j] + a.asc_kk[qa + j] * ((-a.asc_k[qa + j]) * ia[j]);
     } else {
         const float dv = (-(v - vr)) / tau + div_by(I, cm);
         v1 = v + a.dt * dv;
     }
-    if (a.rec_v_pre) a.rec_v_pre[i] = v1;
+    if (REC && a.rec_v_pre) a.rec_v_pre[i] = v1;
     float s = fires(v1 - vth, vth - vr) ? 1.f : 0.f;
     if (a.use_ref) {
         s = s * ((ref == 0.f) ? 1.f : 0.f);
-        float r2 = (ref + s * prm(a.tref, a.tref_s, nidx)) - a.dt;
+        float r2 = (ref + s * prm_or(a.tref, a.tref_s, nidx, q.tref)) - a.dt;
         ref = (r2 < 0.f) ? 0.f : r2;
-        if (a.rec_ref) a.rec_ref[i] = ref;
+        if (REC && a.rec_ref) a.rec_ref[i] = ref;
     }
     v = a.hard ? (v1 - (v1 - vr) * s) : (v1 - (vth - vr) * s);
     if (KIND == kGlif3) {
-        const int64_t q = nidx * a.n_asc;
+        const int64_t qa = nidx * a.n_asc;
 #pragma unroll
         for (int j = 0; j < kMaxAsc; ++j)
-            if (j < a.n_asc) ia[j] = ia[j] + a.asc_amp[q + j] * s;
+            if (j < a.n_asc) ia[j] = ia[j] + a.asc_amp[qa + j] * s;
     }
-    step_outputs(a, lane, nidx, i, s, v, psc);
+    step_outputs<REC>(a, lane, nidx, i, s, v, psc, q);
     return s;
 }
 
@@ -369,13 +389,14 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
 // exponential initiation term, Heaviside spike at the adaptive threshold,
 // hard reset, bAP current for the next step, threshold decay and increment.
 // `xa` is the apical input.  Returns the spike (0/1).
+template <bool REC>
 __device__ __forceinline__ float two_comp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
                                                  int64_t i, float& psc, float wz, float xin,
                                                  float drive, float xa, float& v, float& ia,
-                                                 float& ibap, float& th) {
+                                                 float& ibap, float& th, const PreParams& q) {
 #define TCP(k) prm(a.tcp[k], a.tcs[k], nidx)
-    const float I = soma_input(a, i, psc, wz, xin, drive);
-    if (a.rec_v_prev) a.rec_v_prev[i] = v;
+    const float I = soma_input<REC>(a, i, psc, wz, xin, drive);
+    if (REC && a.rec_v_prev) a.rec_v_prev[i] = v;
     // apical current: w_Ca * sigmoid(i_a - theta_Ca) is +-0 for w_Ca == 0
     // whenever the sigmoid is a number, so it is only evaluated otherwise
     const float wca = TCP(BTX_TC_W_CA);
@@ -385,7 +406,7 @@ __device__ __forceinline__ float two_comp_neuron(const StepArgs& a, int64_t lane
                                                  : wca * (1.f / (1.f + expf(-z)));
     const float ia1 = (ia + TCP(BTX_TC_DA) * ((xa + plateau) + ibap)) / TCP(BTX_TC_DA1);
     // soma
-    const float thr = prm(a.vth, a.vth_s, nidx) + th;
+    const float thr = prm_or(a.vth, a.vth_s, nidx, q.vth) + th;
     const float total = I + TCP(BTX_TC_W_AS) * ia1;
     const float dT = TCP(BTX_TC_DT);
     float ex = 0.f;
@@ -397,17 +418,19 @@ __device__ __forceinline__ float two_comp_neuron(const StepArgs& a, int64_t lane
     const float vp =
         (v + TCP(BTX_TC_DS) * ((TCP(BTX_TC_E_L) + TCP(BTX_TC_R_S) * total) + ex)) /
         TCP(BTX_TC_DS1);
-    if (a.rec_v_pre) a.rec_v_pre[i] = vp;
+    if (REC && a.rec_v_pre) a.rec_v_pre[i] = vp;
     const float s = (vp - thr) >= 0.f ? 1.f : 0.f;
     ia = ia1;
-    v = vp - (vp - prm(a.vr, a.vr_s, nidx)) * s;
+    v = vp - (vp - prm_or(a.vr, a.vr_s, nidx, q.vr)) * s;
     ibap = TCP(BTX_TC_W_SA) * s;
     th = th / TCP(BTX_TC_DTH1) + TCP(BTX_TC_DELTA_TH) * s;
 #undef TCP
-    if (a.rec_ia) a.rec_ia[i] = ia;
-    if (a.rec_ibap) a.rec_ibap[i] = ibap;
-    if (a.rec_th) a.rec_th[i] = th;
-    step_outputs(a, lane, nidx, i, s, v, psc);
+    if (REC) {
+        if (a.rec_ia) a.rec_ia[i] = ia;
+        if (a.rec_ibap) a.rec_ibap[i] = ibap;
+        if (a.rec_th) a.rec_th[i] = th;
+    }
+    step_outputs<REC>(a, lane, nidx, i, s, v, psc, q);
     return s;
 }
 
@@ -509,12 +532,12 @@ __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int
     int base = block_excl_scan<kThreads / 64>(sum, total);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        L.pos[tid * 4 + k] = base;
+        L.cnt[tid * 4 + k] = base;  // a thread rewrites only the counts it read
         base += v[k];
     }
     __syncthreads();
     for (int i = tid; i < c; i += kThreads) {
-        const int p = atomicAdd(&L.pos[L.d[i]], 1);
+        const int p = atomicAdd(&L.cnt[L.d[i]], 1);
         L.ord[p] = (uint16_t)i;
     }
     __syncthreads();
@@ -523,7 +546,7 @@ __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int
         const int cd = v[k];
         if (cd == 0 || n0 + k >= n) continue;
         if (cd <= a.sort_max) {
-            const int end = L.pos[tid * 4 + k], start = end - cd;
+            const int end = L.cnt[tid * 4 + k], start = end - cd;
             // insertion sort of the neuron's terms by virtual row
             for (int x = start + 1; x < end; ++x) {
                 const int key = L.ord[x];
@@ -586,12 +609,10 @@ __device__ __forceinline__ void store_lane4(float* p, int64_t i0, int64_t room, 
         if (k < room) p[i0 + k] = v[k];
 }
 
-// Whether any of four values changed bit pattern (a sign of zero counts).
-__device__ __forceinline__ bool moved4(const float* a, const float* b) {
-    uint32_t d = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) d |= __float_as_uint(a[k]) ^ __float_as_uint(b[k]);
-    return d != 0;
+// Whether any of four values has a bit set (-0.0 counts as set).
+__device__ __forceinline__ bool any4(const float* a) {
+    return (__float_as_uint(a[0]) | __float_as_uint(a[1]) | __float_as_uint(a[2]) |
+            __float_as_uint(a[3])) != 0u;
 }
 
 template <int KIND>
@@ -636,24 +657,50 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
 #pragma unroll
         for (int j = 0; j < (GLIF ? kMaxAsc : 1); ++j)
             ia[k][j] = (GLIF && j < a.n_asc && k < room) ? a.iasc[(i0 + k) * a.n_asc + j] : 0.f;
-    const float r0[4] = {r[0], r[1], r[2], r[3]};
-    const float b0[4] = {tb[0], tb[1], tb[2], tb[3]};
+    // the refractory counter and the bAP current are mostly 0 before and
+    // after the step: they are written back only where either is not
+    const bool ref_live = any4(r);
+    const bool bap_live = TC && any4(tb);
+    // shared parameters and the background drive before the reduction, so
+    // that their loads and the hashing overlap its waits
+    PreParams q;
+    q.vth = a.vth[0];
+    q.vr = a.vr[0];
+    q.cm = TC ? 1.f : a.cm[0];
+    q.tref = a.use_ref ? a.tref[0] : 0.f;
+    q.tsyn = a.psc ? a.tsyn[0] : 1.f;
+    q.ksyn = a.psc ? a.ksyn[0] : 0.f;
+    const uint32_t kicks = poisson_drive4(a, lane, n0);
     float wz[4] = {0.f, 0.f, 0.f, 0.f};
     if (a.prev) reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz);
-    float drv[4] = {0.f, 0.f, 0.f, 0.f};
-    poisson_drive4(a, lane, n0, drv);
     uint32_t nib = 0;
+    // the record-free variant only for LIF (GLIF3 and the two-compartment
+    // neuron keep one variant, to hold their register count)
+    if (KIND != kLif || a.rec_any) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (k < room) {
-            float s;
-            if (TC)
-                s = two_comp_neuron(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv[k], xa[k],
-                                    v[k], ta[k], tb[k], tt[k]);
-            else
-                s = lif_exp_neuron<KIND>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv[k],
-                                         v[k], r[k], ia[k]);
-            nib |= (uint32_t)(s != 0.f) << k;
+        for (int k = 0; k < 4; ++k) {
+            if (k < room) {
+                const float drv = ((kicks >> k) & 1u) ? a.in_kick : 0.f;
+                const float s =
+                    TC ? two_comp_neuron<true>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv,
+                                               xa[k], v[k], ta[k], tb[k], tt[k], q)
+                       : lif_exp_neuron<KIND, true>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k],
+                                                    drv, v[k], r[k], ia[k], q);
+                nib |= (uint32_t)(s != 0.f) << k;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < room) {
+                const float drv = ((kicks >> k) & 1u) ? a.in_kick : 0.f;
+                const float s =
+                    TC ? two_comp_neuron<false>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv,
+                                                xa[k], v[k], ta[k], tb[k], tt[k], q)
+                       : lif_exp_neuron<KIND, false>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k],
+                                                     drv, v[k], r[k], ia[k], q);
+                nib |= (uint32_t)(s != 0.f) << k;
+            }
         }
     }
     if (GLIF) {
@@ -665,13 +712,11 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     }
     if (room > 0) {
         store_lane4(a.v, i0, room, vec, v);
-        // the refractory counter and the bAP current mostly stay 0: they are
-        // written only where they moved
-        if (a.use_ref && moved4(r, r0)) store_lane4(a.ref, i0, room, vec, r);
+        if (a.use_ref && (ref_live || any4(r))) store_lane4(a.ref, i0, room, vec, r);
         if (a.psc) store_lane4(a.psc, i0, room, vec, p);
         if (TC) {
             store_lane4(a.tia, i0, room, vec, ta);
-            if (moved4(tb, b0)) store_lane4(a.tibap, i0, room, vec, tb);
+            if (bap_live || any4(tb)) store_lane4(a.tibap, i0, room, vec, tb);
             store_lane4(a.tth, i0, room, vec, tt);
         }
     }
@@ -795,7 +840,8 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
             for (int t = tid; t < a.nchunks; t += kThreads) {
                 const int ct = L.tile[t];
                 if (ct) {
-                    L.base[t] = atomicAdd(&a.bin_cnt[lane * a.nchunks + t], ct);
+                    const int b = atomicAdd(&a.bin_cnt[lane * a.nchunks + t], ct);
+                    L.base[t] = (uint16_t)(b < a.cap ? b : a.cap);
                     L.tile[t] = 0;
                 }
             }
@@ -824,7 +870,7 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 // Workgroups [0, np) of a row play P (the longer chain, dispatched first),
 // [np, gridDim.x) play F.
 template <int G, bool ORDERED, int KIND>
-__global__ __launch_bounds__(kThreads) void k_step(StepArgs a, PushStep p, int np) {
+__global__ __launch_bounds__(kThreads, KIND == kLif ? 8 : 1) void k_step(StepArgs a, PushStep p, int np) {
     __shared__ StepLds lds;
     const int64_t lane = blockIdx.y;
     if ((int)blockIdx.x < np) p_role<G, ORDERED>(p, lds.push, blockIdx.x, lane);
@@ -1009,6 +1055,12 @@ static int check_neuron(const btx_rsnn_plan* p) {
     return 0;
 }
 
+static int set_rec_any(const StepArgs& a) {
+    return (a.rec_z || a.rec_v || a.rec_ref || a.rec_psc_prev || a.rec_v_pre || a.rec_v_prev ||
+            a.counts || a.rec_sample || a.rec_ia || a.rec_ibap || a.rec_th)
+               ? 1 : 0;
+}
+
 // 16-byte accesses are possible when every state array is aligned and n % 4 == 0.
 static int vec4_ok(const btx_rsnn_plan* p, const float* x) {
     bool ok = (p->n % 4 == 0) && aligned16(p->v) && (!p->psc || aligned16(p->psc)) &&
@@ -1038,6 +1090,7 @@ BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float
     a.rec_v_pre = v_pre;
     a.xa = a.kind == kTwoComp ? p->x_apical : nullptr;
     a.vec4 = vec4_ok(p, x);
+    a.rec_any = set_rec_any(a);
     const dim3 grid((unsigned)a.nchunks, (unsigned)p->lanes);
     const hipStream_t s = (hipStream_t)stream;
     if (a.kind == kGlif3) launch_neuron_only<kGlif3>(a, grid, s);
@@ -1310,6 +1363,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
             a.rec_ibap = p->rec_i_bap ? p->rec_i_bap + t * rec_step : nullptr;
             a.rec_th = p->rec_theta_th ? p->rec_theta_th + t * rec_step : nullptr;
         }
+        a.rec_any = set_rec_any(a);
         a.rec_psc_prev = (p->rec_psc && t > 0) ? p->rec_psc + (t - 1) * rec_step : nullptr;
         ps.clock = p->ring_cursor + t;
         ps.t = t;
