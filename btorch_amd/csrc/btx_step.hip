@@ -218,6 +218,15 @@ union StepLds {
 // Neuron update
 // ---------------------------------------------------------------------------
 
+// A value that no workgroup of the launch writes before reading it (launch
+// arguments, parameters, the previous step's bin count), read through the
+// constant address space: a uniform address then becomes a scalar load,
+// whose wait does not wait for the vector loads issued around it.
+template <typename T>
+__device__ __forceinline__ T ld_const(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // Parameter of neuron n: per-neuron array (stride 1) or one shared value
 // (stride 0); the stride is uniform, so this is a select, not a multiply.
 __device__ __forceinline__ float prm(const float* p, int64_t stride, int64_t n) {
@@ -281,11 +290,28 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 // output q = mix64(stream + (q+1)*gamma) gives neurons 2q (bits 40..63) and
 // 2q+1 (bits 16..39) a uniform in [0, 1) each; a neuron gets the kick when
 // its uniform falls below the lane's rate.
-__device__ __forceinline__ uint32_t poisson_drive4(const StepArgs& a, int64_t lane, int64_t n0) {
+struct DriveArgs {
+    uint64_t seed;
+    int64_t t0;
+    float prob;
+};
+
+// The lane's seed, clock and rate (loaded ahead of the state).
+__device__ __forceinline__ DriveArgs drive_args(const StepArgs& a, int64_t lane) {
+    DriveArgs d{0, 0, 0.f};
+    if (a.in_prob == nullptr) return d;
+    d.seed = (uint64_t)ld_const(a.in_seed + lane);
+    d.t0 = ld_const(a.in_t0 + lane);
+    d.prob = ld_const(a.in_prob + lane);
+    return d;
+}
+
+__device__ __forceinline__ uint32_t poisson_drive4(const StepArgs& a, const DriveArgs& d,
+                                                   int64_t n0) {
     if (a.in_prob == nullptr) return 0u;
-    const uint64_t step = (uint64_t)(a.in_t0[lane] + a.t);
-    const uint64_t stream = mix64((uint64_t)a.in_seed[lane] ^ (step * 0x100000001b3ull));
-    const float pr = a.in_prob[lane];
+    const uint64_t step = (uint64_t)(d.t0 + a.t);
+    const uint64_t stream = mix64(d.seed ^ (step * 0x100000001b3ull));
+    const float pr = d.prob;
     uint32_t m = 0;
 #pragma unroll
     for (int h2 = 0; h2 < 2; ++h2) {
@@ -473,8 +499,30 @@ __device__ float pull_sum(const PullArgs& a, int64_t lane, int64_t dst) {
 
 // s4[k] = summed input of neuron seg*1024 + tid*4 + k of `lane` (+0.0f when
 // it received none).  Every thread of the workgroup calls it.
+// The requests that open the reduction of a segment: the bin's term count
+// and its first kThreads terms (slots past the count hold stale terms and
+// are ignored).  Issued ahead of the state loads, so that waiting for them
+// does not wait for the state.
+struct RedHead {
+    int c;
+    int d, r;
+    float w;
+};
+
+__device__ __forceinline__ RedHead reduce_head(const ReduceIn& a, int64_t nchunks, int64_t lane,
+                                               int64_t seg) {
+    RedHead h{0, 0, 0, 0.f};
+    if (!a.ordered) return h;
+    const int64_t q0 = (lane * nchunks + seg) * (int64_t)kBinCap + threadIdx.x;
+    h.d = a.ev_d[q0];
+    h.r = a.ev_r[q0];
+    h.w = a.ev_w[q0];
+    h.c = ld_const(a.bin_cnt + lane * nchunks + seg);
+    return h;
+}
+
 __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int64_t n, int64_t nchunks,
-                               int64_t lane, int64_t seg, float s4[4]) {
+                               int64_t lane, int64_t seg, float s4[4], const RedHead& h) {
     const int tid = threadIdx.x;
     const int64_t n0 = seg * kChunk + tid * 4;
 #pragma unroll
@@ -494,12 +542,9 @@ __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int
     }
     int* cntp = a.bin_cnt + lane * nchunks + seg;
     const int64_t q0 = (lane * nchunks + seg) * (int64_t)kBinCap;
-    // the first kThreads terms are requested together with the count (the
-    // slots past the count hold stale terms and are ignored)
-    const int d_pre = a.ev_d[q0 + tid];
-    const int r_pre = a.ev_r[q0 + tid];
-    const float w_pre = a.ev_w[q0 + tid];
-    const int c = *cntp;
+    const int d_pre = h.d, r_pre = h.r;
+    const float w_pre = h.w;
+    const int c = h.c;
     if (c == 0) return;  // uniform across the workgroup
 #pragma unroll
     for (int k = 0; k < 4; ++k) L.cnt[tid * 4 + k] = 0;
@@ -567,6 +612,12 @@ __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int
     }
 }
 
+__device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int64_t n,
+                                               int64_t nchunks, int64_t lane, int64_t seg,
+                                               float s4[4]) {
+    reduce_segment(a, L, n, nchunks, lane, seg, s4, reduce_head(a, nchunks, lane, seg));
+}
+
 // ---------------------------------------------------------------------------
 // F role
 // ---------------------------------------------------------------------------
@@ -587,15 +638,17 @@ __device__ __forceinline__ void write_segment(uint32_t bits, int64_t seg_base,
 }
 
 // State of neurons n0 .. n0+3 of a lane (`room` of them exist): one 16-byte
-// access when `vec`, else per neuron; neurons past n read as 0.
-__device__ __forceinline__ void load_lane4(const float* p, int64_t i0, int64_t room, bool vec,
-                                           float* o) {
-    if (vec && room >= 4) {
-        load4(p + i0, o);
+// access when `vec` (uniform), else one per neuron.  Neurons past n read the
+// lane's first element `safe` instead (their values are never used), so no
+// lane branches and the loads of several arrays stay in flight together.
+__device__ __forceinline__ void load_lane4(const float* p, int64_t i0, int64_t room, int64_t safe,
+                                           bool vec, float* o) {
+    if (vec) {
+        load4(p + (room > 0 ? i0 : safe), o);
         return;
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) o[k] = k < room ? p[i0 + k] : 0.f;
+    for (int k = 0; k < 4; ++k) o[k] = p[k < room ? i0 + k : safe];
 }
 
 __device__ __forceinline__ void store_lane4(float* p, int64_t i0, int64_t room, bool vec,
@@ -634,22 +687,32 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     const int64_t room = a.n - n0;
     const bool vec = a.vec4 != 0;
     const int64_t i0 = lane * a.n + n0;
-    // state loads first, so they are in flight during the reduction
+    // requests in the order they are waited for: the bin head, the shared
+    // parameters and drive arguments, then the state
+    RedHead head{0, 0, 0, 0.f};
+    if (a.prev) head = reduce_head(a.red, a.nchunks, lane, seg);
+    PreParams q;
+    q.vth = ld_const(a.vth);
+    q.vr = ld_const(a.vr);
+    q.cm = TC ? 1.f : ld_const(a.cm);
+    q.tref = a.use_ref ? ld_const(a.tref) : 0.f;
+    q.tsyn = a.psc ? ld_const(a.tsyn) : 1.f;
+    q.ksyn = a.psc ? ld_const(a.ksyn) : 0.f;
+    const DriveArgs dargs = drive_args(a, lane);
+    const int64_t safe = lane * a.n;
     float v[4] = {0.f, 0.f, 0.f, 0.f}, p[4] = {0.f, 0.f, 0.f, 0.f};
     float r[4] = {0.f, 0.f, 0.f, 0.f}, x[4] = {0.f, 0.f, 0.f, 0.f};
     float ta[4] = {0.f, 0.f, 0.f, 0.f}, tb[4] = {0.f, 0.f, 0.f, 0.f};
     float tt[4] = {0.f, 0.f, 0.f, 0.f}, xa[4] = {0.f, 0.f, 0.f, 0.f};
-    if (room > 0) {
-        load_lane4(a.v, i0, room, vec, v);
-        if (a.psc) load_lane4(a.psc, i0, room, vec, p);
-        if (a.use_ref) load_lane4(a.ref, i0, room, vec, r);
-        if (a.x) load_lane4(a.x, i0, room, vec, x);
-        if (TC) {
-            load_lane4(a.tia, i0, room, vec, ta);
-            load_lane4(a.tibap, i0, room, vec, tb);
-            load_lane4(a.tth, i0, room, vec, tt);
-            if (a.xa) load_lane4(a.xa, i0, room, vec, xa);
-        }
+    load_lane4(a.v, i0, room, safe, vec, v);
+    if (a.psc) load_lane4(a.psc, i0, room, safe, vec, p);
+    if (a.use_ref) load_lane4(a.ref, i0, room, safe, vec, r);
+    if (a.x) load_lane4(a.x, i0, room, safe, vec, x);
+    if (TC) {
+        load_lane4(a.tia, i0, room, safe, vec, ta);
+        load_lane4(a.tibap, i0, room, safe, vec, tb);
+        load_lane4(a.tth, i0, room, safe, vec, tt);
+        if (a.xa) load_lane4(a.xa, i0, room, safe, vec, xa);
     }
     float ia[4][GLIF ? kMaxAsc : 1];
 #pragma unroll
@@ -657,22 +720,13 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
 #pragma unroll
         for (int j = 0; j < (GLIF ? kMaxAsc : 1); ++j)
             ia[k][j] = (GLIF && j < a.n_asc && k < room) ? a.iasc[(i0 + k) * a.n_asc + j] : 0.f;
+    float wz[4] = {0.f, 0.f, 0.f, 0.f};
+    if (a.prev) reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz, head);
+    const uint32_t kicks = poisson_drive4(a, dargs, n0);
     // the refractory counter and the bAP current are mostly 0 before and
     // after the step: they are written back only where either is not
     const bool ref_live = any4(r);
     const bool bap_live = TC && any4(tb);
-    // shared parameters and the background drive before the reduction, so
-    // that their loads and the hashing overlap its waits
-    PreParams q;
-    q.vth = a.vth[0];
-    q.vr = a.vr[0];
-    q.cm = TC ? 1.f : a.cm[0];
-    q.tref = a.use_ref ? a.tref[0] : 0.f;
-    q.tsyn = a.psc ? a.tsyn[0] : 1.f;
-    q.ksyn = a.psc ? a.ksyn[0] : 0.f;
-    const uint32_t kicks = poisson_drive4(a, lane, n0);
-    float wz[4] = {0.f, 0.f, 0.f, 0.f};
-    if (a.prev) reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz);
     uint32_t nib = 0;
     // the record-free variant only for LIF (GLIF3 and the two-compartment
     // neuron keep one variant, to hold their register count)
@@ -870,7 +924,7 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 // Workgroups [0, np) of a row play P (the longer chain, dispatched first),
 // [np, gridDim.x) play F.
 template <int G, bool ORDERED, int KIND>
-__global__ __launch_bounds__(kThreads, KIND == kLif ? 8 : 1) void k_step(StepArgs a, PushStep p, int np) {
+__global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArgs a, PushStep p, int np) {
     __shared__ StepLds lds;
     const int64_t lane = blockIdx.y;
     if ((int)blockIdx.x < np) p_role<G, ORDERED>(p, lds.push, blockIdx.x, lane);
