@@ -244,11 +244,6 @@ __device__ __forceinline__ float prm_or(const float* p, int64_t stride, int64_t 
     return stride ? p[n] : shared;
 }
 
-// x / c, exactly as IEEE division, without dividing when c == 1.
-__device__ __forceinline__ float div_by(float x, float c) {
-    return c == 1.f ? x : x / c;
-}
-
 // (x >= 0) of the quotient u = a / d (d = v_th - v_reset) as the
 // reference's surrogate Heaviside sees it (surrogate/base.py:6-8).  For
 // finite d > 0 the sign of u is the sign of a, except that a tiny negative a
@@ -362,7 +357,7 @@ __device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, in
 // One LIF or GLIF3 neuron of the fused step.  `psc` enters as the stored psc
 // and leaves as this step's decayed current.  `ia` holds the neuron's
 // after-spike currents (GLIF3).  Returns the spike (0/1).
-template <int KIND, bool REC>
+template <int KIND, bool REC, bool CM1>
 __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
                                                 int64_t i, float& psc, float wz, float xin,
                                                 float drive, float& v, float& ref, float* ia,
@@ -380,7 +375,8 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
 #pragma unroll
         for (int j = 0; j < kMaxAsc; ++j)
             if (j < a.n_asc) isum = isum + ia[j];
-        const float dv = (-(v - prm(a.vrest, a.vrest_s, nidx))) / tau + div_by(I + isum, cm);
+        const float dv = (-(v - prm(a.vrest, a.vrest_s, nidx))) / tau +
+                         (CM1 ? I + isum : (I + isum) / cm);
         v1 = v + prm(a.kv, a.kv_s, nidx) * dv;
         // adaptation
         const int64_t qa = nidx * a.n_asc;
@@ -388,7 +384,7 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         for (int j = 0; j < kMaxAsc; ++j)
             if (j < a.n_asc) ia[j] = ia[j] + a.asc_kk[qa + j] * ((-a.asc_k[qa + j]) * ia[j]);
     } else {
-        const float dv = (-(v - vr)) / tau + div_by(I, cm);
+        const float dv = (-(v - vr)) / tau + (CM1 ? I : I / cm);
         v1 = v + a.dt * dv;
     }
     if (REC && a.rec_v_pre) a.rec_v_pre[i] = v1;
@@ -728,8 +724,10 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     const bool ref_live = any4(r);
     const bool bap_live = TC && any4(tb);
     uint32_t nib = 0;
-    // the record-free variant only for LIF (GLIF3 and the two-compartment
-    // neuron keep one variant, to hold their register count)
+    // variants: with records (always for GLIF3 and the two-compartment
+    // neuron, to hold their register count), and for LIF without records,
+    // with and without the division by a shared c_m of 1 (x / 1 == x)
+    const bool cm1 = a.cm_s == 0 && q.cm == 1.f;
     if (KIND != kLif || a.rec_any) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -738,8 +736,18 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
                 const float s =
                     TC ? two_comp_neuron<true>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv,
                                                xa[k], v[k], ta[k], tb[k], tt[k], q)
-                       : lif_exp_neuron<KIND, true>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k],
-                                                    drv, v[k], r[k], ia[k], q);
+                       : lif_exp_neuron<KIND, true, false>(a, lane, n0 + k, i0 + k, p[k], wz[k],
+                                                           x[k], drv, v[k], r[k], ia[k], q);
+                nib |= (uint32_t)(s != 0.f) << k;
+            }
+        }
+    } else if (cm1) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < room) {
+                const float drv = ((kicks >> k) & 1u) ? a.in_kick : 0.f;
+                const float s = lif_exp_neuron<KIND, false, true>(
+                    a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv, v[k], r[k], ia[k], q);
                 nib |= (uint32_t)(s != 0.f) << k;
             }
         }
@@ -748,11 +756,8 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
         for (int k = 0; k < 4; ++k) {
             if (k < room) {
                 const float drv = ((kicks >> k) & 1u) ? a.in_kick : 0.f;
-                const float s =
-                    TC ? two_comp_neuron<false>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv,
-                                                xa[k], v[k], ta[k], tb[k], tt[k], q)
-                       : lif_exp_neuron<KIND, false>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k],
-                                                     drv, v[k], r[k], ia[k], q);
+                const float s = lif_exp_neuron<KIND, false, false>(
+                    a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv, v[k], r[k], ia[k], q);
                 nib |= (uint32_t)(s != 0.f) << k;
             }
         }
