@@ -148,6 +148,7 @@ struct StepArgs {
     const int64_t* lane_t_end;
     int vec4;              // 16-byte aligned state and n % 4 == 0
     int rec_any;           // 1 = some record, count or sample pointer is set
+    int shared;            // 1 = v_th, v_reset, c_m, tau_ref, tau_syn, k_syn all stride 0
     // two-compartment neuron
     const float* xa;       // step t apical input (lanes, n) or NULL
     float* tia;            // state i_a, i_bap, theta_th (lanes, n)
@@ -244,6 +245,13 @@ __device__ __forceinline__ float prm_or(const float* p, int64_t stride, int64_t 
     return stride ? p[n] : shared;
 }
 
+// SHR: the launch's parameters other than tau are all shared (stride 0), so
+// the values requested at the start are used and no stride is consulted.
+template <bool SHR>
+__device__ __forceinline__ float prm_sh(const float* p, int64_t stride, int64_t n, float shared) {
+    return SHR ? shared : prm_or(p, stride, n, shared);
+}
+
 // (x >= 0) of the quotient u = a / d (d = v_th - v_reset) as the
 // reference's surrogate Heaviside sees it (surrogate/base.py:6-8).  For
 // finite d > 0 the sign of u is the sign of a, except that a tiny negative a
@@ -336,7 +344,7 @@ __device__ __forceinline__ float soma_input(const StepArgs& a, int64_t i, float&
 // Records, spike counts and membrane samples of a neuron after its update
 // (REC: the launch writes any of them), then the synapse's exponential decay
 // of psc.
-template <bool REC>
+template <bool REC, bool SHR = false>
 __device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, int64_t nidx,
                                              int64_t i, float s, float v, float& psc,
                                              const PreParams& q) {
@@ -348,8 +356,8 @@ __device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, in
         if (a.rec_z) a.rec_z[i] = s;
     }
     if (a.psc) {
-        const float tsyn = prm_or(a.tsyn, a.tsyn_s, nidx, q.tsyn);
-        const float k = prm_or(a.ksyn, a.ksyn_s, nidx, q.ksyn);
+        const float tsyn = prm_sh<SHR>(a.tsyn, a.tsyn_s, nidx, q.tsyn);
+        const float k = prm_sh<SHR>(a.ksyn, a.ksyn_s, nidx, q.ksyn);
         psc = psc + k * ((-psc) / tsyn);
     }
 }
@@ -357,16 +365,16 @@ __device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, in
 // One LIF or GLIF3 neuron of the fused step.  `psc` enters as the stored psc
 // and leaves as this step's decayed current.  `ia` holds the neuron's
 // after-spike currents (GLIF3).  Returns the spike (0/1).
-template <int KIND, bool REC, bool CM1>
+template <int KIND, bool REC, bool SHR>
 __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
                                                 int64_t i, float& psc, float wz, float xin,
                                                 float drive, float& v, float& ref, float* ia,
                                                 const PreParams& q) {
     const float I = soma_input<REC>(a, i, psc, wz, xin, drive);
     const float tau = prm(a.tau, a.tau_s, nidx);
-    const float vth = prm_or(a.vth, a.vth_s, nidx, q.vth);
-    const float vr = prm_or(a.vr, a.vr_s, nidx, q.vr);
-    const float cm = prm_or(a.cm, a.cm_s, nidx, q.cm);
+    const float vth = prm_sh<SHR>(a.vth, a.vth_s, nidx, q.vth);
+    const float vr = prm_sh<SHR>(a.vr, a.vr_s, nidx, q.vr);
+    const float cm = prm_sh<SHR>(a.cm, a.cm_s, nidx, q.cm);
     if (REC && a.rec_v_prev) a.rec_v_prev[i] = v;
     float v1;
     if (KIND == kGlif3) {
@@ -376,7 +384,7 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         for (int j = 0; j < kMaxAsc; ++j)
             if (j < a.n_asc) isum = isum + ia[j];
         const float dv = (-(v - prm(a.vrest, a.vrest_s, nidx))) / tau +
-                         (CM1 ? I + isum : (I + isum) / cm);
+                         (I + isum) / cm;
         v1 = v + prm(a.kv, a.kv_s, nidx) * dv;
         // adaptation
         const int64_t qa = nidx * a.n_asc;
@@ -384,14 +392,15 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         for (int j = 0; j < kMaxAsc; ++j)
             if (j < a.n_asc) ia[j] = ia[j] + a.asc_kk[qa + j] * ((-a.asc_k[qa + j]) * ia[j]);
     } else {
-        const float dv = (-(v - vr)) / tau + (CM1 ? I : I / cm);
+        // SHR launches with c_m == 1 skip the division (x / 1 == x)
+        const float dv = (-(v - vr)) / tau + (SHR ? I : I / cm);
         v1 = v + a.dt * dv;
     }
     if (REC && a.rec_v_pre) a.rec_v_pre[i] = v1;
     float s = fires(v1 - vth, vth - vr) ? 1.f : 0.f;
     if (a.use_ref) {
         s = s * ((ref == 0.f) ? 1.f : 0.f);
-        float r2 = (ref + s * prm_or(a.tref, a.tref_s, nidx, q.tref)) - a.dt;
+        float r2 = (ref + s * prm_sh<SHR>(a.tref, a.tref_s, nidx, q.tref)) - a.dt;
         ref = (r2 < 0.f) ? 0.f : r2;
         if (REC && a.rec_ref) a.rec_ref[i] = ref;
     }
@@ -402,7 +411,7 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         for (int j = 0; j < kMaxAsc; ++j)
             if (j < a.n_asc) ia[j] = ia[j] + a.asc_amp[qa + j] * s;
     }
-    step_outputs<REC>(a, lane, nidx, i, s, v, psc, q);
+    step_outputs<REC, SHR>(a, lane, nidx, i, s, v, psc, q);
     return s;
 }
 
@@ -726,8 +735,9 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     uint32_t nib = 0;
     // variants: with records (always for GLIF3 and the two-compartment
     // neuron, to hold their register count), and for LIF without records,
-    // with and without the division by a shared c_m of 1 (x / 1 == x)
-    const bool cm1 = a.cm_s == 0 && q.cm == 1.f;
+    // general or with every parameter but tau shared and c_m == 1 (fewer
+    // uniform values live through the loop)
+    const bool shr = a.shared && q.cm == 1.f;
     if (KIND != kLif || a.rec_any) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -741,7 +751,7 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
                 nib |= (uint32_t)(s != 0.f) << k;
             }
         }
-    } else if (cm1) {
+    } else if (shr) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (k < room) {
@@ -1063,6 +1073,8 @@ static StepArgs neuron_args(const btx_rsnn_plan* p) {
     a.tref = p->tau_ref; a.tref_s = p->tau_ref_stride;
     a.tsyn = p->tau_syn; a.tsyn_s = p->tau_syn_stride;
     a.ksyn = p->k_syn; a.ksyn_s = p->k_syn_stride;
+    a.shared = (a.vth_s == 0 && a.vr_s == 0 && a.cm_s == 0 && a.tref_s == 0 && a.tsyn_s == 0 &&
+                a.ksyn_s == 0) ? 1 : 0;
     a.v = p->v;
     a.ref = p->refractory;
     a.kind = p->neuron_model == BTX_NEURON_GLIF3             ? kGlif3
