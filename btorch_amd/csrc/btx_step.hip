@@ -74,6 +74,7 @@ struct PullArgs {
     int64_t nwords;
     int R;
     int64_t clock;               // cursor + step of the push being reduced
+    int slot0;                   // clock mod R
     int csr_d;
     const int* age_of_sub;       // [csr_d] ring age by virtual-row offset, -1 = not read
 };
@@ -470,40 +471,50 @@ __device__ __forceinline__ float two_comp_neuron(const StepArgs& a, int64_t lane
 // ---------------------------------------------------------------------------
 
 // Virtual row r = src*csr_d + sub is active in the push being reduced when
-// neuron src spiked age_of_sub[sub] steps before it (ring slot clock - age).
-__device__ __forceinline__ bool pulled_active(const PullArgs& a, int64_t lane, int r) {
+// neuron src spiked age_of_sub[sub] steps before it: ring slot
+// (clock - age) mod R = slot0 - age (+R), as 0 <= age < R.  `lane_words`
+// is the lane's row of slot 0.
+__device__ __forceinline__ bool pulled_active(const PullArgs& a, const uint64_t* lane_words,
+                                              int64_t slot_stride, int r) {
     const int src = r / a.csr_d;
     const int age = a.age_of_sub[r - src * a.csr_d];
     if (age < 0) return false;
-    int64_t slot = (a.clock - age) % a.R;
-    if (slot < 0) slot += a.R;
-    const uint64_t* row = a.ring_words + (slot * a.lanes + lane) * a.nwords;
+    int slot = a.slot0 - age;
+    slot += slot < 0 ? a.R : 0;
+    const uint64_t* row = lane_words + slot * slot_stride;
     return (row[src >> 6] >> (src & 63)) & 1ull;
 }
 
 // Input of neuron `dst` of `lane`: its active incoming edges summed in the
-// coalesced (dst, src) order from +0.0f.
+// coalesced (dst, src) order from +0.0f.  Eight edges are requested at a
+// time; the sum stays sequential in edge order.
 __device__ float pull_sum(const PullArgs& a, int64_t lane, int64_t dst) {
     float acc = 0.f;
     int64_t e = a.csc_ptr[dst];
     const int64_t e1 = a.csc_ptr[dst + 1];
-    for (; e + 4 <= e1; e += 4) {
-        const int r0 = a.csc_src[e], r1 = a.csc_src[e + 1];
-        const int r2 = a.csc_src[e + 2], r3 = a.csc_src[e + 3];
-        const bool b0 = pulled_active(a, lane, r0), b1 = pulled_active(a, lane, r1);
-        const bool b2 = pulled_active(a, lane, r2), b3 = pulled_active(a, lane, r3);
-        if (b0) acc = acc + a.csc_w[e];
-        if (b1) acc = acc + a.csc_w[e + 1];
-        if (b2) acc = acc + a.csc_w[e + 2];
-        if (b3) acc = acc + a.csc_w[e + 3];
+    const uint64_t* lw = a.ring_words + lane * a.nwords;
+    const int64_t ss = a.lanes * a.nwords;
+    constexpr int U = 8;
+    for (; e + U <= e1; e += U) {
+        int r[U];
+        float w[U];
+        bool b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            r[u] = a.csc_src[e + u];
+            w[u] = a.csc_w[e + u];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) b[u] = pulled_active(a, lw, ss, r[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b[u]) acc = acc + w[u];
     }
     for (; e < e1; ++e)
-        if (pulled_active(a, lane, a.csc_src[e])) acc = acc + a.csc_w[e];
+        if (pulled_active(a, lw, ss, a.csc_src[e])) acc = acc + a.csc_w[e];
     return acc;
 }
 
-// s4[k] = summed input of neuron seg*1024 + tid*4 + k of `lane` (+0.0f when
-// it received none).  Every thread of the workgroup calls it.
 // The requests that open the reduction of a segment: the bin's term count
 // and its first kThreads terms (slots past the count hold stale terms and
 // are ignored).  Issued ahead of the state loads, so that waiting for them
@@ -526,6 +537,8 @@ __device__ __forceinline__ RedHead reduce_head(const ReduceIn& a, int64_t nchunk
     return h;
 }
 
+// s4[k] = summed input of neuron seg*1024 + tid*4 + k of `lane` (+0.0f when
+// it received none).  Every thread of the workgroup calls it.
 __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int64_t n, int64_t nchunks,
                                int64_t lane, int64_t seg, float s4[4], const RedHead& h) {
     const int tid = threadIdx.x;
@@ -1394,10 +1407,10 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ps.sub0 = p->bin_sub[0];
     for (int j = 1; j < p->nbins; ++j)
         if (p->bin_delay[j] != ps.delay0 + j || p->bin_sub[j] != ps.sub0 + j) ps.affine = 0;
-    // segments per P workgroup: about 64 workgroups per lane, at most
-    // kThreads segments (delay bins x segments) each
-    int cpb = (int)(nchunks / 64);
-    if (cpb < 1) cpb = 1;
+    // one segment per P workgroup (shorter, better balanced push chains than
+    // several; measured 5% faster at 8 lanes of the 140k connectome), at
+    // most kThreads (delay bin, segment) pairs each
+    int cpb = 1;
     if (cpb * p->nbins > kThreads) cpb = kThreads / p->nbins;
     ps.cpb = cpb;
     const int npb = (int)((nchunks + cpb - 1) / cpb);
@@ -1412,6 +1425,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         if (a.prev) {
             bins_of((int)((t - 1) & 1), a.red);
             a.red.pull.clock = p->ring_cursor + t - 1;
+            a.red.pull.slot0 = (int)mod(a.red.pull.clock, R);
         }
         a.x = p->x ? p->x + t * p->x_step_stride : nullptr;
         a.xa = (a.kind == kTwoComp && p->x_apical) ? p->x_apical + t * p->x_apical_step_stride
@@ -1463,6 +1477,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     }
     bins_of((int)((n_steps - 1) & 1), a.red);
     a.red.pull.clock = p->ring_cursor + n_steps - 1;
+    a.red.pull.slot0 = (int)mod(a.red.pull.clock, R);
     kt.begin(3);
     hipLaunchKernelGGL(k_finish, dim3((unsigned)nchunks, (unsigned)lanes), dim3(kThreads), 0, s,
                        a, p->rec_psc ? p->rec_psc + (n_steps - 1) * rec_step : nullptr);

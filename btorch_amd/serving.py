@@ -138,6 +138,12 @@ class LanePool:
                                   device=dev)
         self.alloc = LaneAllocator(lanes)
         self.jobs: list[_Job] = []
+        # chunks enqueued but not known to be finished, and finished jobs
+        # whose results are still being copied out (stream order keeps every
+        # later use of their lanes behind those copies)
+        self.inflight: deque = deque()
+        self.max_inflight = 2
+        self.completing: deque = deque()
         from .models import environ
 
         with environ.context(dt=dt):
@@ -222,32 +228,53 @@ class LanePool:
                 if k > 0:
                     job.traces[lo:lo + k] = self.sample[:k, job.lane0:job.lane0 + job.req.batch]
                 job.steps_done += self.chunk
+            ev = torch.cuda.Event()
+            ev.record(self.stream or torch.cuda.current_stream())
+        # keep the host at most max_inflight chunks ahead of the device
+        self.inflight.append(ev)
+        while len(self.inflight) > self.max_inflight:
+            self.inflight.popleft().synchronize()
 
     @torch.no_grad()
-    def retire(self, now_fn) -> list[SimResult]:
+    def retire(self, now_fn, block: bool = False) -> list[SimResult]:
+        """Results of the jobs whose last chunk has finished on the device.
+        A job whose last chunk is enqueued gives its lanes back at once: the
+        copy of its counts and the reset of its lanes are enqueued on the
+        pool's stream, ahead of anything a later job does with them.
+        ``block``: wait for the device instead of polling."""
         done = [j for j in self.jobs if j.steps_done >= j.req.T]
-        if not done:
-            return []
-        out = []
-        ctx = torch.cuda.stream(self.stream) if self.stream is not None else _nullctx()
-        with ctx:
-            for job in done:
-                sl = slice(job.lane0, job.lane0 + job.req.batch)
-                out.append((job, self.counts[sl].clone()))
-                self._reset_lanes(sl)  # idle lanes stay quiet
-        (self.stream or torch.cuda.current_stream()).synchronize()
-        t = now_fn()
+        if done:
+            ctx = torch.cuda.stream(self.stream) if self.stream is not None else _nullctx()
+            with ctx:
+                for job in done:
+                    sl = slice(job.lane0, job.lane0 + job.req.batch)
+                    counts = self.counts[sl].clone()
+                    self._reset_lanes(sl)  # idle lanes stay quiet
+                    ev = torch.cuda.Event()
+                    ev.record(self.stream or torch.cuda.current_stream())
+                    self.completing.append((job, counts, ev))
+                    self.jobs.remove(job)
+                    self.alloc.give(job.lane0, job.req.batch)
         results = []
-        for job, counts in out:
-            self.jobs.remove(job)
-            self.alloc.give(job.lane0, job.req.batch)
+        while self.completing:
+            job, counts, ev = self.completing[0]
+            if block:
+                ev.synchronize()
+            elif not ev.query():
+                break
+            self.completing.popleft()
             results.append(SimResult(job.req.rid, counts, job.traces, job.req.arrival,
-                                     job.admitted, t))
+                                     job.admitted, now_fn()))
         return results
 
     @property
     def busy(self) -> bool:
-        return bool(self.jobs)
+        return bool(self.jobs) or bool(self.completing)
+
+    @property
+    def running(self) -> bool:
+        """Some job still has chunks to enqueue."""
+        return any(j.steps_done < j.req.T for j in self.jobs)
 
     def lanes_in_use(self) -> int:
         return self.lanes - self.alloc.free_lanes
@@ -341,14 +368,16 @@ class Server:
             queue = waiting
             ran = False
             for pool in [self.shared, *self.private.values()]:
-                if pool.busy:
+                if pool.running:
                     pool.step_chunk()
                     ran = True
+            # with nothing left to enqueue, wait for the device rather than spin
+            idle = not ran and not queue and not pending
             for key, pool in list(self.private.items()):
-                stats.results += pool.retire(now)
+                stats.results += pool.retire(now, block=idle)
                 if not pool.busy:
                     del self.private[key]
-            stats.results += self.shared.retire(now)
+            stats.results += self.shared.retire(now, block=idle)
             if not ran and pending and realtime:
                 time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))
         _sync()
@@ -384,9 +413,9 @@ def run_plain(make_pool, requests: list[SimRequest]) -> ServeStats:
             time.sleep(wait)
         pool = make_pool(req)
         pool.admit(req, now())
-        while pool.busy:
+        while pool.running:
             pool.step_chunk()
-            stats.results += pool.retire(now)
+        stats.results += pool.retire(now, block=True)
     _sync()
     stats.wall = now()
     stats.lane_steps = sum(r.traces.shape[0] * r.traces.shape[1] for r in stats.results)

@@ -45,11 +45,9 @@ def test_pool_poisson_counts_and_traces_bit_exact():
     pool = LanePool(model, lanes=6, chunk=4 * (D + 1), n_sample=7, base_rate_hz=2000.0, kick=12.0)
     req = SimRequest(0, T=53, batch=4, rate_scale=1.5, seed=99)
     assert pool.admit(req, 0.0)
-    out = []
-    while pool.busy:
+    while pool.running:
         pool.step_chunk()
-        out += pool.retire(lambda: 0.0)
-    (res,) = out
+    (res,) = pool.retire(lambda: 0.0, block=True)
     counts, traces = _oracle_job(n, D, cexp, req, pool)
     assert int(counts.sum()) > 100
     assert torch.equal(res.counts.cpu(), counts)

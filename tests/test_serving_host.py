@@ -8,7 +8,7 @@ import pytest
 import torch
 
 from btorch_amd.serving import (
-    AdmissionError, LaneAllocator, Server, ServeStats, SimRequest, SimResult,
+    AdmissionError, LaneAllocator, Server, ServeStats, SimRequest, SimResult, run_plain,
 )
 
 
@@ -59,7 +59,7 @@ class FakePool:
         for j in self.jobs:
             j[2] += self.chunk
 
-    def retire(self, now_fn):
+    def retire(self, now_fn, block=False):
         out = []
         for j in [j for j in self.jobs if j[2] >= j[0].T]:
             self.jobs.remove(j)
@@ -73,6 +73,10 @@ class FakePool:
     @property
     def busy(self):
         return bool(self.jobs)
+
+    @property
+    def running(self):
+        return any(j[2] < j[0].T for j in self.jobs)
 
 
 def test_continuous_admission_while_jobs_run():
@@ -121,3 +125,19 @@ def test_percentiles():
     s = ServeStats()
     s.results = [SimResult(i, None, None, 0.0, 0.0, float(i + 1)) for i in range(100)]
     assert s.pct(50) == 50.0 and s.pct(99) == 99.0 and s.pct(100) == 100.0
+
+
+@pytest.mark.timeout(20)
+def test_plain_path_runs_each_job_to_its_end():
+    # one job at a time: every job's chunks are enqueued, then its result is
+    # collected; the loop ends when the job has no chunk left to enqueue
+    pools = {}
+
+    def make(req):
+        return pools.setdefault(req.batch, FakePool(lanes=req.batch, chunk=10))
+
+    reqs = [SimRequest(0, T=35, batch=2), SimRequest(1, T=10, batch=3),
+            SimRequest(2, T=21, batch=2)]
+    st = run_plain(make, reqs)
+    assert [r.rid for r in st.results] == [0, 1, 2]
+    assert pools[2].chunks == 4 + 3 and pools[3].chunks == 1
