@@ -370,9 +370,8 @@ template <int KIND, bool REC, bool SHR>
 __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
                                                 int64_t i, float& psc, float wz, float xin,
                                                 float drive, float& v, float& ref, float* ia,
-                                                const PreParams& q) {
+                                                const PreParams& q, float tau) {
     const float I = soma_input<REC>(a, i, psc, wz, xin, drive);
-    const float tau = prm(a.tau, a.tau_s, nidx);
     const float vth = prm_sh<SHR>(a.vth, a.vth_s, nidx, q.vth);
     const float vr = prm_sh<SHR>(a.vr, a.vr_s, nidx, q.vr);
     const float cm = prm_sh<SHR>(a.cm, a.cm_s, nidx, q.cm);
@@ -732,6 +731,16 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
         load_lane4(a.tth, i0, room, safe, vec, tt);
         if (a.xa) load_lane4(a.xa, i0, room, safe, vec, xa);
     }
+    // the membrane time constants, per neuron (shared by the lanes) or one value
+    float tau4[4] = {1.f, 1.f, 1.f, 1.f};
+    if (!TC) {
+        if (a.tau_s) {
+            load_lane4(a.tau, n0, room, 0, vec, tau4);
+        } else {
+            const float t0 = ld_const(a.tau);
+            tau4[0] = tau4[1] = tau4[2] = tau4[3] = t0;
+        }
+    }
     float ia[4][GLIF ? kMaxAsc : 1];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -760,7 +769,8 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
                     TC ? two_comp_neuron<true>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv,
                                                xa[k], v[k], ta[k], tb[k], tt[k], q)
                        : lif_exp_neuron<KIND, true, false>(a, lane, n0 + k, i0 + k, p[k], wz[k],
-                                                           x[k], drv, v[k], r[k], ia[k], q);
+                                                           x[k], drv, v[k], r[k], ia[k], q,
+                                                           tau4[k]);
                 nib |= (uint32_t)(s != 0.f) << k;
             }
         }
@@ -770,7 +780,8 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
             if (k < room) {
                 const float drv = ((kicks >> k) & 1u) ? a.in_kick : 0.f;
                 const float s = lif_exp_neuron<KIND, false, true>(
-                    a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv, v[k], r[k], ia[k], q);
+                    a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv, v[k], r[k], ia[k], q,
+                    tau4[k]);
                 nib |= (uint32_t)(s != 0.f) << k;
             }
         }
@@ -780,7 +791,8 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
             if (k < room) {
                 const float drv = ((kicks >> k) & 1u) ? a.in_kick : 0.f;
                 const float s = lif_exp_neuron<KIND, false, false>(
-                    a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv, v[k], r[k], ia[k], q);
+                    a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv, v[k], r[k], ia[k], q,
+                    tau4[k]);
                 nib |= (uint32_t)(s != 0.f) << k;
             }
         }
@@ -1148,7 +1160,8 @@ static int set_rec_any(const StepArgs& a) {
 // 16-byte accesses are possible when every state array is aligned and n % 4 == 0.
 static int vec4_ok(const btx_rsnn_plan* p, const float* x) {
     bool ok = (p->n % 4 == 0) && aligned16(p->v) && (!p->psc || aligned16(p->psc)) &&
-              (!p->use_refractory || aligned16(p->refractory)) && (!x || aligned16(x));
+              (!p->use_refractory || aligned16(p->refractory)) && (!x || aligned16(x)) &&
+              (p->tau == nullptr || p->tau_stride == 0 || aligned16(p->tau));
     if (p->neuron_model == BTX_NEURON_TWO_COMPARTMENT)
         ok = ok && aligned16(p->i_a) && aligned16(p->i_bap) && aligned16(p->theta_th) &&
              (!p->x_apical || aligned16(p->x_apical)) &&
