@@ -153,6 +153,8 @@ _SIGNATURES = {
     "btx_compact_dense": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
     "btx_compact_bits": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
     "btx_spmv_workspace_bytes": (ctypes.c_size_t, [_c_i64, _c_i64, _c_int, _c_int]),
+    "btx_word_popcount": (_c_int, [_vp, _c_i64, _c_i64, _vp, _vp]),
+    "btx_raster_coo": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp]),
     "btx_pack_bits": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _vp]),
     "btx_unpack_bits": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _vp]),
     "btx_spmv_event": (

@@ -747,9 +747,10 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
 #pragma unroll
         for (int j = 0; j < (GLIF ? kMaxAsc : 1); ++j)
             ia[k][j] = (GLIF && j < a.n_asc && k < room) ? a.iasc[(i0 + k) * a.n_asc + j] : 0.f;
+    // the drive's hashing runs while the loads are in flight
+    const uint32_t kicks = poisson_drive4(a, dargs, n0);
     float wz[4] = {0.f, 0.f, 0.f, 0.f};
     if (a.prev) reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz, head);
-    const uint32_t kicks = poisson_drive4(a, dargs, n0);
     // the refractory counter and the bAP current are mostly 0 before and
     // after the step: they are written back only where either is not
     const bool ref_live = any4(r);

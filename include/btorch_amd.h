@@ -121,6 +121,26 @@ int btx_spmv_weight_grad(const int64_t* row_ptr, const int32_t* col, int64_t n_s
                          void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Recorded spikes in coordinate form                                        */
+/* ------------------------------------------------------------------------ */
+
+/* counts[i] = set bits of spike word i of rows x ceil(n/64) words (bits of a
+ * row's last word past n are not counted). */
+int btx_word_popcount(const uint64_t* words, int64_t rows, int64_t n, int64_t* counts,
+                      void* stream);
+
+/* The nonzeros of a 0/1 spike array given as words (rows, ceil(n/64)), in
+ * row-major order, as coordinates: for the k-th nonzero, row r and neuron j,
+ * idx_a[k] = r / row_div, idx_b[k] = r % row_div, idx_n[k] = j.  offsets is
+ * the exclusive prefix sum of btx_word_popcount's counts.  With rows = T*B
+ * and row_div = B these are the time, batch and neuron index arrays of
+ * to_sparse_repr on the dense (T, B, N) spikes
+ * (btorch/io/serialization.py:80-147, np.nonzero order). */
+int btx_raster_coo(const uint64_t* words, int64_t rows, int64_t n, int64_t row_div,
+                   const int64_t* offsets, int32_t* idx_a, int32_t* idx_b, int32_t* idx_n,
+                   void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Fused time-step engine                                                    */
 /* ------------------------------------------------------------------------ */
 

@@ -530,3 +530,17 @@ def poisson_drive(seed, prob, t0, T, n, kick) -> torch.Tensor:
             u[1::2] = ((h >> np.uint64(16)) & np.uint64(0xFFFFFF)).astype(np.float32) * scale
             out[t, b] = np.where(u[:n] < prob[b], np.float32(kick), np.float32(0.0))
     return torch.from_numpy(out)
+
+
+# ---------------------------------------------------------------------------
+# Recorded spikes in coordinate form (io/serialization.py:80-147)
+# ---------------------------------------------------------------------------
+
+def spikes_sparse_repr(z: np.ndarray, var_name: str = "spikes") -> dict:
+    """to_sparse_repr of a dense (T, B, N) array: np.nonzero's index arrays
+    (int32), the values, and the marker's count and shape."""
+    idx = np.nonzero(z)
+    out = {f"{var_name}_idx_{d}": i.astype(np.int32) for d, i in zip(("time", "batch", "neuron"), idx)}
+    out[f"{var_name}_data"] = np.asarray(z[idx])
+    out[var_name] = {"nnz": int(len(idx[0])), "original_shape": list(z.shape)}
+    return out

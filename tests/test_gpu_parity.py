@@ -588,3 +588,47 @@ def test_engine_two_compartment_nonlinear_teacher_forced():
         torch.testing.assert_close(recs[k].cpu()[good], ref[k][good], rtol=1e-5, atol=1e-5)
     # psc sums the (forced) pushes only: bit for bit everywhere
     assert bits_equal(recs["psc"], ref["psc"])
+
+
+# ---------------------------------------------------------------------------
+# Recorded spikes in coordinate form
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n,p", [(1, 0.5), (63, 0.2), (64, 0.2), (65, 0.3), (700, 0.05),
+                                 (700, 0.0), (129, 1.0)])
+def test_raster_to_sparse_matches_nonzero_order(n, p):
+    from btorch_amd import io
+
+    g = torch.Generator().manual_seed(n)
+    z = (torch.rand(7, 3, n, generator=g) < p).float()
+    ref = rsnn.spikes_sparse_repr(z.numpy())
+    words = pack_bits(z.to(DEV))
+    got = io.raster_to_sparse(words, n)
+    assert got["spikes"]["nnz"] == ref["spikes"]["nnz"]
+    for k in ("spikes_idx_time", "spikes_idx_batch", "spikes_idx_neuron"):
+        assert got[k].dtype == torch.int32
+        assert np.array_equal(got[k].cpu().numpy(), ref[k]), k
+    back = io.sparse_to_spikes(got)
+    assert torch.equal(back.cpu(), z)
+    dense = io.spikes_to_sparse(z.to(DEV))
+    assert np.array_equal(dense["spikes_idx_neuron"].cpu().numpy(), ref["spikes_idx_neuron"])
+
+
+def test_engine_raster_to_sparse():
+    # the engine's own raster: the coordinates of a recorded run equal
+    # np.nonzero of the run's dense spikes
+    from btorch_amd import io
+
+    n, B, T = 500, 2, 60
+    model = RecurrentNN(LIF(n, tau_ref=2.0), ExponentialPSC(n, 5.0, SparseConn(
+        _conn(n, n, 0.05, 3))), step_mode="m").to(DEV)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    x = torch.from_numpy(workloads.poisson_input(T, B, n, 300.0, 0.1, seed=2) * 25.0)
+    eng = model.fused_engine()
+    with environ.context(dt=0.1), torch.no_grad():
+        raster, recs = eng.run(x.to(DEV).float().contiguous(), T, B, {"z"})
+    ref = rsnn.spikes_sparse_repr(recs["z"].cpu().numpy())
+    got = io.raster_to_sparse(raster, n)
+    assert ref["spikes"]["nnz"] > 50
+    for k in ("spikes_idx_time", "spikes_idx_batch", "spikes_idx_neuron"):
+        assert np.array_equal(got[k].cpu().numpy(), ref[k]), k

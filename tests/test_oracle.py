@@ -412,3 +412,23 @@ def test_tc_host_coefficients_equal_reference_expressions():
         assert torch.equal(c[_lib.TC_DT], dT)
         assert torch.equal(c[_lib.TC_DTC], torch.clamp(dT, min=1e-4))
         assert torch.equal(c[_lib.TC_R_S], torch.clamp(neuron.R_s, min=1e-4))
+
+
+def test_spikes_sparse_repr_known_answer_and_round_trip():
+    # io/serialization.py:8-20: index arrays in row-major (np.nonzero) order
+    z = np.zeros((2, 2, 3), np.float32)
+    z[0, 1, 2] = z[1, 0, 0] = z[1, 1, 0] = z[0, 0, 1] = 1.0
+    d = rsnn.spikes_sparse_repr(z)
+    assert d["spikes_idx_time"].tolist() == [0, 0, 1, 1]
+    assert d["spikes_idx_batch"].tolist() == [0, 1, 0, 1]
+    assert d["spikes_idx_neuron"].tolist() == [1, 2, 0, 0]
+    assert d["spikes"]["nnz"] == 4 and d["spikes_idx_time"].dtype == np.int32
+    rng = np.random.default_rng(0)
+    z = (rng.random((5, 3, 130)) < 0.1).astype(np.float32)
+    d = rsnn.spikes_sparse_repr(z)
+    back = np.zeros_like(z)
+    back[d["spikes_idx_time"], d["spikes_idx_batch"], d["spikes_idx_neuron"]] = d["spikes_data"]
+    assert np.array_equal(back, z)
+    key = (d["spikes_idx_time"].astype(np.int64) * 3 + d["spikes_idx_batch"]) * 130 \
+        + d["spikes_idx_neuron"]
+    assert np.all(np.diff(key) > 0)  # strictly ascending flat index
