@@ -23,6 +23,11 @@ NEURON_LIF = 0
 NEURON_GLIF3 = 1
 NEURON_TWO_COMPARTMENT = 2
 
+SYN_EXP = 0
+SYN_ALPHA = 1
+SYN_ALPHA_BILLEH = 2
+SYN_DUAL_EXP = 3
+
 # indices into RsnnPlan.tc (BTX_TC_*)
 TC_DA, TC_DA1, TC_DS, TC_DS1, TC_DTH1, TC_R_S, TC_E_L, TC_DT, TC_DTC = range(9)
 TC_W_CA, TC_THETA_CA, TC_W_SA, TC_W_AS, TC_DELTA_TH = range(9, 14)
@@ -111,6 +116,15 @@ class RsnnPlan(ctypes.Structure):
         ("rec_i_a", _vp),
         ("rec_i_bap", _vp),
         ("rec_theta_th", _vp),
+        ("syn_kind", _c_int),
+        ("syn_h1", _vp),
+        ("syn_h2", _vp),
+        ("syn_gain", _vp), ("syn_gain_stride", _c_i64),
+        ("tau_syn2", _vp), ("tau_syn2_stride", _c_i64),
+        ("k_syn2", _vp), ("k_syn2_stride", _c_i64),
+        ("syn_a", _vp), ("syn_a_stride", _c_i64),
+        ("rec_syn_h1", _vp),
+        ("rec_syn_h2", _vp),
         ("workspace", _vp),
         ("workspace_bytes", ctypes.c_size_t),
     ]
@@ -175,6 +189,7 @@ _SIGNATURES = {
                                     ctypes.POINTER(ctypes.c_double), _c_i64]),
     "btx_lif_step": (_c_int, [ctypes.POINTER(RsnnPlan), _vp, _vp, _vp, _vp]),
     "btx_exp_decay": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "btx_syn_decay": (_c_int, [ctypes.POINTER(RsnnPlan), _vp]),
     "btx_adj_lif_step": (_c_int, [ctypes.POINTER(AdjLifArgs), _vp]),
     "btx_adj_transpose": (_c_int, [_vp, _vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp]),
     "btx_adj_wgrad": (_c_int, [_vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp, _vp]),

@@ -98,6 +98,9 @@ constexpr int kMaxAsc = 4;
 constexpr int kLif = 0;
 constexpr int kGlif3 = 1;
 constexpr int kTwoComp = 2;
+// Flag on a kind: the synapse has kinetics other than the exponential one
+// (a second state h, or two conductances), chosen by StepArgs::syn.
+constexpr int kSynX = 4;
 
 struct StepArgs {
     int64_t n;
@@ -160,6 +163,16 @@ struct StepArgs {
     float* rec_ia;         // step t or NULL
     float* rec_ibap;
     float* rec_th;
+    // synapse kinetics other than exponential (BTX_SYN_*)
+    int syn;
+    float* sh1;            // h (alpha) or g_rise (dual exponential), (lanes, n)
+    float* sh2;            // g_decay (dual exponential)
+    const float* sg;       int64_t sg_s;    // gain on the summed push
+    const float* tsyn2;    int64_t tsyn2_s;
+    const float* ksyn2;    int64_t ksyn2_s;
+    const float* sa;       int64_t sa_s;    // psc = sa*(g_decay - g_rise)
+    float* rec_sh1_prev;   // step t-1 or NULL
+    float* rec_sh2_prev;
     ReduceIn red;          // the previous step's push
 };
 
@@ -345,10 +358,9 @@ __device__ __forceinline__ float soma_input(const StepArgs& a, int64_t i, float&
 // Records, spike counts and membrane samples of a neuron after its update
 // (REC: the launch writes any of them), then the synapse's exponential decay
 // of psc.
-template <bool REC, bool SHR = false>
-__device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, int64_t nidx,
-                                             int64_t i, float s, float v, float& psc,
-                                             const PreParams& q) {
+template <bool REC>
+__device__ __forceinline__ void step_records(const StepArgs& a, int64_t lane, int64_t nidx,
+                                             int64_t i, float s, float v) {
     if (REC) {
         if (a.rec_v) a.rec_v[i] = v;
         const bool live = a.lane_t_end == nullptr || a.in_t0[lane] + a.t < a.lane_t_end[lane];
@@ -356,6 +368,13 @@ __device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, in
         if (a.rec_sample && nidx < a.n_sample && live) a.rec_sample[lane * a.n_sample + nidx] = v;
         if (a.rec_z) a.rec_z[i] = s;
     }
+}
+
+template <bool REC, bool SHR = false>
+__device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, int64_t nidx,
+                                             int64_t i, float s, float v, float& psc,
+                                             const PreParams& q) {
+    step_records<REC>(a, lane, nidx, i, s, v);
     if (a.psc) {
         const float tsyn = prm_sh<SHR>(a.tsyn, a.tsyn_s, nidx, q.tsyn);
         const float k = prm_sh<SHR>(a.ksyn, a.ksyn_s, nidx, q.ksyn);
@@ -363,15 +382,73 @@ __device__ __forceinline__ void step_outputs(const StepArgs& a, int64_t lane, in
     }
 }
 
+// Synapse states besides psc of one neuron: h (alpha kinds) or the rise and
+// decay conductances (dual exponential).
+struct Syn2 {
+    float h1, h2;
+};
+
+// Somatic input with alpha or dual-exponential kinetics.  The previous
+// step's summed push W z completes that step's synapse update
+// (synapse.py:243-253, 307-317, 394-406): alpha kinds h = h + gain*(W z),
+// the dual exponential g_r + W z, g_d + W z and psc = a*(g_d - g_r).  psc
+// then holds the current the neuron receives.
+template <bool REC>
+__device__ __forceinline__ float syn_x_input(const StepArgs& a, int64_t i, int64_t nidx,
+                                            float& psc, Syn2& s2, float wz, float xin,
+                                            float drive) {
+    if (a.prev) {
+        if (a.syn == BTX_SYN_DUAL_EXP) {
+            s2.h1 = s2.h1 + wz;
+            s2.h2 = s2.h2 + wz;
+            psc = prm(a.sa, a.sa_s, nidx) * (s2.h2 - s2.h1);
+        } else {
+            s2.h1 = s2.h1 + prm(a.sg, a.sg_s, nidx) * wz;
+        }
+        if (REC) {
+            if (a.rec_psc_prev) a.rec_psc_prev[i] = psc;
+            if (a.rec_sh1_prev) a.rec_sh1_prev[i] = s2.h1;
+            if (a.rec_sh2_prev) a.rec_sh2_prev[i] = s2.h2;
+        }
+    }
+    xin = xin + drive;
+    return psc + xin;
+}
+
+// The decay part of the next step's synapse update (conductance_charge,
+// synapse.py:239-241, 304-305, 390-392, through exp_euler_step, ode.py:50);
+// the push W z of this step completes it at the next step.
+__device__ __forceinline__ void syn_x_decay(const StepArgs& a, int64_t nidx, float& psc,
+                                           Syn2& s2) {
+    const float k = prm(a.ksyn, a.ksyn_s, nidx);
+    if (a.syn == BTX_SYN_ALPHA) {
+        const float ts = prm(a.tsyn, a.tsyn_s, nidx);
+        const float h = s2.h1;
+        psc = psc + k * ((-psc) / ts + h / ts);
+        s2.h1 = h + k * ((-h) / ts);
+    } else if (a.syn == BTX_SYN_ALPHA_BILLEH) {
+        const float h = s2.h1;
+        psc = k * psc + k * h;
+        s2.h1 = k * h;
+    } else {
+        const float tr = prm(a.tsyn, a.tsyn_s, nidx);
+        const float td = prm(a.tsyn2, a.tsyn2_s, nidx);
+        const float kd = prm(a.ksyn2, a.ksyn2_s, nidx);
+        s2.h1 = s2.h1 + k * ((-s2.h1) / tr);
+        s2.h2 = s2.h2 + kd * ((-s2.h2) / td);
+    }
+}
+
 // One LIF or GLIF3 neuron of the fused step.  `psc` enters as the stored psc
 // and leaves as this step's decayed current.  `ia` holds the neuron's
 // after-spike currents (GLIF3).  Returns the spike (0/1).
-template <int KIND, bool REC, bool SHR>
+template <int KIND, bool REC, bool SHR, bool SX = false>
 __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
                                                 int64_t i, float& psc, float wz, float xin,
                                                 float drive, float& v, float& ref, float* ia,
-                                                const PreParams& q, float tau) {
-    const float I = soma_input<REC>(a, i, psc, wz, xin, drive);
+                                                const PreParams& q, float tau, Syn2* s2 = nullptr) {
+    const float I = SX ? syn_x_input<REC>(a, i, nidx, psc, *s2, wz, xin, drive)
+                       : soma_input<REC>(a, i, psc, wz, xin, drive);
     const float vth = prm_sh<SHR>(a.vth, a.vth_s, nidx, q.vth);
     const float vr = prm_sh<SHR>(a.vr, a.vr_s, nidx, q.vr);
     const float cm = prm_sh<SHR>(a.cm, a.cm_s, nidx, q.cm);
@@ -411,7 +488,12 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         for (int j = 0; j < kMaxAsc; ++j)
             if (j < a.n_asc) ia[j] = ia[j] + a.asc_amp[qa + j] * s;
     }
-    step_outputs<REC, SHR>(a, lane, nidx, i, s, v, psc, q);
+    if (SX) {
+        step_records<REC>(a, lane, nidx, i, s, v);
+        syn_x_decay(a, nidx, psc, *s2);
+    } else {
+        step_outputs<REC, SHR>(a, lane, nidx, i, s, v, psc, q);
+    }
     return s;
 }
 
@@ -685,8 +767,10 @@ __device__ __forceinline__ bool any4(const float* a) {
             __float_as_uint(a[3])) != 0u;
 }
 
-template <int KIND>
+template <int KIND_SX>
 __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lane) {
+    constexpr int KIND = KIND_SX & ~kSynX;
+    constexpr bool SX = (KIND_SX & kSynX) != 0;
     if (a.lane_t_end != nullptr && a.in_t0[lane] + a.t >= a.lane_t_end[lane]) {
         // no trial on this lane: consume what the previous push left and keep
         // the state (the lane is reset when a job is admitted to it)
@@ -723,6 +807,11 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     float tt[4] = {0.f, 0.f, 0.f, 0.f}, xa[4] = {0.f, 0.f, 0.f, 0.f};
     load_lane4(a.v, i0, room, safe, vec, v);
     if (a.psc) load_lane4(a.psc, i0, room, safe, vec, p);
+    float h1[4] = {0.f, 0.f, 0.f, 0.f}, h2[4] = {0.f, 0.f, 0.f, 0.f};
+    if (SX) {
+        load_lane4(a.sh1, i0, room, safe, vec, h1);
+        if (a.syn == BTX_SYN_DUAL_EXP) load_lane4(a.sh2, i0, room, safe, vec, h2);
+    }
     if (a.use_ref) load_lane4(a.ref, i0, room, safe, vec, r);
     if (a.x) load_lane4(a.x, i0, room, safe, vec, x);
     if (TC) {
@@ -761,7 +850,21 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     // general or with every parameter but tau shared and c_m == 1 (fewer
     // uniform values live through the loop)
     const bool shr = a.shared && q.cm == 1.f;
-    if (KIND != kLif || a.rec_any) {
+    if (SX) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < room) {
+                const float drv = ((kicks >> k) & 1u) ? a.in_kick : 0.f;
+                Syn2 s2{h1[k], h2[k]};
+                const float s = lif_exp_neuron<KIND, true, false, true>(
+                    a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv, v[k], r[k], ia[k], q,
+                    tau4[k], &s2);
+                h1[k] = s2.h1;
+                h2[k] = s2.h2;
+                nib |= (uint32_t)(s != 0.f) << k;
+            }
+        }
+    } else if (KIND != kLif || a.rec_any) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (k < room) {
@@ -809,6 +912,10 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
         store_lane4(a.v, i0, room, vec, v);
         if (a.use_ref && (ref_live || any4(r))) store_lane4(a.ref, i0, room, vec, r);
         if (a.psc) store_lane4(a.psc, i0, room, vec, p);
+        if (SX) {
+            store_lane4(a.sh1, i0, room, vec, h1);
+            if (a.syn == BTX_SYN_DUAL_EXP) store_lane4(a.sh2, i0, room, vec, h2);
+        }
         if (TC) {
             store_lane4(a.tia, i0, room, vec, ta);
             if (bap_live || any4(tb)) store_lane4(a.tibap, i0, room, vec, tb);
@@ -966,13 +1073,16 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 // [np, gridDim.x) play F.
 template <int G, bool ORDERED, int KIND>
 __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArgs a, PushStep p, int np) {
+    // KIND: kLif, kGlif3 or kTwoComp, with kSynX for the alpha and
+    // dual-exponential synapses
     __shared__ StepLds lds;
     const int64_t lane = blockIdx.y;
     if ((int)blockIdx.x < np) p_role<G, ORDERED>(p, lds.push, blockIdx.x, lane);
     else f_role<KIND>(a, lds, (int64_t)blockIdx.x - np, lane);
 }
 
-// FIN: the last push reduced into psc (and recorded).
+// FIN: the last push reduced into the synapse (and recorded: psc in `rec`,
+// the other synapse states in a.rec_sh1_prev / a.rec_sh2_prev).
 __global__ __launch_bounds__(kThreads) void k_finish(StepArgs a, float* __restrict__ rec) {
     __shared__ StepLds lds;
     const int64_t lane = blockIdx.y, seg = blockIdx.x;
@@ -982,10 +1092,28 @@ __global__ __launch_bounds__(kThreads) void k_finish(StepArgs a, float* __restri
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (n0 + k < a.n) {
-            const int64_t i = lane * a.n + n0 + k;
-            const float q = a.psc[i] + wz[k];
-            a.psc[i] = q;
-            if (rec) rec[i] = q;
+            const int64_t nidx = n0 + k;
+            const int64_t i = lane * a.n + nidx;
+            if (a.syn == BTX_SYN_EXP) {
+                const float q = a.psc[i] + wz[k];
+                a.psc[i] = q;
+                if (rec) rec[i] = q;
+            } else if (a.syn == BTX_SYN_DUAL_EXP) {
+                const float g1 = a.sh1[i] + wz[k];
+                const float g2 = a.sh2[i] + wz[k];
+                const float q = prm(a.sa, a.sa_s, nidx) * (g2 - g1);
+                a.sh1[i] = g1;
+                a.sh2[i] = g2;
+                a.psc[i] = q;
+                if (rec) rec[i] = q;
+                if (a.rec_sh1_prev) a.rec_sh1_prev[i] = g1;
+                if (a.rec_sh2_prev) a.rec_sh2_prev[i] = g2;
+            } else {
+                const float h = a.sh1[i] + prm(a.sg, a.sg_s, nidx) * wz[k];
+                a.sh1[i] = h;
+                if (rec) rec[i] = a.psc[i];
+                if (a.rec_sh1_prev) a.rec_sh1_prev[i] = h;
+            }
         }
     }
 }
@@ -1022,6 +1150,21 @@ __global__ __launch_bounds__(256) void k_exp_decay(float* __restrict__ psc, int6
     const int64_t nidx = i % n;
     const float p = psc[i];
     psc[i] = p + prm(ksyn, ksyn_s, nidx) * ((-p) / prm(tsyn, tsyn_s, nidx));
+}
+
+// Standalone decay of an alpha or dual-exponential synapse: the same update
+// as the fused step's (psc from psc and h, then h; or both conductances).
+__global__ __launch_bounds__(256) void k_syn_decay(StepArgs a, int64_t m) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const int64_t nidx = i % a.n;
+    const bool dual = a.syn == BTX_SYN_DUAL_EXP;
+    float psc = a.psc[i];
+    Syn2 s2{a.sh1[i], dual ? a.sh2[i] : 0.f};
+    syn_x_decay(a, nidx, psc, s2);
+    a.sh1[i] = s2.h1;
+    if (dual) a.sh2[i] = s2.h2;
+    else a.psc[i] = psc;
 }
 
 // Per-run tables, written from the launch arguments by one workgroup (a
@@ -1154,7 +1297,8 @@ static int check_neuron(const btx_rsnn_plan* p) {
 
 static int set_rec_any(const StepArgs& a) {
     return (a.rec_z || a.rec_v || a.rec_ref || a.rec_psc_prev || a.rec_v_pre || a.rec_v_prev ||
-            a.counts || a.rec_sample || a.rec_ia || a.rec_ibap || a.rec_th)
+            a.counts || a.rec_sample || a.rec_ia || a.rec_ibap || a.rec_th || a.rec_sh1_prev ||
+            a.rec_sh2_prev)
                ? 1 : 0;
 }
 
@@ -1167,6 +1311,9 @@ static int vec4_ok(const btx_rsnn_plan* p, const float* x) {
         ok = ok && aligned16(p->i_a) && aligned16(p->i_bap) && aligned16(p->theta_th) &&
              (!p->x_apical || aligned16(p->x_apical)) &&
              (p->x_apical_step_stride % 4 == 0);
+    if (p->syn_kind != BTX_SYN_EXP)
+        ok = ok && aligned16(p->syn_h1) &&
+             (p->syn_kind != BTX_SYN_DUAL_EXP || aligned16(p->syn_h2));
     ok = ok && (p->x_step_stride % 4 == 0);
     return ok ? 1 : 0;
 }
@@ -1206,6 +1353,33 @@ BTX_API int btx_exp_decay(float* psc, int64_t lanes, int64_t n, const float* tau
                        (hipStream_t)stream, psc, n, m, tau_syn, tau_syn_stride, k_syn,
                        k_syn_stride);
     return check_launch("btx_exp_decay");
+}
+
+BTX_API int btx_syn_decay(const btx_rsnn_plan* p, void* stream) {
+    if (p == nullptr) return fail("btx_syn_decay: null plan");
+    const int64_t m = p->lanes * p->n;
+    if (m <= 0) return 0;
+    if (p->syn_kind <= BTX_SYN_EXP || p->syn_kind > BTX_SYN_DUAL_EXP)
+        return fail("btx_syn_decay: syn_kind must be an alpha or dual-exponential kind");
+    if (p->psc == nullptr || p->syn_h1 == nullptr || p->tau_syn == nullptr ||
+        p->k_syn == nullptr)
+        return fail("btx_syn_decay: psc, syn_h1, tau_syn and k_syn are required");
+    if (p->syn_kind == BTX_SYN_DUAL_EXP &&
+        (p->syn_h2 == nullptr || p->tau_syn2 == nullptr || p->k_syn2 == nullptr))
+        return fail("btx_syn_decay: the dual exponential synapse needs syn_h2, tau_syn2, k_syn2");
+    StepArgs a{};
+    a.n = p->n;
+    a.psc = p->psc;
+    a.syn = p->syn_kind;
+    a.sh1 = p->syn_h1;
+    a.sh2 = p->syn_h2;
+    a.tsyn = p->tau_syn; a.tsyn_s = p->tau_syn_stride;
+    a.ksyn = p->k_syn; a.ksyn_s = p->k_syn_stride;
+    a.tsyn2 = p->tau_syn2; a.tsyn2_s = p->tau_syn2_stride;
+    a.ksyn2 = p->k_syn2; a.ksyn2_s = p->k_syn2_stride;
+    hipLaunchKernelGGL(k_syn_decay, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, a, m);
+    return check_launch("btx_syn_decay");
 }
 
 BTX_API size_t btx_rsnn_workspace_bytes(int64_t lanes, int64_t n, int slots, int ordered) {
@@ -1271,6 +1445,11 @@ static void launch_step_g(const StepArgs& a, const PushStep& ps, int np, int gro
 template <bool ORDERED>
 static void launch_step_k(const StepArgs& a, const PushStep& ps, int np, int group, dim3 grid,
                           hipStream_t s) {
+    if (a.syn != BTX_SYN_EXP) {
+        if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3 | kSynX>(a, ps, np, group, grid, s);
+        else launch_step_g<ORDERED, kLif | kSynX>(a, ps, np, group, grid, s);
+        return;
+    }
     if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3>(a, ps, np, group, grid, s);
     else if (a.kind == kTwoComp) launch_step_g<ORDERED, kTwoComp>(a, ps, np, group, grid, s);
     else launch_step_g<ORDERED, kLif>(a, ps, np, group, grid, s);
@@ -1309,6 +1488,23 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         return fail("btx_rsnn_run: psc, connectivity and raster are required");
     if (p->ring_idx == nullptr || p->ring_cnt == nullptr || p->ring_words == nullptr)
         return fail("btx_rsnn_run: the ring lists and words are required");
+    if (p->tau_syn == nullptr || p->k_syn == nullptr)
+        return fail("btx_rsnn_run: tau_syn and k_syn are required");
+    if (p->syn_kind < BTX_SYN_EXP || p->syn_kind > BTX_SYN_DUAL_EXP)
+        return fail("btx_rsnn_run: unknown syn_kind");
+    if (p->syn_kind != BTX_SYN_EXP) {
+        if (p->neuron_model == BTX_NEURON_TWO_COMPARTMENT)
+            return fail("btx_rsnn_run: the two-compartment neuron runs with BTX_SYN_EXP only");
+        if (p->syn_h1 == nullptr)
+            return fail("btx_rsnn_run: this synapse kind needs the syn_h1 state");
+        if (p->syn_kind == BTX_SYN_DUAL_EXP &&
+            (p->syn_h2 == nullptr || p->tau_syn2 == nullptr || p->k_syn2 == nullptr ||
+             p->syn_a == nullptr))
+            return fail("btx_rsnn_run: the dual exponential synapse needs syn_h2, tau_syn2, "
+                        "k_syn2 and syn_a");
+        if (p->syn_kind != BTX_SYN_DUAL_EXP && p->syn_gain == nullptr)
+            return fail("btx_rsnn_run: the alpha synapses need syn_gain");
+    }
     if (p->n * (int64_t)p->csr_d >= ((int64_t)1 << 31))
         return fail("btx_rsnn_run: n*csr_d exceeds int32 row ids");
     if (p->nbins < 1 || p->nbins > BTX_MAX_BINS) return fail("btx_rsnn_run: bad nbins");
@@ -1382,6 +1578,15 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     };
     StepArgs a = neuron_args(p);
     a.psc = p->psc;
+    a.syn = p->syn_kind;
+    if (a.syn != BTX_SYN_EXP) {
+        a.sh1 = p->syn_h1;
+        a.sh2 = p->syn_h2;
+        a.sg = p->syn_gain; a.sg_s = p->syn_gain_stride;
+        a.tsyn2 = p->tau_syn2; a.tsyn2_s = p->tau_syn2_stride;
+        a.ksyn2 = p->k_syn2; a.ksyn2_s = p->k_syn2_stride;
+        a.sa = p->syn_a; a.sa_s = p->syn_a_stride;
+    }
     a.in_seed = p->in_seed;
     a.in_prob = p->in_prob;
     a.in_t0 = p->in_t0;
@@ -1462,8 +1667,10 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
             a.rec_ibap = p->rec_i_bap ? p->rec_i_bap + t * rec_step : nullptr;
             a.rec_th = p->rec_theta_th ? p->rec_theta_th + t * rec_step : nullptr;
         }
-        a.rec_any = set_rec_any(a);
         a.rec_psc_prev = (p->rec_psc && t > 0) ? p->rec_psc + (t - 1) * rec_step : nullptr;
+        a.rec_sh1_prev = (p->rec_syn_h1 && t > 0) ? p->rec_syn_h1 + (t - 1) * rec_step : nullptr;
+        a.rec_sh2_prev = (p->rec_syn_h2 && t > 0) ? p->rec_syn_h2 + (t - 1) * rec_step : nullptr;
+        a.rec_any = set_rec_any(a);
         ps.clock = p->ring_cursor + t;
         ps.t = t;
         ReduceIn cur{};
@@ -1490,6 +1697,8 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         }
     }
     bins_of((int)((n_steps - 1) & 1), a.red);
+    a.rec_sh1_prev = p->rec_syn_h1 ? p->rec_syn_h1 + (n_steps - 1) * rec_step : nullptr;
+    a.rec_sh2_prev = p->rec_syn_h2 ? p->rec_syn_h2 + (n_steps - 1) * rec_step : nullptr;
     a.red.pull.clock = p->ring_cursor + n_steps - 1;
     a.red.pull.slot0 = (int)mod(a.red.pull.clock, R);
     kt.begin(3);

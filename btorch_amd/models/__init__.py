@@ -8,7 +8,8 @@ from .neurons import GLIF3, IF, LIF, MixedNeuronPopulation, TwoCompartmentGLIF
 from .rnn import (ApicalRecurrentNN, RecurrentNN, RecurrentNNAbstract, SomaApicalRecurrentNN,
                   make_rnn)
 from .surrogate import ATan, Erf, Sigmoid, SuperSpike, SurrogateFunctionBase
-from .synapse import BasePSC, DelayedPSC, ExponentialPSC, HeterSynapsePSC
+from .synapse import (AlphaPSC, AlphaPSCBilleh, BasePSC, DelayedPSC, DualExponentialPSC,
+                      ExponentialPSC, HeterSynapsePSC)
 
 __all__ = [
     "environ", "functional", "BaseNode", "MemoryModule", "ParamBufferMixin", "StepModule",
@@ -16,4 +17,5 @@ __all__ = [
     "TwoCompartmentGLIF", "MixedNeuronPopulation", "RecurrentNN", "RecurrentNNAbstract",
     "ApicalRecurrentNN", "SomaApicalRecurrentNN", "make_rnn", "ATan", "Erf", "Sigmoid", "SuperSpike",
     "SurrogateFunctionBase", "BasePSC", "DelayedPSC", "ExponentialPSC", "HeterSynapsePSC",
+    "AlphaPSC", "AlphaPSCBilleh", "DualExponentialPSC",
 ]

@@ -56,6 +56,10 @@ def _surrogate_code(fn, require_spiking: bool = True) -> tuple[int, float, float
 
 
 def _check_supported(spec, lanes):
+    if spec.syn_keys:
+        raise NotImplementedError("the fused adjoint covers the exponential synapse (the alpha "
+                                  "and dual-exponential synapses are forward-only in the "
+                                  "fused engine)")
     neuron = spec.neuron
     if hasattr(neuron, "Iasc") or hasattr(neuron, "i_bap"):
         raise NotImplementedError("the fused adjoint covers LIF neurons (GLIF3 and the "

@@ -9,10 +9,12 @@ Supported cells (everything else runs the generic per-step loop):
            MixedNeuronPopulation of that one group (how the reference's
            ApicalRecurrentNN drives it); a one-group population of the
            other neurons too
-  synapse  ExponentialPSC(linear=SparseConn(N, N))
-           DelayedPSC(ExponentialPSC(SparseConn(N, N)), max_delay_steps=m)
-           HeterSynapsePSC(base_psc=ExponentialPSC, n_receptor=1,
+  synapse  K(linear=SparseConn(N, N))
+           DelayedPSC(K(SparseConn(N, N)), max_delay_steps=m)
+           HeterSynapsePSC(base_psc=K, n_receptor=1,
                            linear=SparseConn(N*D, N), max_delay_steps=D)
+           with kinetics K = ExponentialPSC, AlphaPSC, AlphaPSCBilleh or
+           DualExponentialPSC (the last three with LIF or GLIF3 neurons)
   input    x (T, *batch, N), optionally through neuron_inp_module, applied to
            the whole sequence at once (a SparseConn becomes one batched
            event-driven product over T*B lanes)
@@ -35,13 +37,16 @@ from .neurons.glif import GLIF3
 from .neurons.lif import LIF
 from .neurons.mixed import MixedNeuronPopulation
 from .neurons.two_compartment import TwoCompartmentGLIF
-from .synapse import DelayedPSC, ExponentialPSC, HeterSynapsePSC
+from .synapse import (AlphaPSC, AlphaPSCBilleh, DelayedPSC, DualExponentialPSC,
+                      ExponentialPSC, HeterSynapsePSC)
+
+KINETICS = (ExponentialPSC, AlphaPSC, AlphaPSCBilleh, DualExponentialPSC)
 
 
 @dataclass
 class CellSpec:
     neuron: LIF | GLIF3 | TwoCompartmentGLIF
-    exp_psc: ExponentialPSC          # module whose psc decays and receives W z
+    exp_psc: ExponentialPSC          # kinetics module: its psc decays and receives W z
     conn: BaseSparseConn
     history: SpikeHistory | None
     csr_d: int
@@ -50,6 +55,7 @@ class CellSpec:
     psc_keys: list[str]              # dotted names whose state equals psc
     psc_modules: list = field(default_factory=list)
     neuron_key: str = "neuron."      # dotted prefix of the neuron's memories
+    syn_keys: dict = field(default_factory=dict)  # dotted state name -> record key
 
 
 def _bin_nonempty(conn: BaseSparseConn, csr_d: int) -> list[bool]:
@@ -79,10 +85,10 @@ def match_cell(rnn) -> CellSpec | None:
     if getattr(rnn, "syn_inp_module", None) is not None:
         return None
     n = neuron.size
-    if isinstance(syn, ExponentialPSC):
+    if isinstance(syn, KINETICS):
         conn = syn.linear
         spec = CellSpec(neuron, syn, conn, None, 1, 1, [(0, 0)], ["synapse.psc"], [syn])
-    elif isinstance(syn, DelayedPSC) and isinstance(syn.psc_module, ExponentialPSC):
+    elif isinstance(syn, DelayedPSC) and isinstance(syn.psc_module, KINETICS):
         conn = syn.psc_module.linear
         if syn.history is None:
             spec = CellSpec(neuron, syn.psc_module, conn, None, 1, 1, [(0, 0)],
@@ -91,7 +97,7 @@ def match_cell(rnn) -> CellSpec | None:
             m = syn.max_delay_steps
             spec = CellSpec(neuron, syn.psc_module, conn, syn.history, 1, m + 1, [(m, 0)],
                             ["synapse.psc_module.psc"], [syn.psc_module])
-    elif (isinstance(syn, HeterSynapsePSC) and isinstance(syn.base_psc, ExponentialPSC)
+    elif (isinstance(syn, HeterSynapsePSC) and isinstance(syn.base_psc, KINETICS)
           and syn.n_receptor == 1):
         conn = syn.linear
         keys = ["synapse.psc", "synapse.base_psc.psc"]
@@ -116,6 +122,12 @@ def match_cell(rnn) -> CellSpec | None:
         spec.bins = [(d, d) for d in range(spec.csr_d) if ne[d]] or [(0, 0)]
     if len(spec.bins) > _lib.MAX_BINS:
         return None
+    kin = spec.exp_psc
+    if not isinstance(kin, ExponentialPSC):
+        if isinstance(neuron, TwoCompartmentGLIF):
+            return None
+        prefix = spec.psc_keys[-1][: -len("psc")]
+        spec.syn_keys = {prefix + k: f"syn_h{j + 1}" for j, k in enumerate(kin._STATES)}
     spec.neuron_key = neuron_key
     return spec
 
@@ -166,12 +178,10 @@ class Engine:
         p.ordered = int(cfg.ordered)
         p.slots = int(cfg.slots)
         p.bin_cap = int(getattr(cfg, "bin_cap", 0))
-        tau, ts, k, ks = syn.decay_coefficients(dev)
-        keep += [tau, k]
-        p.tau_syn, p.tau_syn_stride = tau.data_ptr(), ts
-        p.k_syn, p.k_syn_stride = k.data_ptr(), ks
+        syn.synapse_plan(p, dev, keep)
         # state
-        for mod, name in ((neuron, "v"), (syn, "psc")):
+        states = [(neuron, "v"), (syn, "psc")] + [(syn, k) for k in getattr(syn, "_STATES", ())]
+        for mod, name in states:
             t = getattr(mod, name)
             if t is None or t.numel() != lanes * n or t.dtype != torch.float32:
                 raise RuntimeError(f"{type(mod).__name__}.{name} is not initialised for "
@@ -181,6 +191,10 @@ class Engine:
         p.v = neuron.v.data_ptr()
         p.psc = syn.psc.data_ptr()
         keep += [neuron.v, syn.psc]
+        for j, k in enumerate(getattr(syn, "_STATES", ())):
+            t = getattr(syn, k)
+            setattr(p, f"syn_h{j + 1}", t.data_ptr())
+            keep.append(t)
         if getattr(neuron, "_use_refractory", False):
             if not neuron.refractory.is_contiguous():
                 neuron.refractory = neuron.refractory.contiguous()
@@ -314,7 +328,7 @@ class Engine:
             p.teacher = teacher.data_ptr()
         recs = {}
         for key in ("z", "v", "refractory", "psc", "v_pre", "v_prev", "i_a", "i_bap",
-                    "theta_th"):
+                    "theta_th", "syn_h1", "syn_h2"):
             if key in record:
                 recs[key] = torch.empty((T, lanes, n), dtype=torch.float32, device=dev)
         for key in ("i_a", "i_bap", "theta_th"):
@@ -328,6 +342,11 @@ class Engine:
         p.rec_psc = recs["psc"].data_ptr() if "psc" in recs else None
         p.rec_v_pre = recs["v_pre"].data_ptr() if "v_pre" in recs else None
         p.rec_v_prev = recs["v_prev"].data_ptr() if "v_prev" in recs else None
+        for key in ("syn_h1", "syn_h2"):
+            if key in recs:
+                if key not in sp.syn_keys.values():
+                    raise ValueError(f"'{key}' is not a state of this synapse")
+                setattr(p, "rec_" + key, recs[key].data_ptr())
         ws = self._workspace(lanes, n, p.slots, p.ordered, dev)
         p.workspace = ws.data_ptr()
         p.workspace_bytes = ws.numel()
@@ -424,6 +443,7 @@ class ResidentRun:
         self.tc_state = ([neuron.i_a, neuron.i_bap, neuron.theta_th]
                          if isinstance(neuron, TwoCompartmentGLIF) else [])
         self.psc = sp.exp_psc.psc
+        self.syn_state = [getattr(sp.exp_psc, k) for k in getattr(sp.exp_psc, "_STATES", ())]
         self.iasc = getattr(neuron, "Iasc", None)
         self.v_reset = neuron.v_reset
         self.idx, self.cnt, self.words = engine._ring_buffers(sp.ring_d, lanes, n, dev)
@@ -453,6 +473,8 @@ class ResidentRun:
         if self.refractory is not None:
             self.refractory.view(self.lanes, -1)[sl] = 0.0
         self.psc.view(self.lanes, -1)[sl] = 0.0
+        for t in self.syn_state:
+            t.view(self.lanes, -1)[sl] = 0.0
         if self.iasc is not None:
             self.iasc.view(self.lanes, -1)[sl] = 0.0
         for t in self.tc_state:

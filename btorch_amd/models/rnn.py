@@ -216,6 +216,8 @@ class RecurrentNN(RecurrentNNAbstract):
             return cls._NEURON_RECORDS.get(k[len(sp.neuron_key):])
         if k in sp.psc_keys:
             return "psc"
+        if k in sp.syn_keys:
+            return sp.syn_keys[k]
         if k.endswith("history.history"):
             return "history"
         return None

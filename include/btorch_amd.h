@@ -149,6 +149,12 @@ int btx_raster_coo(const uint64_t* words, int64_t rows, int64_t n, int64_t row_d
 #define BTX_NEURON_GLIF3 1    /* btorch/models/neurons/glif.py:264-342 */
 #define BTX_NEURON_TWO_COMPARTMENT 2  /* btorch/models/neurons/two_compartment.py:208-292 */
 
+/* Synapse kinetics of the fused step (plan.syn_kind). */
+#define BTX_SYN_EXP 0          /* ExponentialPSC, btorch/models/synapse.py:138-187 */
+#define BTX_SYN_ALPHA 1        /* AlphaPSC, synapse.py:274-334 */
+#define BTX_SYN_ALPHA_BILLEH 2 /* AlphaPSCBilleh, synapse.py:201-268 */
+#define BTX_SYN_DUAL_EXP 3     /* DualExponentialPSC, synapse.py:337-418 */
+
 /* Coefficients of the two-compartment neuron, indices into plan.tc[]. */
 #define BTX_TC_DA 0        /* dt / max(tau_a, 1e-4) */
 #define BTX_TC_DA1 1       /* 1 + DA */
@@ -318,6 +324,30 @@ typedef struct btx_rsnn_plan {
     float* rec_i_a;       /* [T][lanes][n] or NULL */
     float* rec_i_bap;     /* [T][lanes][n] or NULL */
     float* rec_theta_th;  /* [T][lanes][n] or NULL */
+    /* synapse kinetics (BTX_SYN_*), with W z the step's summed push; the
+     * coefficients are (n,) or one value (stride 0), k = expm1(dt*l)/l with
+     * l = -1/tau evaluated by the caller:
+     *   EXP        psc = psc + k_syn*((-psc)/tau_syn) ; psc = psc + W z
+     *   ALPHA      psc = psc + k_syn*((-psc)/tau_syn + h/tau_syn) ;
+     *              h = (h + k_syn*((-h)/tau_syn)) + syn_gain*(W z)
+     *   ALPHA_BILLEH  psc = c*psc + c*h ; h = c*h + syn_gain*(W z),
+     *              c = k_syn (exp(-1/tau)), syn_gain = e/tau
+     *   DUAL_EXP   g_r = g_r + k_syn*((-g_r)/tau_syn) ;
+     *              g_d = g_d + k_syn2*((-g_d)/tau_syn2) ;
+     *              g_r = g_r + W z ; g_d = g_d + W z ; psc = syn_a*(g_d - g_r)
+     * h (or g_r) is syn_h1, g_d is syn_h2: state (lanes, n), updated in
+     * place; rec_syn_h1 / rec_syn_h2 record them like rec_psc.  The neuron
+     * input of a step is the psc the synapse left the step before.  LIF and
+     * GLIF3 neurons run with every kind, the two-compartment neuron with EXP. */
+    int syn_kind;
+    float* syn_h1;
+    float* syn_h2;
+    const float* syn_gain;  int64_t syn_gain_stride;
+    const float* tau_syn2;  int64_t tau_syn2_stride;
+    const float* k_syn2;    int64_t k_syn2_stride;
+    const float* syn_a;     int64_t syn_a_stride;
+    float* rec_syn_h1;    /* [T][lanes][n] or NULL */
+    float* rec_syn_h2;    /* [T][lanes][n] or NULL */
     /* workspace from btx_rsnn_workspace_bytes(), zero-filled once */
     void* workspace;      size_t workspace_bytes;
 } btx_rsnn_plan;
@@ -343,6 +373,15 @@ int btx_lif_step(const btx_rsnn_plan* plan, const float* x, float* z, float* v_p
 int btx_exp_decay(float* psc, int64_t lanes, int64_t n, const float* tau_syn,
                   int64_t tau_syn_stride, const float* k_syn, int64_t k_syn_stride,
                   void* stream);
+
+/* The decay part of one step of an alpha or dual-exponential synapse over
+ * plan.lanes x plan.n (plan.syn_kind, psc, syn_h1, syn_h2, tau_syn, k_syn,
+ * tau_syn2, k_syn2 as in btx_rsnn_plan): ALPHA psc and h decay,
+ * ALPHA_BILLEH psc = c*psc + c*h and h = c*h, DUAL_EXP both conductances
+ * decay; the caller then adds the push (AlphaPSC / AlphaPSCBilleh /
+ * DualExponentialPSC.conductance_charge and the decay of adaptation_charge,
+ * btorch/models/synapse.py:239-253, 304-317, 390-406). */
+int btx_syn_decay(const btx_rsnn_plan* plan, void* stream);
 
 /* btx_rsnn_run with a HIP event pair around every kernel launch of the
  * step sequence, recorded on `stream`.  On return (after a stream

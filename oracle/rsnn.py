@@ -257,6 +257,99 @@ def exp_psc_decay(psc, tau_syn, dt: float):
     return psc + torch.expm1(dt * linear) / linear * derivative
 
 
+def _exp_euler(x, derivative, tau, dt: float):
+    """exp_euler_step (ode.py:20-50) with the linear term -1/tau."""
+    linear = -1.0 / tau
+    return x + torch.expm1(dt * linear) / linear * derivative
+
+
+@dataclass
+class AlphaSyn:
+    """AlphaPSC (synapse.py:274-334): conductance_charge then adaptation_charge."""
+    tau_syn: torch.Tensor
+    g_max: torch.Tensor
+    states = ("h",)
+
+    def to(self, dtype):
+        return AlphaSyn(torch.as_tensor(self.tau_syn).to(dtype),
+                        torch.as_tensor(self.g_max).to(dtype))
+
+    def conductance(self, st, dt):
+        psc, h, tau = st["psc"], st["h"], self.tau_syn
+        st["psc"] = _exp_euler(psc, -psc / tau + h / tau, tau, dt)     # synapse.py:296-305
+
+    def adaptation(self, st, mm_z, dt):
+        wz = self.g_max * mm_z                                           # synapse.py:314
+        st["h"] = _exp_euler(st["h"], -st["h"] / self.tau_syn, self.tau_syn, dt) + wz
+
+
+@dataclass
+class BillehSyn:
+    """AlphaPSCBilleh (synapse.py:201-271); dt must be 1."""
+    tau_syn: torch.Tensor
+    states = ("h",)
+
+    def to(self, dtype):
+        return BillehSyn(torch.as_tensor(self.tau_syn).to(dtype))
+
+    @property
+    def syn_decay(self):
+        return torch.exp(-1.0 / self.tau_syn)                            # synapse.py:235-237
+
+    def conductance(self, st, dt):
+        c = self.syn_decay
+        st["psc"] = c * st["psc"] + c * st["h"]                          # synapse.py:239-241
+
+    def adaptation(self, st, mm_z, dt):
+        st["h"] = self.syn_decay * st["h"] + torch.e / self.tau_syn * mm_z  # synapse.py:253
+
+
+@dataclass
+class DualExpSyn:
+    """DualExponentialPSC (synapse.py:337-418) with the default A (or a given one)."""
+    tau_decay: torch.Tensor
+    tau_rise: torch.Tensor
+    A: torch.Tensor | None = None
+    states = ("g_rise", "g_decay")
+
+    def to(self, dtype):
+        return DualExpSyn(torch.as_tensor(self.tau_decay).to(dtype),
+                          torch.as_tensor(self.tau_rise).to(dtype),
+                          None if self.A is None else torch.as_tensor(self.A).to(dtype))
+
+    @property
+    def a(self):
+        td, tr = self.tau_decay, self.tau_rise
+        A = self.A
+        if A is None:                                                    # synapse.py:369-375
+            A = td / (td - tr) * (tr / td) ** (tr / (tr - td))
+        return (td - tr) / tr / td * A                                   # synapse.py:377
+
+    def conductance(self, st, dt):                                       # synapse.py:390-392
+        st["g_rise"] = _exp_euler(st["g_rise"], -st["g_rise"] / self.tau_rise, self.tau_rise, dt)
+        st["g_decay"] = _exp_euler(st["g_decay"], -st["g_decay"] / self.tau_decay,
+                                   self.tau_decay, dt)
+
+    def adaptation(self, st, mm_z, dt):                                  # synapse.py:403-406
+        st["g_rise"] = st["g_rise"] + mm_z
+        st["g_decay"] = st["g_decay"] + mm_z
+        st["psc"] = self.a * (st["g_decay"] - st["g_rise"])
+
+
+def syn_impulse_response(syn, T: int, dt: float, dtype=torch.float64) -> torch.Tensor:
+    """psc over T steps of one synapse given W z = 1 at step 0 (the state the
+    neuron reads at the next step)."""
+    syn = syn.to(dtype)
+    st = {"psc": torch.zeros(1, dtype=dtype)}
+    st.update({k: torch.zeros(1, dtype=dtype) for k in syn.states})
+    out = []
+    for t in range(T):
+        syn.conductance(st, dt)
+        syn.adaptation(st, torch.full((1,), 1.0 if t == 0 else 0.0, dtype=dtype), dt)
+        out.append(st["psc"].clone())
+    return torch.cat(out)
+
+
 # ---------------------------------------------------------------------------
 # Delay ring (history.py:28-358)
 # ---------------------------------------------------------------------------
@@ -298,6 +391,7 @@ class Net:
     delay_mode: str = "none"   # "none" | "hetero" (HeterSynapsePSC) | "fixed" (DelayedPSC)
     max_delay_steps: int = 1
     circular: bool = False
+    syn: object = None         # AlphaSyn | BillehSyn | DualExpSyn; None = ExponentialPSC(tau_syn)
 
 
 def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, teacher=None,
@@ -331,6 +425,11 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
     v = st.get("v", v0).to(dtype).clone()
     ref = st.get("refractory", torch.zeros(B, n, dtype=dtype)).to(dtype).clone()
     psc = st.get("psc", torch.zeros(B, n, dtype=dtype)).to(dtype).clone()
+    syn = net.syn.to(dtype) if net.syn is not None else None
+    sst = {}
+    if syn is not None:
+        sst = {k: st.get(k, torch.zeros(B, n, dtype=dtype)).to(dtype).clone()
+               for k in syn.states}
     iasc = (st.get("iasc", torch.zeros(B, n, n_asc, dtype=dtype)).to(dtype).clone()
             if glif else None)
     if tc:
@@ -343,7 +442,8 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
     elif net.delay_mode == "fixed" and net.max_delay_steps > 1:
         ring = Ring(B, n, net.max_delay_steps + 1, net.circular, dtype, st.get("history"),
                     st.get("cursor", 0))
-    rec = {k: [] for k in ("z", "v", "refractory", "psc", "v_pre", "i_a", "theta_th")}
+    rec = {k: [] for k in ("z", "v", "refractory", "psc", "v_pre", "i_a", "theta_th",
+                           *(syn.states if syn is not None else ()))}
     x_seq = x_seq.to(dtype)
     for t in range(T):
         if tc:
@@ -355,7 +455,11 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
         else:
             z, v, ref, v_pre = lif_step(v, ref, psc + x_seq[t], p, net.dt)
         zin = teacher[t].to(dtype) if teacher is not None else z
-        psc = exp_psc_decay(psc, tau_syn, net.dt)
+        if syn is None:
+            psc = exp_psc_decay(psc, tau_syn, net.dt)
+        else:
+            sst["psc"] = psc
+            syn.conductance(sst, net.dt)
         if ring is None:
             zf = zin
         elif net.delay_mode == "hetero":
@@ -364,7 +468,13 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
         else:
             ring.update(zin)
             zf = ring.get_delay(net.max_delay_steps)
-        psc = psc + mm(zf)
+        if syn is None:
+            psc = psc + mm(zf)
+        else:
+            syn.adaptation(sst, mm(zf), net.dt)
+            psc = sst["psc"]
+            for k in syn.states:
+                rec[k].append(sst[k])
         for k, val in (("z", z), ("v", v), ("refractory", ref), ("psc", psc), ("v_pre", v_pre)):
             rec[k].append(val)
         if tc:
@@ -372,6 +482,8 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
             rec["theta_th"].append(th)
     out = {k: torch.stack(vals) for k, vals in rec.items() if vals}
     out["final"] = {"v": v, "refractory": ref, "psc": psc}
+    if syn is not None:
+        out["final"].update({k: sst[k] for k in syn.states})
     if glif:
         out["final"]["iasc"] = iasc
     if tc:

@@ -1,0 +1,234 @@
+"""GPU parity of the alpha and dual-exponential synapses (synapse.py:190-418):
+standalone steps and the fused engine against the oracle, bit-exact in
+float32; the sequence (convolution) form and the backward to the stated
+tolerances."""
+
+import numpy as np
+import pytest
+import torch
+
+from btorch_amd import workloads
+from btorch_amd.models import (
+    GLIF3, LIF, AlphaPSC, AlphaPSCBilleh, DelayedPSC, DualExponentialPSC, HeterSynapsePSC,
+    RecurrentNN, SparseConn, environ, functional,
+)
+from oracle import rsnn
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+KINDS = ["alpha", "billeh", "dual"]
+
+
+def bits_equal(a, b) -> bool:
+    a = a.detach().cpu().float()
+    b = b.detach().cpu().float()
+    return bool(((a == b) | (a.isnan() & b.isnan())).all())
+
+
+def _dt(kind):
+    return 1.0 if kind == "billeh" else 0.1
+
+
+def _make(kind, n, linear, per_neuron=False, **kw):
+    """(module, oracle synapse) of one kinetics."""
+    tau = (torch.from_numpy(np.random.default_rng(2).uniform(3, 9, n).astype(np.float32))
+           if per_neuron else torch.tensor(5.0))
+    with environ.context(dt=_dt(kind)):
+        if kind == "alpha":
+            g = kw.get("g_max", 1.3)
+            return AlphaPSC(n, tau, linear, g_max=g), rsnn.AlphaSyn(tau, torch.tensor(g))
+        if kind == "billeh":
+            return AlphaPSCBilleh(n, tau, linear), rsnn.BillehSyn(tau)
+        td = tau * 3.0
+        return DualExponentialPSC(n, td, tau, linear), rsnn.DualExpSyn(td, tau)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("per_neuron", [False, True])
+def test_standalone_step_bit_exact(kind, per_neuron):
+    n, B, T = 300, 3, 40
+    m = workloads.plain_lif_net(n=n, p=0.05, seed=21).matrix()
+    mod, osyn = _make(kind, n, SparseConn(m), per_neuron)
+    mod = mod.to(DEV)
+    functional.init_net_state(mod, batch_size=B, device=DEV, dtype=torch.float32)
+    mm = rsnn.NativeSparse(m)
+    st = {"psc": torch.zeros(B, n)}
+    st.update({k: torch.zeros(B, n) for k in osyn.states})
+    g = torch.Generator().manual_seed(5)
+    dt = _dt(kind)
+    for t in range(T):
+        z = (torch.rand(B, n, generator=g) < 0.1).float()
+        with environ.context(dt=dt), torch.no_grad():
+            out = mod(z.to(DEV))
+        osyn.conductance(st, dt)
+        osyn.adaptation(st, mm(z), dt)
+        assert bits_equal(out, st["psc"]), (kind, t)
+        for k in osyn.states:
+            assert bits_equal(getattr(mod, k), st[k]), (kind, k, t)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("batch_shape", [(), (2,), (2, 3)])
+def test_loop_equals_convolution_form(kind, batch_shape):
+    # the reference's loop-vs-conv parity (test_synapse.py:199-223), tolerance 1e-5
+    n, T = 4, 16
+    lin = torch.nn.Linear(n, n, bias=False)
+    torch.nn.init.eye_(lin.weight)
+    mod, _ = _make(kind, n, lin)
+    mod = mod.to(DEV)
+    z = torch.randn(T, *batch_shape, n, generator=torch.Generator().manual_seed(42)).to(DEV)
+    with environ.context(dt=1.0), torch.no_grad():
+        functional.init_net_state(mod, batch_size=batch_shape or None, device=DEV,
+                                  dtype=torch.float32)
+        loop = torch.stack([mod.single_step_forward(z[t]) for t in range(T)])
+        mod.reset()
+        functional.init_net_state(mod, batch_size=batch_shape or None, device=DEV,
+                                  dtype=torch.float32)
+        conv = mod.multi_step_forward(z, kernel_len=64)
+    torch.testing.assert_close(loop, conv, atol=1e-5, rtol=1e-5)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_standalone_backward_matches_float64(kind):
+    # gradient of a weighted sum of every step's psc w.r.t. the input
+    # sequence, against torch autograd of the oracle in float64 (1e-4 relative)
+    n, B, T = 64, 2, 25
+    m = workloads.plain_lif_net(n=n, p=0.1, seed=8).matrix()
+    mod, osyn = _make(kind, n, SparseConn(m))
+    mod = mod.to(DEV)
+    dt = _dt(kind)
+    g = torch.Generator().manual_seed(9)
+    x = torch.rand(T, B, n, generator=g)
+    wsum = torch.randn(T, B, n, generator=g)
+    xd = x.to(DEV).requires_grad_(True)
+    functional.init_net_state(mod, batch_size=B, device=DEV, dtype=torch.float32)
+    with environ.context(dt=dt):
+        outs = torch.stack([mod.single_step_forward(xd[t]) for t in range(T)])
+    (outs * wsum.to(DEV)).sum().backward()
+    s64 = osyn.to(torch.float64)
+    mm = rsnn.NativeSparse(m, torch.float64)
+    x64 = x.double().requires_grad_(True)
+    st = {"psc": torch.zeros(B, n, dtype=torch.float64)}
+    st.update({k: torch.zeros(B, n, dtype=torch.float64) for k in s64.states})
+    ref = []
+    for t in range(T):
+        s64.conductance(st, dt)
+        s64.adaptation(st, mm(x64[t]), dt)
+        ref.append(st["psc"])
+    (torch.stack(ref) * wsum.double()).sum().backward()
+    gd, gr = xd.grad.double().cpu(), x64.grad
+    assert float((gd - gr).abs().max()) <= 1e-4 * float(gr.abs().max())
+
+
+def _engine_case(kind, neuron, n, B, T, seed, wrap="plain", D=6, per_neuron=False):
+    dt = _dt(kind)
+    if wrap == "hetero":
+        c = workloads.powerlaw_net_fast(n=n, k_min=5, k_max=60, delay_lo=1, delay_hi=D - 1,
+                                        seed=seed)
+        conn = c.expanded(D)
+    else:
+        conn = workloads.plain_lif_net(n=n, p=0.05, seed=seed).matrix()
+    mod, osyn = _make(kind, n, SparseConn(conn), per_neuron)
+    if wrap == "hetero":
+        base = type(mod)
+        kw = {"alpha": dict(tau_syn=5.0, g_max=1.3), "billeh": dict(tau_syn=5.0),
+              "dual": dict(tau_decay=15.0, tau_rise=5.0)}[kind]
+        with environ.context(dt=dt):
+            syn = HeterSynapsePSC(n, 1, None, SparseConn(conn), base_psc=base,
+                                  max_delay_steps=D, use_circular_buffer=True, **kw)
+        prefix = "synapse.base_psc."
+        net_kw = dict(delay_mode="hetero", max_delay_steps=D, circular=True)
+    elif wrap == "delayed":
+        syn = DelayedPSC(mod, max_delay_steps=3)
+        prefix = "synapse.psc_module."
+        net_kw = dict(delay_mode="fixed", max_delay_steps=3)
+    else:
+        syn = mod
+        prefix = "synapse."
+        net_kw = {}
+    names = ["neuron.v", prefix + "psc"] + [prefix + k for k in osyn.states]
+    model = RecurrentNN(neuron, syn, step_mode="m", update_state_names=names).to(DEV)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    scale = 25.0 if isinstance(neuron, LIF) else 8.0
+    x = torch.from_numpy(workloads.poisson_input(T, B, n, 400.0, 0.1, seed=seed) * scale)
+    with environ.context(dt=dt), torch.no_grad():
+        z, st = model(x.to(DEV))
+    return model, conn, osyn, x, z, st, prefix, net_kw
+
+
+def _oracle_net(neuron_params, n, conn, osyn, dt, **kw):
+    return rsnn.Net(n, conn, neuron_params, torch.tensor(5.0), dt, syn=osyn, **kw)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("n", [700, 703])
+def test_engine_lif_kinetics_bit_exact(kind, n):
+    # n = 703 runs the scalar state accesses, n = 700 the 4-wide ones
+    B, T = 2, 200
+    model, conn, osyn, x, z, st, prefix, _ = _engine_case(
+        kind, LIF(n, tau_ref=2.0), n, B, T, 31, per_neuron=(n == 703))
+    assert model.fused_engine() is not None
+    ref = rsnn.simulate(_oracle_net(rsnn.LIFParams.make(n, tau_ref=2.0), n, conn, osyn,
+                                    _dt(kind)), x)
+    assert ref["z"].sum() > 50, "test network too quiet"
+    assert bits_equal(z, ref["z"])
+    assert bits_equal(st["neuron.v"], ref["v"])
+    assert bits_equal(st[prefix + "psc"], ref["psc"])
+    for k in osyn.states:
+        assert bits_equal(st[prefix + k], ref[k]), k
+        assert bits_equal(getattr(model.synapse, k), ref["final"][k]), k
+    assert bits_equal(model.synapse.psc, ref["final"]["psc"])
+    # a second call continues from the carried synapse state
+    x2 = torch.from_numpy(workloads.poisson_input(40, B, n, 400.0, 0.1, seed=32) * 25.0)
+    with environ.context(dt=_dt(kind)), torch.no_grad():
+        z2, _ = model(x2.to(DEV))
+    ref2 = rsnn.simulate(_oracle_net(rsnn.LIFParams.make(n, tau_ref=2.0), n, conn, osyn,
+                                     _dt(kind)), x2, state=ref["final"])
+    assert bits_equal(z2, ref2["z"])
+    assert bits_equal(model.synapse.psc, ref2["final"]["psc"])
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_engine_hetero_delays_kinetics_bit_exact(kind):
+    n, B, T, D = 600, 3, 150, 6
+    model, conn, osyn, x, z, st, prefix, net_kw = _engine_case(
+        kind, LIF(n, tau_ref=2.0), n, B, T, 41, wrap="hetero", D=D)
+    base = model.synapse.base_psc
+    osyn = {"alpha": rsnn.AlphaSyn(torch.tensor(5.0), torch.tensor(1.3)),
+            "billeh": rsnn.BillehSyn(torch.tensor(5.0)),
+            "dual": rsnn.DualExpSyn(torch.tensor(15.0), torch.tensor(5.0))}[kind]
+    ref = rsnn.simulate(_oracle_net(rsnn.LIFParams.make(n, tau_ref=2.0), n, conn, osyn,
+                                    _dt(kind), **net_kw), x)
+    assert ref["z"].sum() > 50
+    assert bits_equal(z, ref["z"])
+    assert bits_equal(st[prefix + "psc"], ref["psc"])
+    for k in osyn.states:
+        assert bits_equal(getattr(base, k), ref["final"][k]), k
+    assert bits_equal(model.synapse.psc, ref["final"]["psc"])
+
+
+def test_engine_glif3_dual_exponential_delayed_bit_exact():
+    n, B, T = 400, 2, 150
+    rng = np.random.default_rng(3)
+    tau = torch.from_numpy(rng.uniform(10.0, 30.0, n).astype(np.float32))
+    kw = dict(v_threshold=-50.0, v_reset=-70.0, c_m=0.05, tau=tau, k=[0.3, 0.02],
+              asc_amps=[-0.6, 0.9], tau_ref=2.0)
+    model, conn, osyn, x, z, st, prefix, net_kw = _engine_case(
+        "dual", GLIF3(n, **kw), n, B, T, 51, wrap="delayed")
+    ref = rsnn.simulate(_oracle_net(rsnn.GLIF3Params.make(n, **kw), n, conn, osyn, 0.1,
+                                    **net_kw), x)
+    assert ref["z"].sum() > 50
+    assert bits_equal(z, ref["z"]) and bits_equal(st["neuron.v"], ref["v"])
+    assert bits_equal(st[prefix + "psc"], ref["psc"])
+    assert bits_equal(st[prefix + "g_decay"], ref["g_decay"])
+    assert bits_equal(model.neuron.Iasc, ref["final"]["iasc"])
+
+
+def test_engine_kinetics_gradient_is_refused():
+    n = 64
+    mod, _ = _make("alpha", n, SparseConn(workloads.plain_lif_net(n=n, p=0.1, seed=1).matrix()))
+    model = RecurrentNN(LIF(n), mod, step_mode="m").to(DEV)
+    functional.init_net_state(model, batch_size=1, device=DEV, dtype=torch.float32)
+    x = torch.rand(5, 1, n, device=DEV, requires_grad=True)
+    with environ.context(dt=0.1), pytest.raises(NotImplementedError, match="exponential"):
+        model(x)

@@ -236,3 +236,58 @@ def test_engine_matches_two_compartment_cells():
         MixedNeuronPopulation([])
     with pytest.raises(ValueError):
         MixedNeuronPopulation([(0, LIF(1))])
+
+
+def _kinetics_modules(n, linear):
+    from btorch_amd.models import AlphaPSC, AlphaPSCBilleh, DualExponentialPSC
+    with environ.context(dt=1.0):
+        return [AlphaPSC(n, 4.0, linear, g_max=1.5), AlphaPSCBilleh(n, 6.0, linear),
+                DualExponentialPSC(n, 9.0, 2.0, linear)]
+
+
+def test_psc_kernels_equal_oracle_impulse_responses():
+    """get_kernel (the sequence form) of each kinetics equals the oracle's
+    step loop driven by one impulse."""
+    mods = _kinetics_modules(4, torch.nn.Identity())
+    syns = [rsnn.AlphaSyn(torch.tensor(4.0), torch.tensor(1.5)), rsnn.BillehSyn(torch.tensor(6.0)),
+            rsnn.DualExpSyn(torch.tensor(9.0), torch.tensor(2.0))]
+    for m, s in zip(mods, syns):
+        k = m.get_kernel(1.0, 64).double()
+        r = rsnn.syn_impulse_response(s, 64, 1.0)
+        np.testing.assert_allclose(k.numpy(), r.numpy(), rtol=1e-5, atol=1e-7)
+
+
+def test_kinetics_modules_state_and_engine_match():
+    from btorch_amd.models import AlphaPSC, DelayedPSC, TwoCompartmentGLIF
+    from btorch_amd.models._engine import match_cell
+    from btorch_amd.models.neurons.mixed import MixedNeuronPopulation
+    n = 12
+    m = scipy.sparse.random(n, n, density=0.3, format="coo", random_state=5)
+    for syn in _kinetics_modules(n, SparseConn(m)):
+        syn.init_state(batch_size=2)
+        assert syn.psc.shape == (2, n)
+        for k in syn._STATES:
+            assert getattr(syn, k).shape == (2, n)
+        spec = match_cell(RecurrentNN(LIF(n), syn))
+        assert spec is not None and spec.exp_psc is syn
+        assert spec.syn_keys == {f"synapse.{k}": f"syn_h{j + 1}"
+                                 for j, k in enumerate(syn._STATES)}
+        with pytest.raises(RuntimeError, match="HIP device"):
+            syn.single_step_forward(torch.ones(2, n))
+    # delayed and receptor-typed forms; the default base_psc is AlphaPSC
+    delays = np.array([1, 2] * (m.nnz // 2) + [1] * (m.nnz % 2))
+    conn = SparseConn(expand_conn_for_delays(m, delays, 3))
+    het = HeterSynapsePSC(n, 1, None, conn, max_delay_steps=3, tau_syn=5.0)
+    assert isinstance(het.base_psc, AlphaPSC)
+    spec = match_cell(RecurrentNN(LIF(n), het))
+    assert spec.syn_keys == {"synapse.base_psc.h": "syn_h1"}
+    spec = match_cell(RecurrentNN(LIF(n), DelayedPSC(AlphaPSC(n, 5.0, SparseConn(m)), 2)))
+    assert spec.syn_keys == {"synapse.psc_module.h": "syn_h1"}
+    # the two-compartment neuron runs the exponential synapse only
+    one = MixedNeuronPopulation([(n, TwoCompartmentGLIF(n))])
+    from btorch_amd.models import ApicalRecurrentNN
+    assert match_cell(ApicalRecurrentNN(one, AlphaPSC(n, 5.0, SparseConn(m)))) is None
+    # Billeh's model is defined for dt == 1
+    from btorch_amd.models import AlphaPSCBilleh
+    with environ.context(dt=0.5), pytest.raises(ValueError):
+        AlphaPSCBilleh(n, 5.0, SparseConn(m))

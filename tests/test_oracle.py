@@ -432,3 +432,87 @@ def test_spikes_sparse_repr_known_answer_and_round_trip():
     key = (d["spikes_idx_time"].astype(np.int64) * 3 + d["spikes_idx_batch"]) * 130 \
         + d["spikes_idx_neuron"]
     assert np.all(np.diff(key) > 0)  # strictly ascending flat index
+
+
+# ---------------------------------------------------------------------------
+# alpha and dual-exponential synapses (synapse.py:190-418)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("tau,g,dt", [(5.0, 1.0, 1.0), (2.0, 0.5, 0.1), (20.0, 3.0, 0.25)])
+def test_alpha_psc_impulse_response_closed_form(tau, g, dt):
+    # solving the two linear updates: k[0] = 0, k[t] = g*t*(1 - a)*a^(t-1), a = exp(-dt/tau)
+    T = 80
+    r = rsnn.syn_impulse_response(rsnn.AlphaSyn(torch.tensor(tau), torch.tensor(g)), T, dt)
+    a = math.exp(-dt / tau)
+    cf = [0.0] + [g * t * (1 - a) * a ** (t - 1) for t in range(1, T)]
+    np.testing.assert_allclose(r.numpy(), cf, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("tau", [3.0, 5.0, 12.0])
+def test_billeh_alpha_closed_form_and_unit_peak_at_tau(tau):
+    T = 60
+    r = rsnn.syn_impulse_response(rsnn.BillehSyn(torch.tensor(tau)), T, 1.0)
+    c = math.exp(-1.0 / tau)
+    cf = [t * (math.e / tau) * c ** t for t in range(T)]
+    np.testing.assert_allclose(r.numpy(), cf, rtol=1e-12, atol=1e-14)
+    # the model's normalisation: amplitude 1 at t = tau_syn (integer tau)
+    assert int(torch.argmax(r)) == int(tau)
+    assert abs(float(r[int(tau)]) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("td,tr,dt", [(20.0, 5.0, 1.0), (8.0, 2.0, 0.5), (3.0, 0.5, 0.1)])
+def test_dual_exponential_closed_form_and_normalisation(td, tr, dt):
+    T = 100
+    syn = rsnn.DualExpSyn(torch.tensor(td), torch.tensor(tr))
+    r = rsnn.syn_impulse_response(syn, T, dt)
+    ad, ar = math.exp(-dt / td), math.exp(-dt / tr)
+    a = float(syn.to(torch.float64).a)
+    np.testing.assert_allclose(r.numpy(), [a * (ad ** t - ar ** t) for t in range(T)],
+                               rtol=1e-12, atol=1e-14)
+    # default A: the difference of exponentials peaks at 1 (continuous time,
+    # t_peak = td*tr/(td - tr)*ln(td/tr)), and a = A*(td - tr)/(td*tr)
+    A = a * td * tr / (td - tr)
+    tp = td * tr / (td - tr) * math.log(td / tr)
+    assert abs(A * (math.exp(-tp / td) - math.exp(-tp / tr)) - 1.0) < 1e-12
+    grid = np.linspace(0, 10 * td, 200001)
+    assert abs(A * np.max(np.exp(-grid / td) - np.exp(-grid / tr)) - 1.0) < 1e-8
+    # a given A replaces the default
+    a2 = float(rsnn.DualExpSyn(torch.tensor(td), torch.tensor(tr), torch.tensor(2.0))
+               .to(torch.float64).a)
+    assert abs(a2 - 2.0 * (td - tr) / (td * tr)) < 1e-12
+
+
+@pytest.mark.parametrize("syn,dt", [
+    (rsnn.AlphaSyn(torch.tensor(4.0), torch.tensor(1.5)), 0.5),
+    (rsnn.BillehSyn(torch.tensor(6.0)), 1.0),
+    (rsnn.DualExpSyn(torch.tensor(9.0), torch.tensor(2.0)), 0.2)])
+def test_psc_step_loop_equals_convolution_with_impulse_response(syn, dt):
+    # the synapse is linear and time-invariant: a step loop over any push
+    # sequence equals the causal convolution with its impulse response
+    # (the reference's loop-vs-conv parity, test_synapse.py:199-223)
+    T = 50
+    rng = np.random.default_rng(3)
+    wz = torch.from_numpy(rng.normal(size=(T, 4)))
+    s64 = syn.to(torch.float64)
+    st = {"psc": torch.zeros(4, dtype=torch.float64)}
+    st.update({k: torch.zeros(4, dtype=torch.float64) for k in s64.states})
+    loop = []
+    for t in range(T):
+        s64.conductance(st, dt)
+        s64.adaptation(st, wz[t], dt)
+        loop.append(st["psc"].clone())
+    loop = torch.stack(loop)
+    k = rsnn.syn_impulse_response(syn, T, dt)
+    conv = torch.stack([sum(k[t - j] * wz[j] for j in range(t + 1)) for t in range(T)])
+    np.testing.assert_allclose(loop.numpy(), conv.numpy(), rtol=1e-10, atol=1e-12)
+
+
+def test_alpha_order_mistake_is_caught():
+    # the closed form separates the reference's order (psc reads h before the
+    # push) from the swapped one, which would answer at step 0
+    syn = rsnn.AlphaSyn(torch.tensor(5.0), torch.tensor(1.0)).to(torch.float64)
+    st = {"psc": torch.zeros(1, dtype=torch.float64), "h": torch.zeros(1, dtype=torch.float64)}
+    syn.adaptation(st, torch.ones(1, dtype=torch.float64), 1.0)
+    syn.conductance(st, 1.0)
+    assert float(st["psc"]) != 0.0
+    assert float(rsnn.syn_impulse_response(syn, 2, 1.0)[0]) == 0.0
