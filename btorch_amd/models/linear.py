@@ -19,6 +19,7 @@ from __future__ import annotations
 
 from typing import Any
 
+import numpy as np
 import scipy.sparse
 import torch
 import torch.nn as nn
@@ -244,4 +245,124 @@ class SparseConn(BaseSparseConn):
             self.magnitude.data = (self.magnitude * self.initial_sign).relu() * self.initial_sign
 
 
-__all__ = ["BaseSparseConn", "SparseConn", "available_sparse_backends"]
+class SparseConstrainedConn(BaseSparseConn):
+    """Sparse linear whose connections share one learnable magnitude per
+    constraint group (btorch/models/linear.py:594-867): the effective weight
+    of a connection is ``initial_weight * magnitude[group]``.  ``constraint``
+    has the shape of ``conn`` and holds 1-based group ids.  The product and
+    its backward are the same HIP kernels as ``SparseConn``'s; the per-edge
+    weight gradient reaches ``magnitude`` through the gather, which sums it
+    by group."""
+
+    def __init__(self, conn, constraint, enforce_dale: bool = True, bias=None,
+                 sparse_backend: str | None = None,
+                 sparse_config: SparseBackendConfig | dict[str, Any] | None = None,
+                 device=None, dtype=None, persist_initial_weight: bool = False):
+        self.enforce_dale = enforce_dale
+        self.persist_initial_weight = persist_initial_weight
+        self.constraint_info = None
+        constraint = scipy.sparse.coo_array(constraint).T.tocoo()
+        constraint.eliminate_zeros()
+        self._constraint_matrix = constraint
+        super().__init__(conn, bias=bias, sparse_backend=sparse_backend,
+                         sparse_config=sparse_config, device=device, dtype=dtype)
+
+    def _init_weights(self, value: torch.Tensor):
+        self.register_buffer("initial_weight", value, persistent=self.persist_initial_weight)
+        num_groups = int(self._constraint_matrix.data.max())
+        self.magnitude = nn.Parameter(torch.empty(num_groups, device=value.device))
+        idx = self._precompute_scatter_indices(self._constraint_matrix)
+        self.register_buffer("_constraint_scatter_indices",
+                             torch.as_tensor(idx, dtype=torch.long, device=value.device),
+                             persistent=False)
+        self.reset_parameters()
+
+    def reset_parameters(self) -> None:
+        nn.init.ones_(self.magnitude)
+
+    def _precompute_scatter_indices(self, constraint) -> np.ndarray:
+        """0-based group of every connection, in the coalesced (dst, src)
+        order (the reference matches the (row, col) pairs by a left join)."""
+        idx = self.indices.cpu().numpy()
+        n_src = self.in_features
+        key = idx[0].astype(np.int64) * n_src + idx[1].astype(np.int64)
+        ckey = constraint.row.astype(np.int64) * n_src + constraint.col.astype(np.int64)
+        order = np.argsort(ckey, kind="stable")
+        ckey, group = ckey[order], constraint.data[order].astype(np.int64)
+        pos = np.searchsorted(ckey, key)
+        hit = pos < ckey.size
+        hit[hit] = ckey[pos[hit]] == key[hit]
+        if not hit.all():
+            raise AssertionError("Constraint missing for some connections.")
+        return group[pos] - 1
+
+    def _get_effective_weight(self) -> torch.Tensor:
+        return self.initial_weight * self.magnitude[self._constraint_scatter_indices]
+
+    @torch.no_grad()
+    def constrain(self, *args, **kwargs):
+        if self.enforce_dale:
+            self.magnitude.data = self.magnitude.relu()
+
+    @classmethod
+    def from_hetersynapse(cls, conn, constraint, receptor_type_index, enforce_dale: bool = True,
+                          bias=None, sparse_backend=None, sparse_config=None, device=None,
+                          dtype=None, persist_initial_weight: bool = False):
+        """From the output of make_hetersynapse_constrained_conn, keeping the
+        receptor table for ``set_group_magnitude(receptor_pair=...)``."""
+        inst = cls(conn, constraint, enforce_dale=enforce_dale, bias=bias,
+                   sparse_backend=sparse_backend, sparse_config=sparse_config, device=device,
+                   dtype=dtype, persist_initial_weight=persist_initial_weight)
+        inst.constraint_info = {"receptor_type_index": receptor_type_index}
+        return inst
+
+    def get_group_info(self, include_weights: bool = False):
+        """One row per group: group_id, num_connections, current_magnitude
+        (and the mean/std of the initial weights)."""
+        import pandas as pd
+
+        n_groups = len(self.magnitude)
+        gid = self._constraint_scatter_indices
+        rows = []
+        for g in range(n_groups):
+            mask = gid == g
+            info = {"group_id": g, "num_connections": int(mask.sum()),
+                    "current_magnitude": float(self.magnitude[g].detach())}
+            if include_weights:
+                w = self.initial_weight[mask]
+                info["mean_initial_weight"] = float(w.mean())
+                info["std_initial_weight"] = float(w.std(unbiased=False))
+            rows.append(info)
+        df = pd.DataFrame(rows)
+        if self.constraint_info is not None:
+            rti = self.constraint_info["receptor_type_index"]
+            if len(rti) == n_groups:
+                df = pd.concat([df, rti.reset_index(drop=True)], axis=1)
+        return df
+
+    def set_group_magnitude(self, group_id: int | None = None,
+                            receptor_pair: tuple[str, str] | None = None,
+                            value: float | torch.Tensor = 1.0):
+        if (group_id is None) == (receptor_pair is None):
+            raise ValueError("Exactly one of group_id or receptor_pair must be specified")
+        if receptor_pair is not None:
+            if self.constraint_info is None:
+                raise ValueError("receptor_pair can only be used if created via "
+                                 "from_hetersynapse")
+            rti = self.constraint_info["receptor_type_index"]
+            if "pre_receptor_type" not in rti.columns:
+                raise ValueError("receptor_pair requires neuron mode "
+                                 "(pre_receptor_type, post_receptor_type)")
+            table = rti.set_index(["pre_receptor_type", "post_receptor_type"])
+            group_id = int(table.loc[tuple(receptor_pair), "receptor_index"])
+        if isinstance(value, (int, float)):
+            value = torch.tensor(value, dtype=self.magnitude.dtype, device=self.magnitude.device)
+        self.magnitude.data[group_id] = value
+
+    def get_weights_by_group(self) -> dict[int, torch.Tensor]:
+        gid = self._constraint_scatter_indices
+        return {g: self.initial_weight[gid == g] * self.magnitude[g]
+                for g in range(len(self.magnitude))}
+
+
+__all__ = ["BaseSparseConn", "SparseConn", "SparseConstrainedConn", "available_sparse_backends"]

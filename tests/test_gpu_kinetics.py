@@ -232,3 +232,50 @@ def test_engine_kinetics_gradient_is_refused():
     x = torch.rand(5, 1, n, device=DEV, requires_grad=True)
     with environ.context(dt=0.1), pytest.raises(NotImplementedError, match="exponential"):
         model(x)
+
+
+def test_engine_constrained_conn_forward_and_group_gradient():
+    # SparseConstrainedConn (linear.py:594-867): group-tied magnitudes; the
+    # engine's rasters equal the oracle's on the effective weights, and the
+    # adjoint's magnitude gradient equals the reference-style autograd summed
+    # by group (rel. L2 1e-4)
+    import math
+
+    import scipy.sparse
+
+    from btorch_amd.models import ExponentialPSC, SparseConstrainedConn, SuperSpike
+
+    n, B, T, G = 256, 3, 60, 5
+    conn = workloads.plain_lif_net(n=n, p=0.08, seed=61).matrix()
+    rng = np.random.default_rng(61)
+    cons = scipy.sparse.coo_array(conn)
+    cons.data = rng.integers(1, G + 1, size=cons.nnz).astype(np.float64)
+    sc = SparseConstrainedConn(conn, cons)
+    with torch.no_grad():
+        sc.magnitude.copy_(torch.from_numpy(rng.uniform(0.5, 1.5, G).astype(np.float32)))
+    sur = SuperSpike(alpha=25.0 / (math.sqrt(2) - 1))
+    lif = LIF(n, tau=20.0, tau_ref=2.0, surrogate_function=sur)
+    model = RecurrentNN(lif, ExponentialPSC(n, 5.0, sc), step_mode="m",
+                        update_state_names=[]).to(DEV)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    x = torch.from_numpy(workloads.poisson_input(T, B, n, 400.0, 0.1, seed=61) * 25.0)
+    R = torch.randn(T, B, n, generator=torch.Generator().manual_seed(1))
+    with environ.context(dt=0.1):
+        z, _ = model(x.to(DEV))
+    (z * R.to(DEV)).sum().backward()
+    dst, src = sc.indices[0].cpu(), sc.indices[1].cpu()
+    gid = sc._constraint_scatter_indices.cpu()
+    w0 = sc.initial_weight.cpu()
+    mag = sc.magnitude.detach().cpu().clone().requires_grad_(True)
+    w_eff = w0 * mag[gid]
+    eff = scipy.sparse.coo_array((w_eff.detach().numpy(), (src.numpy(), dst.numpy())),
+                                 shape=(n, n))
+    ref = rsnn.simulate(rsnn.Net(n, eff, rsnn.LIFParams.make(n, tau_ref=2.0),
+                                 torch.tensor(5.0), 0.1), x)
+    assert ref["z"].sum() > 20
+    assert bits_equal(z, ref["z"])
+    zr, *_ = rsnn.simulate_grad(n, dst, src, w_eff, x, torch.tensor(20.0), 0.1, 5.0,
+                                tau_ref=2.0, surrogate=("superspike", sur.alpha, 1.0, True))
+    (zr * R).sum().backward()
+    a, b = sc.magnitude.grad.double().cpu(), mag.grad.double()
+    assert float((a - b).norm() / b.norm()) < 1e-4

@@ -291,3 +291,50 @@ def test_kinetics_modules_state_and_engine_match():
     from btorch_amd.models import AlphaPSCBilleh
     with environ.context(dt=0.5), pytest.raises(ValueError):
         AlphaPSCBilleh(n, 5.0, SparseConn(m))
+
+
+def _constrained(seed=0, n_src=30, n_dst=20, groups=3):
+    from btorch_amd.models import SparseConstrainedConn
+    rng = np.random.default_rng(seed)
+    dense = (rng.random((n_src, n_dst)) < 0.3) * rng.normal(size=(n_src, n_dst))
+    m = scipy.sparse.coo_array(dense.astype(np.float32))
+    g = m.copy()
+    g.data = rng.integers(1, groups + 1, size=m.nnz).astype(np.float64)
+    return SparseConstrainedConn(m, g), dense.astype(np.float32), g.toarray().astype(int)
+
+
+def test_constrained_conn_groups_and_effective_weight():
+    conn, dense, gdense = _constrained()
+    dst, src = conn.indices.numpy()
+    # every connection maps to the group the constraint matrix gives it
+    assert np.array_equal(conn._constraint_scatter_indices.numpy(), gdense[src, dst] - 1)
+    assert torch.equal(conn.initial_weight, torch.from_numpy(dense[src, dst]))
+    with torch.no_grad():
+        conn.magnitude.copy_(torch.tensor([0.5, -2.0, 3.0]))
+    w = conn._get_effective_weight()
+    assert torch.equal(w, conn.initial_weight * conn.magnitude.detach()[gdense[src, dst] - 1])
+    conn.constrain()
+    assert torch.equal(conn.magnitude.detach(), torch.tensor([0.5, 0.0, 3.0]))
+    info = conn.get_group_info(include_weights=True)
+    assert info["num_connections"].sum() == conn.nnz
+    conn.set_group_magnitude(group_id=1, value=4.0)
+    assert float(conn.magnitude[1]) == 4.0
+    groups = conn.get_weights_by_group()
+    assert sum(v.numel() for v in groups.values()) == conn.nnz
+    with pytest.raises(ValueError):
+        conn.set_group_magnitude()
+    # the magnitude gradient sums the per-connection gradient by group
+    conn.reset_parameters()
+    gw = torch.randn(conn.nnz, generator=torch.Generator().manual_seed(0))
+    (conn._get_effective_weight() * gw).sum().backward()
+    want = torch.zeros(3).index_add_(0, conn._constraint_scatter_indices,
+                                     gw * conn.initial_weight)
+    assert torch.allclose(conn.magnitude.grad, want, atol=1e-6)
+    # a connection without a group is an error, as in the reference
+    from btorch_amd.models import SparseConstrainedConn
+    m = scipy.sparse.coo_array(np.eye(4, dtype=np.float32))
+    g = scipy.sparse.coo_array(np.diag([1.0, 1.0, 0.0, 2.0]))
+    with pytest.raises(AssertionError):
+        SparseConstrainedConn(m, g)
+    # state dict: magnitude only (initial weights on request)
+    assert set(conn.state_dict()) == {"indices", "magnitude"}
