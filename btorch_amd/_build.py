@@ -14,7 +14,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 CSRC = PKG_DIR / "csrc"
 LIB_PATH = PKG_DIR / "libbtx.so"
-SOURCES = ["btx_spmv.hip", "btx_step.hip", "btx_adjoint.hip", "btx_io.hip"]
+SOURCES = ["btx_spmv.hip", "btx_step.hip", "btx_adjoint.hip", "btx_io.hip", "btx_build.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

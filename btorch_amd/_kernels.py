@@ -69,6 +69,79 @@ def build_csr(dst: torch.Tensor, src: torch.Tensor, n_src: int, n_dst: int) -> C
                      csc_ptr.contiguous(), src.to(torch.int32).contiguous())
 
 
+@dataclass
+class BuiltConn:
+    """A connection built on the device by ``build_conn``: coalesced
+    (dst, src) ``indices``, their ``values``, the CSR ``layout`` and the
+    matrix ``shape`` (n_src, n_dst)."""
+
+    indices: torch.Tensor
+    values: torch.Tensor
+    layout: CsrLayout
+    shape: tuple[int, int]
+
+
+def build_conn(pre: torch.Tensor, post: torch.Tensor, w: torch.Tensor,
+               delay: torch.Tensor | None = None, n_delay_bins: int = 1,
+               n_pre: int | None = None, n_post: int | None = None) -> BuiltConn:
+    """Delay-expanded, coalesced connectivity and its CSR layout built on the
+    device by ``btx_conn_build`` (the reference's expand_conn_for_delays,
+    btorch/connectome/connection.py:973-1027, and BaseSparseConn's coalescing,
+    btorch/models/linear.py:276-291).  Virtual source row of an edge:
+    pre*n_delay_bins + clip(delay, 0, n_delay_bins - 1)."""
+    _lib.require_hip(pre, post, w)
+    if delay is not None:
+        _lib.require_hip(delay)
+    nnz = int(pre.numel())
+    if post.numel() != nnz or w.numel() != nnz or (delay is not None and delay.numel() != nnz):
+        raise ValueError("pre, post, w (and delay) must have one entry per edge")
+    if n_delay_bins < 1:
+        raise ValueError("n_delay_bins must be at least 1")
+    dev = pre.device
+    pre = pre.reshape(-1).to(torch.int64).contiguous()
+    post = post.reshape(-1).to(torch.int64).contiguous()
+    w = w.reshape(-1).to(torch.float32).contiguous()
+    if delay is not None:
+        delay = delay.reshape(-1).to(torch.int64).contiguous()
+    if nnz:
+        lo = torch.stack([pre.min(), post.min()] + ([delay.min()] if delay is not None else []))
+        hi = torch.stack([pre.max(), post.max()])
+        lo, hi = lo.tolist(), hi.tolist()
+        if min(lo[:2]) < 0:
+            raise ValueError("pre and post indices must be non-negative")
+        if delay is not None and lo[2] < 0:
+            raise ValueError("delays must be non-negative")
+        n_pre = hi[0] + 1 if n_pre is None else n_pre
+        n_post = hi[1] + 1 if n_post is None else n_post
+        if hi[0] >= n_pre or hi[1] >= n_post:
+            raise ValueError("an edge lies outside the (n_pre, n_post) matrix")
+    n_pre = int(n_pre or 0)
+    n_post = int(n_post or 0)
+    n_src = n_pre * n_delay_bins
+    m = max(nnz, 1)
+    i64 = dict(dtype=torch.int64, device=dev)
+    i32 = dict(dtype=torch.int32, device=dev)
+    out_dst, out_src, perm = (torch.empty(m, **i64) for _ in range(3))
+    out_w = torch.empty(m, dtype=torch.float32, device=dev)
+    col, csc_src = torch.empty(m, **i32), torch.empty(m, **i32)
+    row_ptr = torch.empty(n_src + 1, **i64)
+    csc_ptr = torch.empty(n_post + 1, **i64)
+    nnz_out = torch.zeros(1, **i64)
+    lib = _lib.load()
+    ws = torch.empty(lib.btx_conn_build_workspace_bytes(nnz), dtype=torch.uint8, device=dev)
+    ptr = (lambda t: None if t is None else t.data_ptr())
+    _lib.check(lib.btx_conn_build(ptr(pre), ptr(post), ptr(delay), ptr(w), nnz, n_pre, n_post,
+                                  int(n_delay_bins), ptr(out_dst), ptr(out_src), ptr(out_w),
+                                  ptr(nnz_out), ptr(row_ptr), ptr(col), ptr(perm), ptr(csc_ptr),
+                                  ptr(csc_src), ptr(ws), ws.numel(), _lib.stream_ptr()),
+               "btx_conn_build")
+    u = int(nnz_out.item())
+    layout = CsrLayout(row_ptr, col[:u].contiguous(), perm[:u].contiguous(), n_src, n_post,
+                       csc_ptr, csc_src[:u].contiguous())
+    indices = torch.stack([out_dst[:u], out_src[:u]])
+    return BuiltConn(indices, out_w[:u].contiguous(), layout, (n_src, n_post))
+
+
 def compact_dense(x2d: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     """Ascending nonzero columns of every row of ``x2d`` (lanes, n)."""
     _lib.require_hip(x2d)

@@ -169,6 +169,9 @@ _SIGNATURES = {
     "btx_spmv_workspace_bytes": (ctypes.c_size_t, [_c_i64, _c_i64, _c_int, _c_int]),
     "btx_word_popcount": (_c_int, [_vp, _c_i64, _c_i64, _vp, _vp]),
     "btx_raster_coo": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp]),
+    "btx_conn_build_workspace_bytes": (ctypes.c_size_t, [_c_i64]),
+    "btx_conn_build": (_c_int, [_vp, _vp, _vp, _vp, _c_i64, _c_i64, _c_i64, _c_int, _vp, _vp,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "btx_pack_bits": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _vp]),
     "btx_unpack_bits": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _vp]),
     "btx_spmv_event": (

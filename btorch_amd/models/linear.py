@@ -89,7 +89,14 @@ class BaseSparseConn(nn.Module):
         self._prepared_sparse_weight: torch.Tensor | None = None
         self._sparse_prepare_depth = 0
         self._workspaces: dict[int, torch.Tensor] = {}
-        if isinstance(conn, torch.Tensor) and conn.is_sparse:
+        prebuilt = None
+        if isinstance(conn, _kernels.BuiltConn):
+            # built on the device by btx_conn_build (see ``from_edges``)
+            self.in_features, self.out_features = conn.shape
+            indices = conn.indices
+            value = conn.values.to(dtype or torch.float32)
+            prebuilt = conn.layout
+        elif isinstance(conn, torch.Tensor) and conn.is_sparse:
             # torch COO (n_src, n_dst): transpose and coalesce on its own device
             # (sorted by (dst, src), duplicates summed), e.g. on the GPU
             self.in_features, self.out_features = conn.shape
@@ -109,14 +116,28 @@ class BaseSparseConn(nn.Module):
             value = torch.as_tensor(conn_t.data, dtype=dtype or torch.float32).to(device)
         self.shape = (self.in_features, self.out_features)
         self.register_buffer("indices", indices)
-        self._rebuild_layout()
+        self._rebuild_layout(prebuilt)
         self.bias = nn.Parameter(bias) if bias is not None else None
         self._init_weights(value)
 
     # -- layout --------------------------------------------------------------
-    def _rebuild_layout(self) -> None:
-        lay = _kernels.build_csr(self.indices[0], self.indices[1], self.in_features,
-                                 self.out_features)
+    @classmethod
+    def from_edges(cls, pre: torch.Tensor, post: torch.Tensor, weight: torch.Tensor,
+                   delay: torch.Tensor | None = None, n_delay_bins: int = 1,
+                   shape: tuple[int, int] | None = None, **kwargs):
+        """The connection of the delay-expanded matrix of edges (pre, post,
+        delay, weight), built on the device (``btx_conn_build``): equal to
+        ``cls(expand_conn_for_delays(coo((weight, (pre, post))), delay,
+        n_delay_bins))`` without the host round trip.  ``shape`` is
+        (n_pre, n_post); by default the largest indices + 1."""
+        n_pre, n_post = shape if shape is not None else (None, None)
+        built = _kernels.build_conn(pre, post, weight, delay, n_delay_bins, n_pre, n_post)
+        return cls(built, **kwargs)
+
+    def _rebuild_layout(self, lay: _kernels.CsrLayout | None = None) -> None:
+        if lay is None:
+            lay = _kernels.build_csr(self.indices[0], self.indices[1], self.in_features,
+                                     self.out_features)
         for name, t in (("_csr_row_ptr", lay.row_ptr), ("_csr_col", lay.col),
                         ("_csr_perm", lay.perm), ("_csc_ptr", lay.csc_ptr),
                         ("_csc_src", lay.csc_src)):

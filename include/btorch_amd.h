@@ -141,6 +141,38 @@ int btx_raster_coo(const uint64_t* words, int64_t rows, int64_t n, int64_t row_d
                    void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Connectivity built on the device                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace of btx_conn_build for nnz edges. */
+size_t btx_conn_build_workspace_bytes(int64_t nnz);
+
+/* A delay-expanded connection from nnz edges (pre, post, delay, w), as
+ * expand_conn_for_delays (btorch/connectome/connection.py:973-1027) followed
+ * by BaseSparseConn's transpose and scipy sum_duplicates
+ * (btorch/models/linear.py:276-291) and this library's CSR layout:
+ *   virtual source row  vsrc = pre*n_delay_bins + clip(delay, 0, D-1)
+ *                       (delay NULL = 0);
+ *   out_dst/out_src/out_w  the unique (post, vsrc) pairs in ascending
+ *                       (dst, src) order, duplicate weights summed left to
+ *                       right in input order; *nnz_out (device) = their count;
+ *   row_ptr (n_pre*D + 1), col, perm  source-major CSR, destinations
+ *                       ascending per row, perm = position in the coalesced
+ *                       order;
+ *   csc_ptr (n_post + 1), csc_src  the coalesced order's row pointer and
+ *                       sources.
+ * Every output array holds nnz entries (the first *nnz_out are valid).
+ * Indices must lie in [0, n_pre) and [0, n_post) (not checked on the device;
+ * out-of-range input gives wrong output, never a stray access).  Stream
+ * ordered; nnz < 2^31, n_pre*D < 2^31. */
+int btx_conn_build(const int64_t* pre, const int64_t* post, const int64_t* delay,
+                   const float* w, int64_t nnz, int64_t n_pre, int64_t n_post,
+                   int n_delay_bins, int64_t* out_dst, int64_t* out_src, float* out_w,
+                   int64_t* nnz_out, int64_t* row_ptr, int32_t* col, int64_t* perm,
+                   int64_t* csc_ptr, int32_t* csc_src, void* workspace,
+                   size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Fused time-step engine                                                    */
 /* ------------------------------------------------------------------------ */
 

@@ -32,6 +32,43 @@ def coalesce(conn) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     return t.row.astype(np.int64), t.col.astype(np.int64), t.data.astype(np.float32)
 
 
+def numpy_pairwise_sum(a) -> np.float32:
+    """NumPy's float32 pairwise sum (numpy/_core/src/umath/loops_utils.h.src,
+    pairwise_sum): sequential from -0.0 below 8 terms, eight accumulators up
+    to 128, halves split at a multiple of 8 above."""
+    f32 = np.float32
+    n = len(a)
+    if n < 8:
+        res = f32(-0.0)
+        for x in a:
+            res = f32(res + x)
+        return res
+    if n <= 128:
+        r = [f32(x) for x in a[:8]]
+        i = 8
+        while i < n - n % 8:
+            for j in range(8):
+                r[j] = f32(r[j] + a[i + j])
+            i += 8
+        res = f32(f32(f32(r[0] + r[1]) + f32(r[2] + r[3])) + f32(f32(r[4] + r[5]) + f32(r[6] + r[7])))
+        for x in a[i:]:
+            res = f32(res + x)
+        return res
+    n2 = n // 2
+    n2 -= n2 % 8
+    return f32(numpy_pairwise_sum(a[:n2]) + numpy_pairwise_sum(a[n2:]))
+
+
+def duplicate_sum(values) -> np.float32:
+    """Weight of duplicated entries after scipy's sum_duplicates
+    (np.add.reduceat over the stably sorted entries, linear.py:283-284): the
+    first value plus the pairwise sum of the rest, in input order."""
+    values = [np.float32(v) for v in values]
+    if len(values) == 1:
+        return values[0]
+    return np.float32(values[0] + numpy_pairwise_sum(values[1:]))
+
+
 def expand_conn_for_delays(conn, delays, n_delay_bins: int):
     """connection.py:973-1027: row pre*D + clip(delay, 0, D-1)."""
     delays = np.asarray(delays)

@@ -516,3 +516,33 @@ def test_alpha_order_mistake_is_caught():
     syn.conductance(st, 1.0)
     assert float(st["psc"]) != 0.0
     assert float(rsnn.syn_impulse_response(syn, 2, 1.0)[0]) == 0.0
+
+
+@pytest.mark.parametrize("n,nnz", [(300, 20000), (50, 20000), (3, 20000), (1, 3000)])
+def test_duplicate_sum_order_equals_scipy(n, nnz):
+    # the order the device builder sums duplicate edges in: scipy's
+    # sum_duplicates (first value + NumPy pairwise sum of the rest), pinned
+    # against scipy itself for segments of 1..3000 terms
+    rng = np.random.default_rng(n)
+    pre, post = rng.integers(0, n, nnz), rng.integers(0, n, nnz)
+    w = rng.normal(size=nnz).astype(np.float32)
+    m = scipy.sparse.coo_array((w, (pre, post)), shape=(n, n)).T.tocoo()
+    m.sum_duplicates()
+    groups = {}
+    for p, q, x in zip(pre.tolist(), post.tolist(), w):
+        groups.setdefault((q, p), []).append(x)
+    want = np.array([rsnn.duplicate_sum(groups[(r, c)]) for r, c in
+                     zip(m.row.tolist(), m.col.tolist())], dtype=np.float32)
+    assert np.array_equal(want, m.data)
+    if max(len(g) for g in groups.values()) > 1:
+        # a left-to-right sum differs somewhere (the order matters)
+        seq = np.array([_seq(groups[(r, c)]) for r, c in zip(m.row.tolist(), m.col.tolist())],
+                       dtype=np.float32)
+        assert n == 300 or not np.array_equal(seq, m.data)
+
+
+def _seq(vals):
+    s = np.float32(vals[0])
+    for v in vals[1:]:
+        s = np.float32(s + v)
+    return s
