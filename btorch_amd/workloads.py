@@ -17,6 +17,7 @@ from dataclasses import dataclass
 
 import numpy as np
 import scipy.sparse
+import torch
 
 
 @dataclass
@@ -149,3 +150,42 @@ def er_net_fast(n=4096, p=0.02, exc_frac=0.8, seed=0, w_mu=-1.0, w_sigma=0.5,
     pre, post = key // n, key % n
     w = _signed_lognormal(rng, pre, n_exc, inh_scale, w_mu, w_sigma)
     return Connectome(n, pre, post, w, None, n_exc)
+
+
+def sheet_net_device(side: int = 1000, mean_degree: float = 1000.0, sigma: float = 0.1,
+                     delay_lo: int = 1, delay_hi: int = 40, exc_frac: float = 0.8,
+                     w_mu: float = 0.0, w_sigma: float = 0.5, inh_scale: float = 4.0,
+                     seed: int = 4, device="cuda", chunk: int = 1 << 27):
+    """Edges of a distance-dependent random net on a periodic unit square,
+    generated on the device: side*side neurons on a grid (neuron i at
+    ((i % side + 0.5)/side, (i // side + 0.5)/side)); out-degree Poisson with
+    the given mean; each target at a Gaussian displacement (sigma in units of
+    the side) from the source, wrapped, snapped to its grid cell; delays
+    U{delay_lo..delay_hi}; lognormal weights, negative and x inh_scale for
+    the inhibitory sources (every fifth neuron with exc_frac = 0.8).
+    Returns device tensors (pre, post, w, delay) in source order."""
+    n = side * side
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    deg = torch.poisson(torch.full((n,), float(mean_degree), device=device), generator=g)
+    deg = deg.to(torch.int64)
+    pre = torch.repeat_interleave(torch.arange(n, device=device), deg)
+    E = pre.numel()
+    post = torch.empty(E, dtype=torch.int64, device=device)
+    w = torch.empty(E, dtype=torch.float32, device=device)
+    period = max(1, round(1.0 / max(1e-9, 1.0 - exc_frac)))
+    for s in range(0, E, chunk):
+        p = pre[s:s + chunk]
+        m = p.numel()
+        x = (p % side).to(torch.float32) + 0.5
+        y = torch.div(p, side, rounding_mode="floor").to(torch.float32) + 0.5
+        tx = torch.floor(x + torch.randn(m, generator=g, device=device) * (sigma * side))
+        ty = torch.floor(y + torch.randn(m, generator=g, device=device) * (sigma * side))
+        tx = torch.remainder(tx.to(torch.int64), side)
+        ty = torch.remainder(ty.to(torch.int64), side)
+        post[s:s + chunk] = ty * side + tx
+        mag = torch.exp(w_mu + w_sigma * torch.randn(m, generator=g, device=device))
+        inh = (p % period) == 0
+        w[s:s + chunk] = torch.where(inh, -inh_scale * mag, mag)
+    delay = torch.randint(delay_lo, delay_hi + 1, (E,), generator=g, device=device)
+    return pre, post, w, delay

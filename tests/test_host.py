@@ -338,3 +338,24 @@ def test_constrained_conn_groups_and_effective_weight():
         SparseConstrainedConn(m, g)
     # state dict: magnitude only (initial weights on request)
     assert set(conn.state_dict()) == {"indices", "magnitude"}
+
+
+def test_sheet_generator_statistics():
+    """The 2-D sheet workload (largest single-GPU configuration), generated
+    small on the CPU: degrees, displacement, E/I split and delays."""
+    from btorch_amd import workloads
+    side = 40
+    pre, post, w, d = workloads.sheet_net_device(side=side, mean_degree=50.0, sigma=0.1,
+                                                 device="cpu", chunk=4096)
+    n = side * side
+    assert pre.numel() == post.numel() == w.numel() == d.numel()
+    assert abs(pre.numel() / n - 50.0) < 1.0
+    assert int(post.min()) >= 0 and int(post.max()) < n
+    assert int(d.min()) >= 1 and int(d.max()) <= 40
+    # Gaussian displacement of 0.1 side = 4 cells per axis, wrapped
+    for a, b in ((pre % side, post % side), (pre // side, post // side)):
+        delta = torch.remainder(b - a + side // 2, side) - side // 2
+        assert abs(float(delta.float().std()) - 4.0) < 0.2
+    inh = (pre % 5) == 0
+    assert bool((w[inh] < 0).all()) and bool((w[~inh] > 0).all())
+    assert abs(float(inh.float().mean()) - 0.2) < 0.01
