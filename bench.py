@@ -116,9 +116,11 @@ def delivery_table(raster, n, deg):
     act = torch.zeros((T, deg.shape[1]), dtype=torch.float64, device=deg.device)
     nz = (deg > 0).to(torch.float64)
     for t0 in range(0, T, 64):
-        s = unpack_bits(raster[t0:t0 + 64], n).to(torch.float64)   # (t, lanes, n)
-        out[t0:t0 + 64] = (s @ deg).sum(1)
-        act[t0:t0 + 64] = (s @ nz).sum(1)
+        s = unpack_bits(raster[t0:t0 + 64], n)                      # (t, lanes, n)
+        t_idx, _, j = s.nonzero(as_tuple=True)
+        # integer counts in float64: exact whatever the summation order
+        out[t0:t0 + 64].index_add_(0, t_idx, deg[j])
+        act[t0:t0 + 64].index_add_(0, t_idx, nz[j])
     return out, act
 
 
