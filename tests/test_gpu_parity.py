@@ -632,3 +632,27 @@ def test_engine_raster_to_sparse():
     assert ref["spikes"]["nnz"] > 50
     for k in ("spikes_idx_time", "spikes_idx_batch", "spikes_idx_neuron"):
         assert np.array_equal(got[k].cpu().numpy(), ref[k]), k
+
+
+@pytest.mark.parametrize("scale", [1.0, 30.0])
+def test_plain_config_full_size_free_running_bit_exact(scale):
+    """The reference's CPU-runnable configuration at full size: N = 4096 LIF
+    (80/20 E/I, ER p = 0.02), 1000 Poisson input channels at 10 Hz through a
+    fixed 0.05-density projection, B = 1, T = 1000, dt = 0.1 ms; free
+    running (no teacher forcing), every step's raster, v and psc equal to the
+    oracle bit for bit.  scale 30 multiplies the input so the net spikes."""
+    n, n_in, B, T = 4096, 1000, 1, 1000
+    model, conn, inp = _plain_model(n, 0.02, 0, n_in=n_in)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    xin = torch.from_numpy(workloads.poisson_input(T, B, n_in, 10.0, 0.1, seed=0) * scale)
+    model.update_state_names = ["neuron.v", "synapse.psc"]
+    with environ.context(dt=0.1), torch.no_grad():
+        z, st = model(xin.to(DEV))
+    x_proj = rsnn.NativeSparse(workloads.input_weights(n_in, n, 0.05, 0))(
+        xin.reshape(T * B, n_in)).reshape(T, B, n)
+    ref = rsnn.simulate(_net(n, conn), x_proj)
+    print(f"\nconfig-0 scale {scale}: {int(ref['z'].sum())} spikes over {T} steps", flush=True)
+    if scale > 1.0:
+        assert ref["z"].sum() > 100
+    assert bits_equal(z, ref["z"])
+    assert bits_equal(st["neuron.v"], ref["v"]) and bits_equal(st["synapse.psc"], ref["psc"])
