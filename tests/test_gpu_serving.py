@@ -137,3 +137,43 @@ def test_serving_graph_replay_matches_eager():
     counts, traces = _oracle_job(n, D, cexp, req, LanePool(model, lanes=1, chunk=chunk, **kw))
     assert torch.equal(got[req.rid].counts.cpu(), counts)
     assert torch.equal(got[req.rid].traces.cpu(), traces)
+
+
+def test_serving_alpha_synapse_alone_vs_mix_bit_exact():
+    # the same continuous batching with AlphaPSC kinetics (a second synapse
+    # state per lane, reset when a lane is reused)
+    from btorch_amd.models import AlphaPSC
+
+    n, D = 400, 5
+    c = workloads.powerlaw_net_fast(n=n, k_min=5, k_max=60, delay_lo=1, delay_hi=D - 1,
+                                    seed=29)
+    cexp = c.expanded(D)
+    syn = HeterSynapsePSC(n, 1, None, SparseConn(cexp), base_psc=AlphaPSC,
+                          max_delay_steps=D, use_circular_buffer=True, tau_syn=5.0)
+    model = RecurrentNN(LIF(n, tau_ref=2.0), syn, step_mode="m").to(DEV)
+    chunk = 6 * (D + 1)
+    kw = dict(n_sample=5, base_rate_hz=1500.0, kick=12.0)
+    rng = np.random.default_rng(8)
+    reqs = [SimRequest(i, T=int(rng.integers(20, 140)), batch=int(rng.integers(1, 5)),
+                       rate_scale=float(rng.uniform(0.5, 2.0)), seed=int(rng.integers(1 << 30)))
+            for i in range(8)]
+    shared = LanePool(model, lanes=7, chunk=chunk, **kw)
+    mix = Server(shared).serve(reqs, realtime=False)
+    got = {r.rid: r for r in mix.results}
+    assert sorted(got) == list(range(8))
+    alone = run_plain(lambda req: LanePool(model, lanes=req.batch, chunk=chunk, **kw), reqs)
+    for r in alone.results:
+        assert torch.equal(r.counts, got[r.rid].counts), r.rid
+        assert torch.equal(r.traces, got[r.rid].traces), r.rid
+    req = reqs[3]
+    lanes = np.arange(req.batch, dtype=np.int64)
+    seeds = (int(req.seed) * 1_000_003 + lanes) & 0x7FFFFFFFFFFFFFFF
+    prob = np.full(req.batch, np.float32(shared.base_p * req.rate_scale), dtype=np.float32)
+    x = rsnn.poisson_drive(seeds, prob, np.zeros(req.batch, np.int64), req.T, n, shared.kick)
+    net = rsnn.Net(n, cexp, rsnn.LIFParams.make(n, tau_ref=2.0), torch.tensor(5.0), DT,
+                   delay_mode="hetero", max_delay_steps=D, circular=True,
+                   syn=rsnn.AlphaSyn(torch.tensor(5.0), torch.tensor(1.0)))
+    ref = rsnn.simulate(net, x)
+    assert int(ref["z"].sum()) > 50
+    assert torch.equal(got[req.rid].counts.cpu(), ref["z"].sum(0).to(torch.int32))
+    assert torch.equal(got[req.rid].traces.cpu(), ref["v"][:, :, : shared.n_sample])
