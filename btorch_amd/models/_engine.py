@@ -309,8 +309,11 @@ class Engine:
                      if hist.use_circular_buffer
                      else [ring_d - 1 - j for j in range(ring_d)])
             from .._kernels import pack_rows
-            for j in range(ring_d):
-                ring_words[j] = pack_rows(h[:, order[j]])
+            # one launch packs every (lane, slot) row; the slots are then
+            # put in device order by a small gather of the words
+            words = pack_rows(h.reshape(lanes * ring_d, n)).reshape(lanes, ring_d, -1)
+            idx = torch.tensor(order, dtype=torch.long, device=dev)
+            ring_words[:ring_d] = words.index_select(1, idx).transpose(0, 1)
         keep += [ring_idx, ring_cnt, ring_words]
         p.ring_idx = ring_idx.data_ptr()
         p.ring_cnt = ring_cnt.data_ptr()
@@ -386,19 +389,20 @@ class Engine:
         h = hist.history
         hv = h.reshape(lanes, D, n)
         take = min(T, D)
+        # the last `take` steps' spikes, unpacked by one launch: (take, lanes, n)
+        last = unpack_rows(raster[T - take:T].reshape(take * lanes, -1), n).reshape(take, lanes, n)
         if hist.use_circular_buffer:
             if not hv.is_contiguous():
                 hv = hv.contiguous()
-            for i in range(take):
-                t = T - take + i
-                unpack_rows(raster[t], n, out=hv[:, (cursor + t) % D])
+            slots = torch.tensor([(cursor + T - take + i) % D for i in range(take)],
+                                 dtype=torch.long, device=hv.device)
+            hv.index_copy_(1, slots, last.transpose(0, 1))
             hist.history = hv.reshape(h.shape)
             hist.set_cursor(cursor + T)
             return
         # cat mode: index d holds delay d, i.e. step T-1-d
         new = torch.empty_like(hv)
-        for d in range(take):
-            unpack_rows(raster[T - 1 - d], n, out=new[:, d])
+        new[:, :take] = last.flip(0).transpose(0, 1)
         if take < D:
             new[:, take:] = hv[:, : D - take]
         hist.history = new.reshape(h.shape)
