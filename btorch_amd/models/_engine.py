@@ -305,15 +305,13 @@ class Engine:
             # circular: module slot of delay d is (cursor - 1 - d) mod D, so
             # device slot j = ring_d - 1 - d maps to module slot (cursor + j) mod D;
             # cat: module index d is delay d
-            order = ([(cursor + j) % ring_d for j in range(ring_d)]
-                     if hist.use_circular_buffer
-                     else [ring_d - 1 - j for j in range(ring_d)])
+            j = torch.arange(ring_d, dtype=torch.long, device=dev)
+            order = (j + cursor) % ring_d if hist.use_circular_buffer else ring_d - 1 - j
             from .._kernels import pack_rows
             # one launch packs every (lane, slot) row; the slots are then
             # put in device order by a small gather of the words
             words = pack_rows(h.reshape(lanes * ring_d, n)).reshape(lanes, ring_d, -1)
-            idx = torch.tensor(order, dtype=torch.long, device=dev)
-            ring_words[:ring_d] = words.index_select(1, idx).transpose(0, 1)
+            ring_words[:ring_d] = words.index_select(1, order).transpose(0, 1)
         keep += [ring_idx, ring_cnt, ring_words]
         p.ring_idx = ring_idx.data_ptr()
         p.ring_cnt = ring_cnt.data_ptr()
@@ -394,8 +392,8 @@ class Engine:
         if hist.use_circular_buffer:
             if not hv.is_contiguous():
                 hv = hv.contiguous()
-            slots = torch.tensor([(cursor + T - take + i) % D for i in range(take)],
-                                 dtype=torch.long, device=hv.device)
+            slots = (torch.arange(take, dtype=torch.long, device=hv.device)
+                     + (cursor + T - take)) % D
             hv.index_copy_(1, slots, last.transpose(0, 1))
             hist.history = hv.reshape(h.shape)
             hist.set_cursor(cursor + T)

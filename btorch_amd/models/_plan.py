@@ -10,8 +10,12 @@ from . import environ
 
 def param_vec(t, n: int, shape: tuple[int, ...], device) -> tuple[torch.Tensor, int]:
     """(fp32 contiguous tensor, stride) for a scalar or per-neuron parameter."""
-    t = torch.as_tensor(t)
-    t = t.detach().to(device=device, dtype=torch.float32)
+    t = torch.as_tensor(t).detach()
+    if t.device.type == "cpu" and torch.device(device).type == "cuda":
+        # through pinned memory: an asynchronous copy, no wait on the stream
+        t = t.to(torch.float32).contiguous().pin_memory().to(device, non_blocking=True)
+    else:
+        t = t.to(device=device, dtype=torch.float32)
     if t.numel() == 1:
         return t.reshape(1).contiguous(), 0
     if t.numel() == n:

@@ -249,10 +249,11 @@ class AlphaPSCBilleh(_Adaptive2VarPSC):
         self.register_buffer("tau_syn", torch.as_tensor(tau_syn, dtype=torch.float32),
                              persistent=False)
         self.register_buffer("syn_decay", torch.exp(-1.0 / self.tau_syn), persistent=False)
+        # e/tau_syn in the reference's ops (python scalar over the tensor)
+        self.register_buffer("syn_gain", torch.e / self.tau_syn, persistent=False)
 
     def gain(self) -> torch.Tensor:
-        """e/tau_syn in the reference's ops (python scalar over the tensor)."""
-        return torch.e / self.tau_syn.detach().to("cpu", torch.float32)
+        return self.syn_gain
 
     def synapse_plan(self, p, device, keep: list) -> None:
         p.syn_kind = self._KIND
@@ -261,7 +262,7 @@ class AlphaPSCBilleh(_Adaptive2VarPSC):
         p.syn_gain, p.syn_gain_stride = self._vec(self.gain(), device, keep)
 
     def adaptation_charge(self, z: torch.Tensor):
-        self.h = self.h + self.gain().to(self.h.device) * self._push(z)
+        self.h = self.h + self.syn_gain * self._push(z)
 
     def get_kernel(self, dt, kernel_len):
         """k[0] = 0, k[t] = t*(e/tau_syn)*c^t (synapse.py:255-271)."""
