@@ -115,12 +115,14 @@ def delivery_table(raster, n, deg):
     out = torch.zeros((T, deg.shape[1]), dtype=torch.float64, device=deg.device)
     act = torch.zeros((T, deg.shape[1]), dtype=torch.float64, device=deg.device)
     nz = (deg > 0).to(torch.float64)
-    for t0 in range(0, T, 64):
-        s = unpack_bits(raster[t0:t0 + 64], n)                      # (t, lanes, n)
+    # steps per chunk: the unpacked chunk stays below 2^30 elements
+    step = max(1, min(64, (1 << 30) // max(1, raster.shape[1] * n)))
+    for t0 in range(0, T, step):
+        s = unpack_bits(raster[t0:t0 + step], n)                    # (t, lanes, n)
         t_idx, _, j = s.nonzero(as_tuple=True)
         # integer counts in float64: exact whatever the summation order
-        out[t0:t0 + 64].index_add_(0, t_idx, deg[j])
-        act[t0:t0 + 64].index_add_(0, t_idx, nz[j])
+        out[t0:t0 + step].index_add_(0, t_idx, deg[j])
+        act[t0:t0 + step].index_add_(0, t_idx, nz[j])
     return out, act
 
 

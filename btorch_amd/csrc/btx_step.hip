@@ -1626,10 +1626,12 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ps.sub0 = p->bin_sub[0];
     for (int j = 1; j < p->nbins; ++j)
         if (p->bin_delay[j] != ps.delay0 + j || p->bin_sub[j] != ps.sub0 + j) ps.affine = 0;
-    // one segment per P workgroup (shorter, better balanced push chains than
-    // several; measured 5% faster at 8 lanes of the 140k connectome), at
-    // most kThreads (delay bin, segment) pairs each
-    int cpb = 1;
+    // segments per P workgroup: one while the launch has a few workgroups
+    // per CU (shorter, better balanced push chains: 4% faster than two at 8
+    // lanes of the 140k connectome), two once lanes x segments reaches 16 per
+    // CU of the 256 (7% faster at 64 lanes, 9% at 256, equal at 32); at most
+    // kThreads (delay bin, segment) pairs each
+    int cpb = lanes * nchunks >= 4096 ? 2 : 1;
     if (cpb * p->nbins > kThreads) cpb = kThreads / p->nbins;
     ps.cpb = cpb;
     const int npb = (int)((nchunks + cpb - 1) / cpb);

@@ -656,3 +656,23 @@ def test_plain_config_full_size_free_running_bit_exact(scale):
         assert ref["z"].sum() > 100
     assert bits_equal(z, ref["z"])
     assert bits_equal(st["neuron.v"], ref["v"]) and bits_equal(st["synapse.psc"], ref["psc"])
+
+
+def test_engine_many_lanes_two_segments_per_push_workgroup():
+    # lanes x segments >= 4096 switches the push to two segments per
+    # workgroup; rasters, v and psc stay bit-exact
+    n, B, T, D = 40_000, 104, 24, 5
+    c, cexp = _delayed_net(n, D, 71, k_max=30)
+    assert B * ((n + 1023) // 1024) >= 4096
+    syn = HeterSynapsePSC(n, 1, None, SparseConn(cexp), base_psc=ExponentialPSC,
+                          max_delay_steps=D, use_circular_buffer=True, tau_syn=5.0)
+    model = RecurrentNN(LIF(n, tau_ref=2.0), syn, step_mode="m",
+                        update_state_names=["neuron.v", "synapse.psc"]).to(DEV)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    x = torch.from_numpy(workloads.poisson_input(T, B, n, 2000.0, 0.1, seed=71) * 25.0)
+    with environ.context(dt=0.1), torch.no_grad():
+        z, st = model(x.to(DEV))
+    ref = rsnn.simulate(_net(n, cexp, delay_mode="hetero", max_delay_steps=D, circular=True), x)
+    assert ref["z"].sum() > 1000
+    assert bits_equal(z, ref["z"])
+    assert bits_equal(st["neuron.v"], ref["v"]) and bits_equal(st["synapse.psc"], ref["psc"])
