@@ -221,3 +221,31 @@ def test_standalone_lif_step_backward_matches_reference_autograd(hard):
     assert zr.sum() > 50
     assert rel_err(xg.grad, xr.grad) < 1e-4
     assert rel_err(lif.tau.grad, tau_r.grad) < 1e-4
+
+
+def test_per_step_loop_gradients_equal_fused_adjoint():
+    # the same cell differentiated two ways: the fused engine's adjoint
+    # (model(x)) and torch autograd through the standalone HIP steps of the
+    # per-step loop (model(x, None) takes that loop); rasters equal, gradients
+    # within 1e-4 relative
+    n, B, T = 256, 2, 40
+    rng = np.random.default_rng(11)
+    tau = torch.from_numpy(rng.uniform(10, 30, n).astype(np.float32))
+    sur = SuperSpike(alpha=25.0 / (math.sqrt(2) - 1))
+    model, _ = _model(n, 43, tau, sur)
+    x = torch.from_numpy(workloads.poisson_input(T, B, n, 400.0, 0.1, seed=43) * 25.0).to(DEV)
+    R = torch.randn(T, B, n, generator=torch.Generator().manual_seed(5)).to(DEV)
+    grads, rasters = [], []
+    for per_step in (False, True):
+        model.zero_grad(set_to_none=True)
+        functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+        xg = x.clone().requires_grad_(True)
+        with environ.context(dt=0.1):
+            z, _ = model(xg, None) if per_step else model(xg)
+        (z * R).sum().backward()
+        rasters.append(z.detach().cpu())
+        grads.append((xg.grad.clone(), model.synapse.linear.magnitude.grad.clone(),
+                       model.neuron.tau.grad.clone()))
+    assert torch.equal(rasters[0], rasters[1]) and rasters[0].sum() > 20
+    for a, b in zip(*grads):
+        assert rel_err(b, a) < 1e-4
