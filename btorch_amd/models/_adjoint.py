@@ -55,6 +55,16 @@ def _surrogate_code(fn, require_spiking: bool = True) -> tuple[int, float, float
     return kind, float(fn.alpha), float(fn.damping_factor)
 
 
+def fused_adjoint_covers(spec, lanes: int) -> bool:
+    """Whether the fused adjoint differentiates this cell; otherwise
+    RecurrentNN trains through the per-step loop of standalone steps."""
+    try:
+        _check_supported(spec, lanes)
+    except NotImplementedError:
+        return False
+    return True
+
+
 def _check_supported(spec, lanes):
     if spec.syn_keys:
         raise NotImplementedError("the fused adjoint covers the exponential synapse (the alpha "

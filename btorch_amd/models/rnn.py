@@ -171,10 +171,15 @@ class RecurrentNN(RecurrentNNAbstract):
         if (eng is None or len(args) != 1 or kwargs or not torch.is_tensor(x)
                 or x.device.type != "cuda"):
             return super()._multi_step_loop(*args, loop_args=loop_args, **kwargs)
-        from ._adjoint import needs_grad, run_with_grad
+        from ._adjoint import fused_adjoint_covers, needs_grad, run_with_grad
 
         if needs_grad(self, x):
-            return run_with_grad(self, eng, x)
+            lanes = x[0].numel() // max(1, x.shape[-1]) if x.ndim > 1 else 1
+            if fused_adjoint_covers(eng.spec, lanes):
+                return run_with_grad(self, eng, x)
+            # autograd through the standalone HIP steps (which raise for
+            # what they cannot differentiate)
+            return super()._multi_step_loop(*args, loop_args=loop_args, **kwargs)
         with torch.no_grad():
             return self._fused_forward(eng, x)
 
