@@ -249,3 +249,50 @@ def test_per_step_loop_gradients_equal_fused_adjoint():
     assert torch.equal(rasters[0], rasters[1]) and rasters[0].sum() > 20
     for a, b in zip(*grads):
         assert rel_err(b, a) < 1e-4
+
+
+def test_standalone_glif3_step_backward_matches_reference_autograd():
+    # GLIF3.single_step_forward with gradients: input and tau gradients, and
+    # the after-spike currents carried through time, against the
+    # reference-style autograd of glif.py's ops in float32 on the CPU
+    from btorch_amd.models import GLIF3
+
+    n, B, T, dt = 300, 2, 40, 0.1
+    rng = np.random.default_rng(9)
+    tau = torch.from_numpy(rng.uniform(10, 30, n).astype(np.float32))
+    sur = SuperSpike(alpha=25.0 / (math.sqrt(2) - 1))
+    kw = dict(v_threshold=-50.0, v_reset=-70.0, c_m=0.05, k=[0.3, 0.02], asc_amps=[-0.6, 0.9],
+              tau_ref=2.0)
+    glif = GLIF3(n, tau=tau, trainable_param={"tau"}, surrogate_function=sur, **kw).to(DEV)
+    functional.init_net_state(glif, batch_size=B, device=DEV, dtype=torch.float32)
+    x = torch.from_numpy(workloads.poisson_input(T, B, n, 3000.0, 0.1, seed=9) * 8.0)
+    R = torch.randn(T, B, n, generator=torch.Generator().manual_seed(2))
+    xg = x.to(DEV).requires_grad_(True)
+    with environ.context(dt=dt):
+        z = torch.stack([glif(xg[t]) for t in range(T)])
+    ((z * R.to(DEV)).sum() + glif.Iasc.sum()).backward()
+    p = rsnn.GLIF3Params.make(n, tau=tau, **kw)
+    xr = x.clone().requires_grad_(True)
+    tr = tau.clone().requires_grad_(True)
+    v = torch.full((B, n), -70.0)
+    ia = torch.zeros(B, n, 2)
+    ref = torch.zeros(B, n)
+    zs = []
+    for t in range(T):
+        derivative = -(v - p.v_rest) / tr + (xr[t] + ia.sum(-1)) / p.c_m
+        lin = -1.0 / tr
+        v = v + torch.expm1(dt * lin) / lin * derivative
+        lin_i = -p.k
+        ia = ia + torch.expm1(dt * lin_i) / lin_i * (-p.k * ia)
+        s = rsnn._Spike.apply((v - p.v_threshold) / (p.v_threshold - p.v_reset), "superspike",
+                              sur.alpha, 1.0, True)
+        s = s * (ref == 0).detach().to(s.dtype)
+        v = v - (p.v_threshold - p.v_reset) * s
+        ia = ia + p.asc_amps * s[..., None]
+        ref = torch.relu(ref + s * p.tau_ref - dt)
+        zs.append(s)
+    zr = torch.stack(zs)
+    ((zr * R).sum() + ia.sum()).backward()
+    assert torch.equal(z.detach().cpu(), zr.detach()) and zr.sum() > 50
+    assert rel_err(xg.grad, xr.grad) < 1e-4
+    assert rel_err(glif.tau.grad, tr.grad) < 1e-4
