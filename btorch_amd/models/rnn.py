@@ -315,13 +315,16 @@ class ApicalRecurrentNN(RecurrentNN):
         if not fusable:
             return RecurrentNNAbstract._multi_step_loop(self, *args, loop_args=loop_args,
                                                         **kwargs)
-        from ._adjoint import needs_grad, run_with_grad
+        from ._adjoint import fused_adjoint_covers, needs_grad, run_with_grad
 
         if needs_grad(self, x) or (x_apical is not None and x_apical.requires_grad
                                    and torch.is_grad_enabled()):
-            if x_apical is not None:
-                raise NotImplementedError("the fused adjoint has no apical input")
-            return run_with_grad(self, eng, x)
+            lanes = x[0].numel() // max(1, x.shape[-1]) if x.ndim > 1 else 1
+            if x_apical is None and fused_adjoint_covers(eng.spec, lanes):
+                return run_with_grad(self, eng, x)
+            # autograd through the standalone HIP steps
+            return RecurrentNNAbstract._multi_step_loop(self, *args, loop_args=loop_args,
+                                                        **kwargs)
         with torch.no_grad():
             return self._fused_forward(eng, x, x_apical)
 

@@ -296,3 +296,101 @@ def test_standalone_glif3_step_backward_matches_reference_autograd():
     assert torch.equal(z.detach().cpu(), zr.detach()) and zr.sum() > 50
     assert rel_err(xg.grad, xr.grad) < 1e-4
     assert rel_err(glif.tau.grad, tr.grad) < 1e-4
+
+
+def _tc_autograd_step(st, soma, apical, p, dt, sur_alpha):
+    """two_compartment.py:208-306 op for op, with the surrogate backward."""
+    da = dt / torch.clamp(p["tau_a"], min=1e-4)
+    i_a = (st["i_a"] + da * ((apical + p["w_Ca"] * torch.sigmoid(st["i_a"] - p["theta_Ca"]))
+                             + st["i_bap"])) / (1.0 + da)
+    thr = p["v_threshold"] + st["theta_th"]
+    d_t = torch.clamp(p["delta_T"], min=0.0)
+    arg = torch.clamp((st["v"] - thr) / torch.clamp(d_t, min=1e-4), max=10.0)
+    ex = torch.where(d_t > 0.0, d_t * torch.exp(arg), torch.zeros_like(st["v"]))
+    ds = dt / torch.clamp(p["tau_s"], min=1e-4)
+    v_pre = (st["v"] + ds * ((p["E_L"] + torch.clamp(p["R_s"], min=1e-4)
+                              * (soma + p["w_as"] * i_a)) + ex)) / (1.0 + ds)
+    s = rsnn._Spike.apply(v_pre - thr, "superspike", sur_alpha, 1.0, True)
+    dth = dt / torch.clamp(p["tau_th"], min=1e-4)
+    return s, {"v": v_pre - (v_pre - p["v_reset"]) * s, "i_a": i_a, "i_bap": p["w_sa"] * s,
+               "theta_th": st["theta_th"] / (1.0 + dth) + p["delta_th"] * s}
+
+
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_standalone_two_compartment_backward_matches_autograd(nonlinear):
+    # TwoCompartmentGLIF.single_step_forward with gradients. Linear regime:
+    # spiking, bit-exact forward, gradients vs float32 CPU autograd (1e-4).
+    # Plateau and exponential initiation on: a few sub-threshold steps (the
+    # device's exp and sigmoid differ in the last ulp), vs float64 (1e-4).
+    from btorch_amd.models import TwoCompartmentGLIF
+
+    n, B, dt = 200, 2, 0.1
+    T = 5 if nonlinear else 30
+    sur = SuperSpike(alpha=25.0 / (math.sqrt(2) - 1))
+    kw = dict(v_threshold=-50.0, v_reset=-65.0, tau_s=20.0, R_s=1.0, E_L=-70.0, tau_a=100.0,
+              tau_th=50.0, delta_th=1.5, w_sa=0.5, w_as=1.0)
+    kw.update(dict(delta_T=2.0, w_Ca=3.0, theta_Ca=1.0) if nonlinear
+              else dict(delta_T=0.0, w_Ca=0.0, theta_Ca=1.0))
+    cell = TwoCompartmentGLIF(n, surrogate_function=sur, **kw).to(DEV)
+    functional.init_net_state(cell, batch_size=B, device=DEV, dtype=torch.float32)
+    g = torch.Generator().manual_seed(12)
+    scale = 50.0 if nonlinear else 3000.0
+    soma = torch.rand(T, B, n, generator=g) * scale
+    apical = torch.rand(T, B, n, generator=g) * 20.0
+    R = torch.randn(T, B, n, generator=g)
+    sg_, ag_ = soma.to(DEV).requires_grad_(True), apical.to(DEV).requires_grad_(True)
+    with environ.context(dt=dt):
+        outs = [cell.single_step_forward(sg_[t], ag_[t]) for t in range(T)]
+    z = torch.stack([o[0] for o in outs])
+    ((z * R.to(DEV)).sum() + outs[-1][1].sum() + cell.i_a.sum() + cell.theta_th.sum()).backward()
+    dtype = torch.float64 if nonlinear else torch.float32
+    p = {k: torch.tensor(float(v), dtype=dtype) for k, v in kw.items()}
+    sr = soma.to(dtype).requires_grad_(True)
+    ar = apical.to(dtype).requires_grad_(True)
+    st = {"v": torch.full((B, n), -65.0, dtype=dtype), "i_a": torch.zeros(B, n, dtype=dtype),
+          "i_bap": torch.zeros(B, n, dtype=dtype), "theta_th": torch.zeros(B, n, dtype=dtype)}
+    zs = []
+    for t in range(T):
+        s, st = _tc_autograd_step(st, sr[t], ar[t], p, dt, sur.alpha)
+        zs.append(s)
+    zr = torch.stack(zs)
+    ((zr * R.to(dtype)).sum() + st["v"].sum() + st["i_a"].sum() + st["theta_th"].sum()).backward()
+    assert torch.equal(z.detach().cpu(), zr.detach().float())
+    if nonlinear:
+        assert zr.sum() == 0
+    else:
+        assert zr.sum() > 50
+    assert rel_err(sg_.grad, sr.grad) < 1e-4
+    assert rel_err(ag_.grad, ar.grad) < 1e-4
+
+
+def test_apical_two_compartment_cell_trains_through_the_per_step_loop():
+    # ApicalRecurrentNN over a two-compartment population with gradients:
+    # the per-step loop of standalone HIP steps; its raster equals the fused
+    # forward's and the input gradients are finite and not all zero
+    import scipy.sparse
+
+    from btorch_amd.models import (ApicalRecurrentNN, ExponentialPSC, MixedNeuronPopulation,
+                                   TwoCompartmentGLIF)
+
+    n, B, T = 300, 2, 30
+    m = workloads.plain_lif_net(n=n, p=0.05, seed=13).matrix()
+    m = scipy.sparse.coo_array((m.data * 40.0, (m.row, m.col)), shape=m.shape)
+    sur = SuperSpike(alpha=25.0 / (math.sqrt(2) - 1))
+    cell = TwoCompartmentGLIF(n, surrogate_function=sur, delta_th=1.0)
+    model = ApicalRecurrentNN(MixedNeuronPopulation([(n, cell)]),
+                              ExponentialPSC(n, 5.0, SparseConn(m)), step_mode="m",
+                              update_state_names=[]).to(DEV)
+    g = torch.Generator().manual_seed(13)
+    x = (torch.rand(T, B, n, generator=g) * 3000.0).to(DEV)
+    xa = (torch.rand(T, B, n, generator=g) * 20.0).to(DEV)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    with environ.context(dt=0.1), torch.no_grad():
+        z0, _ = model(x, None, xa)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    xg = x.clone().requires_grad_(True)
+    with environ.context(dt=0.1):
+        z1, _ = model(xg, None, xa)
+    z1.sum().backward()
+    assert torch.equal(z0.cpu(), z1.detach().cpu()) and z0.sum() > 50
+    assert torch.isfinite(xg.grad).all() and xg.grad.abs().sum() > 0
