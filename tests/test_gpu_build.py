@@ -106,3 +106,17 @@ def test_engine_on_device_built_connectome_equals_host_built():
         rasters.append(z.cpu())
     assert rasters[0].sum() > 50
     assert torch.equal(rasters[0], rasters[1])
+
+
+@pytest.mark.parametrize("n_pre,n_post,nnz", [(3, 2, 20_000), (10, 10, 2_000), (50, 40, 30_000)])
+def test_device_build_heavy_duplicates_sum_like_scipy(n_pre, n_post, nnz):
+    # segments of ~3300, ~20 and ~15 duplicates: the recursive, the
+    # eight-accumulator and the sequential branches of the pairwise sum
+    rng = np.random.default_rng(nnz)
+    pre = rng.integers(0, n_pre, nnz).astype(np.int64)
+    post = rng.integers(0, n_post, nnz).astype(np.int64)
+    w = rng.normal(size=nnz).astype(np.float32)
+    host = _host(pre, post, w, None, 1, n_pre, n_post)
+    dev = _device(pre, post, w, None, 1, n_pre, n_post)
+    assert host.nnz <= n_pre * n_post
+    _assert_same(host, dev)
