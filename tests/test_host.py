@@ -359,3 +359,24 @@ def test_sheet_generator_statistics():
     inh = (pre % 5) == 0
     assert bool((w[inh] < 0).all()) and bool((w[~inh] > 0).all())
     assert abs(float(inh.float().mean()) - 0.2) < 0.01
+
+
+def test_fused_adjoint_coverage_decision():
+    """Which cells the fused adjoint differentiates; the others train through
+    the per-step loop of standalone steps."""
+    from btorch_amd.models import GLIF3, AlphaPSC
+    from btorch_amd.models._adjoint import fused_adjoint_covers
+    from btorch_amd.models._engine import match_cell
+    n = 12
+    m = scipy.sparse.random(n, n, density=0.3, format="coo", random_state=6)
+    lif_exp = match_cell(RecurrentNN(LIF(n), ExponentialPSC(n, 5.0, SparseConn(m))))
+    assert fused_adjoint_covers(lif_exp, 8) and fused_adjoint_covers(lif_exp, 64)
+    assert not fused_adjoint_covers(lif_exp, 65)
+    with environ.context(dt=0.1):
+        alpha = match_cell(RecurrentNN(LIF(n), AlphaPSC(n, 5.0, SparseConn(m))))
+    assert not fused_adjoint_covers(alpha, 8)
+    glif = match_cell(RecurrentNN(GLIF3(n), ExponentialPSC(n, 5.0, SparseConn(m))))
+    assert not fused_adjoint_covers(glif, 8)
+    trainable_vth = match_cell(RecurrentNN(LIF(n, trainable_param={"v_threshold"}),
+                                           ExponentialPSC(n, 5.0, SparseConn(m))))
+    assert not fused_adjoint_covers(trainable_vth, 8)
