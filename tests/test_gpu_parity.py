@@ -676,3 +676,16 @@ def test_engine_many_lanes_two_segments_per_push_workgroup():
     assert ref["z"].sum() > 1000
     assert bits_equal(z, ref["z"])
     assert bits_equal(st["neuron.v"], ref["v"]) and bits_equal(st["synapse.psc"], ref["psc"])
+
+
+def test_sparse_conn_matches_committed_sparse_mm_fixture():
+    # the committed torch-CPU sparse.mm output (tests/golden): the HIP
+    # product in ordered mode reproduces it bit for bit
+    from pathlib import Path
+    g = np.load(Path(__file__).with_name("golden") / "sparse_mm_order.npz")
+    n_src, n_dst = int(g["n_src"]), int(g["n_dst"])
+    m = scipy.sparse.coo_array((g["w"], (g["src"], g["dst"])), shape=(n_src, n_dst))
+    conn = SparseConn(m, enforce_dale=False).to(DEV)
+    with torch.no_grad():
+        out = conn(torch.from_numpy(g["x"]).to(DEV)).cpu().numpy()
+    assert np.array_equal(out.view(np.uint32), g["out"].view(np.uint32))

@@ -546,3 +546,32 @@ def _seq(vals):
     for v in vals[1:]:
         s = np.float32(s + v)
     return s
+
+
+def _golden_sparse_mm():
+    from pathlib import Path
+    return np.load(Path(__file__).with_name("golden") / "sparse_mm_order.npz")
+
+
+def test_oracle_product_matches_committed_sparse_mm_fixture():
+    # the committed torch-CPU sparse.mm output (tests/golden, made by
+    # make_sparse_mm_order.py): the oracle's product and an ascending-source
+    # sequential fp32 sum from +0.0 reproduce it bit for bit; the descending
+    # order does not, so the fixture tells the orders apart
+    g = _golden_sparse_mm()
+    dst, src, w, x, out = g["dst"], g["src"], g["w"], g["x"], g["out"]
+    n_src, n_dst = int(g["n_src"]), int(g["n_dst"])
+    mm = rsnn.NativeSparse.from_coalesced(dst, src, w, n_src, n_dst)
+    assert np.array_equal(mm(torch.from_numpy(x)).numpy().view(np.uint32), out.view(np.uint32))
+
+    def seq(order):
+        res = np.zeros((x.shape[0], n_dst), dtype=np.float32)
+        for b in range(x.shape[0]):
+            for d in range(n_dst):
+                acc = np.float32(0.0)
+                for e in order(np.flatnonzero(dst == d)):
+                    acc = np.float32(acc + np.float32(w[e] * x[b, src[e]]))
+                res[b, d] = acc
+        return res
+    assert np.array_equal(seq(lambda e: e), out)
+    assert not np.array_equal(seq(lambda e: e[::-1]), out)
