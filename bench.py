@@ -71,6 +71,10 @@ def parse():
     ap.add_argument("--serve-tmax", type=int, default=5000)
     ap.add_argument("--train-steps", type=int, default=2,
                     help="timed training steps of the differentiable 16k-neuron net (0 = skip)")
+    ap.add_argument("--no-sweep", dest="sweep", action="store_false",
+                    help="skip the activity sweep of the step kernel")
+    ap.add_argument("--no-c0", dest="c0", action="store_false",
+                    help="skip the 4096-neuron CPU-runnable configuration (GPU and CPU oracle)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 512 MiB device copy first (HBM counter calibration)")
     return ap.parse_args()
@@ -164,9 +168,12 @@ def _traffic(args):
     cfg = d.get("config", {})
     if (cfg.get("n"), cfg.get("lanes"), cfg.get("kmin"), cfg.get("w_mu"), cfg.get("mode"),
             cfg.get("input")) != (args.n, args.lanes, args.kmin, args.w_mu, args.mode,
-                                  "in-kernel Poisson"):
+                                  "x tensor, dense spike record (model(x))"):
         return None
-    return d.get("bytes_per_launch")
+    return {"bytes_per_launch": d.get("bytes_per_launch"),
+            "source": "profiles/k_step_140k_8lanes_traffic.txt: rocprofv3 FETCH_SIZE and "
+                      "WRITE_SIZE of a separate run, scaled by the same counters on a copy "
+                      "of known size (L2-to-fabric requests, an upper bound on DRAM traffic)"}
 
 
 def sheet_bench(args, dev):
@@ -306,29 +313,32 @@ def main():
     eng = model.fused_engine()
     n, lanes, D = args.n, args.lanes, args.delay_max + 1
     W, K, KS = args.warmup, args.steps, args.kernel_steps
-    # Poisson background generated in the step kernel: lane b's neuron j gets
-    # the kick at local step s when the counter-based hash of (seed_b, s, j)
-    # falls below rate*dt (the CPU oracle draws the same numbers)
+    # Poisson background: lane b's neuron j gets the kick at step s when the
+    # counter-based hash of (seed_b, s, j) falls below rate*dt; generated on
+    # the device as the input tensor x (T, lanes, n) before the clock (the
+    # CPU oracle draws the same numbers)
     seeds = (np.arange(lanes, dtype=np.int64) * 7919 + 1009 * args.seed) & 0x7FFFFFFFFFFFFFFF
     p_in = np.float32(5.0 * 0.1 * 1e-3)
-    drive = {"seed": torch.from_numpy(seeds).to(dev),
-             "prob": torch.full((lanes,), float(p_in), dtype=torch.float32, device=dev),
-             "t0": torch.zeros(lanes, dtype=torch.int64, device=dev), "kick": args.kick}
-
-    def run(T, timings=None):
-        raster, _ = eng.run(None, T, lanes, set(), drive=drive, timings=timings)
-        drive["t0"] += T
-        return raster
-
+    from btorch_amd._kernels import pack_rows, poisson_fill
+    seed_t = torch.from_numpy(seeds).to(dev)
+    prob_t = torch.full((lanes,), float(p_in), dtype=torch.float32, device=dev)
+    x_all = poisson_fill(seed_t, prob_t, torch.zeros(lanes, dtype=torch.int64, device=dev),
+                         args.kick, W + K + KS, n)
+    x_w, x_t, x_k = x_all[:W], x_all[W:W + K], x_all[W + K:]
     torch.cuda.synchronize()
 
-    # as RecurrentNN.multi_step_forward does: the weights are packed into CSR
-    # order once for the calls that follow, before the clock starts
-    from btorch_amd.models.functional import prepare_sparse_modules
-    with environ.context(dt=0.1), torch.no_grad(), prepare_sparse_modules(model, lanes):
-        r_w = run(W) if W else None
+    def words(z):
+        return pack_rows(z.reshape(-1, n)).reshape(z.shape[0], lanes, -1)
+
+    # The timed region is one call of the module, model(x) -> (z, states),
+    # with x resident in HBM: the drop-in API a btorch user calls.
+    with environ.context(dt=0.1), torch.no_grad():
+        t_first = time.perf_counter()
+        r_w = words(model(x_w)[0]) if W else None
         torch.cuda.synchronize()
-        # snapshot of the state entering the timed region (for the CPU sample)
+        first_call_ms = 1e3 * (time.perf_counter() - t_first)
+        # snapshot of the state entering the timed region (for the CPU
+        # sample); reading the history writes it back here, before the clock
         snap = {"v": model.neuron.v.detach().cpu().clone(),
                 "refractory": model.neuron.refractory.detach().cpu().clone(),
                 "psc": model.synapse.base_psc.psc.detach().cpu().clone(),
@@ -338,13 +348,21 @@ def main():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev0.record()
-        r_t = run(K)
+        z_t, _ = model(x_t)
+        host_ms = 1e3 * (time.perf_counter() - t0)
         ev1.record()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         region_ms = ev0.elapsed_time(ev1)
+        r_t = words(z_t)
+        z_cpu_sample = z_t[:args.cpu_steps].cpu() if not args.no_cpu else None
+        del z_t
+        # the same launches (input x, dense spike record) with a HIP event
+        # pair around every launch
+        from btorch_amd.models.functional import prepare_sparse_modules
         timings: dict = {}
-        r_k = run(KS, timings)
+        with prepare_sparse_modules(model, lanes):
+            r_k, _ = eng.run(x_k, KS, lanes, {"z"}, timings=timings)
         torch.cuda.synchronize()
 
     rasters = [r for r in (r_w, r_t, r_k) if r is not None]
@@ -356,42 +374,48 @@ def main():
     value = ev_t / elapsed
     ms_step = 1e3 * elapsed / K
 
-    # roofline of the dominant kernel over the kernel-timing pass
     per = {k: (v[0] / max(v[1], 1), v[1], v[0]) for k, v in timings.items()
            if not k.endswith("_each")}
     pct = {k[:-5]: [round(float(np.percentile(v, q)) * 1e3, 2) for q in (50, 90, 99)]
            for k, v in timings.items() if k.endswith("_each") and v}
-    # every step is one launch (F and P workgroups) when no delay bin has age 0
-    dom = "step"
     nwords = (n + 63) // 64
-    a_rows_per_launch = rows_k / KS
-    ev_per_launch = ev_k / KS
-    spikes_per_launch = float(unpack_count(r_k)) / KS
-    bytes_model = {
-        # F: v and psc read+write, refractory read (20 B per neuron-lane;
-        # the refractory counter is written back only where it changes, a
-        # few per cent of neuron-lanes, not counted), per-neuron tau, spike
-        # words to the raster and the ring, ring indices of the spikes; the
-        # previous push's term records read (10 B per term: row, weight,
-        # destination).  P: per active row the index and the int64 row_ptr
-        # pair (20 B), per term col + weight read and the term record written
-        # (18 B).  The Poisson background is generated in the kernel.
-        "step": (20.0 * n * lanes + 4.0 * n + 16.0 * nwords * lanes + 4.0 * spikes_per_launch
-                 + 20.0 * a_rows_per_launch + 28.0 * ev_per_launch),
-    }
-    # average launch duration from HIP events around the timed region (the
-    # step launches are back to back on this stream; one launch per step)
+    # algorithmic bytes of one step launch over the timed region: F reads
+    # v, psc and refractory and the input x and writes v, psc and the spike
+    # record z (28 B per neuron-lane; the refractory counter is written back
+    # only where it changes, a few per cent of neuron-lanes, not counted),
+    # per-neuron tau, spike words to the raster and the ring, ring indices of
+    # the spikes; the previous push's term records read (10 B per term: row,
+    # weight, destination).  P: per active row the index and the int64
+    # row_ptr pair (20 B), per term col + weight read and the term record
+    # written (18 B).
+    spikes_per_step = spikes_t / K
+    bytes_launch = (28.0 * n * lanes + 4.0 * n + 16.0 * nwords * lanes + 4.0 * spikes_per_step
+                    + 20.0 * rows_t / K + 28.0 * ev_t / K)
     avg_ms = region_ms / K
-    achieved = bytes_model[dom] / (avg_ms * 1e-3)
+    achieved = bytes_launch / (avg_ms * 1e-3)
+    traffic = _traffic(args)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": _traffic(args),
+                "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                "traffic": None if traffic is None else traffic["bytes_per_launch"],
+                "traffic_source": None if traffic is None else traffic["source"],
                 "kernel": "k_step (fused step + push + reduction)",
-                "bytes_per_launch": bytes_model[dom], "avg_us": avg_ms * 1e3,
-                "avg_us_source": "HIP events around the timed region / steps",
+                "bytes_per_launch": bytes_launch, "avg_us": avg_ms * 1e3,
+                "avg_us_source": "HIP events around the timed model(x) call / steps "
+                                 "(includes the host's enqueue of the call and the final "
+                                 "reduction launch)",
                 "per_launch_event_us": {k: round(v[0] * 1e3, 3) for k, v in per.items()},
-                "per_launch_event_p50_p90_p99_us": pct}
-
+                "per_launch_event_p50_p90_p99_us": pct,
+                "per_launch_event_source": f"HIP event pair around each launch of a "
+                                           f"{KS}-step run of the same launches"}
+    calls = {"host_enqueue_ms": round(host_ms, 3), "first_call_ms": round(first_call_ms, 2),
+             "what": "host_enqueue_ms: wall time of the timed model(x) call until it "
+                     "returned (work enqueued); first_call_ms: the warm-up call, which "
+                     "builds the plan, workspace and delay ring"}
     _progress(f"kernel: {ms_step:.4f} ms per step")
+    sweep = activity_sweep(args, model, eng, deg, dev) if args.sweep else None
+    _progress("activity sweep done")
+    c0 = c0_bench(args, dev) if args.c0 else None
+    _progress("C0 done")
     serving = serve_bench(args, model, dev) if args.serve_requests > 0 else None
     _progress("serving done")
     training = train_bench(args, dev) if args.train_steps > 0 else None
@@ -403,7 +427,8 @@ def main():
         x_cpu = rsnn.poisson_drive(seeds, np.full(lanes, p_in, dtype=np.float32),
                                    np.full(lanes, W, dtype=np.int64), args.cpu_steps, n,
                                    args.kick)
-        cpu = cpu_baseline(args, conn, snap, x_cpu, r_t, deg, tables, W)
+        cpu = cpu_baseline(args, conn, snap, x_cpu, z_cpu_sample, tables, W)
+        cpu["input_equal_to_gpu_x"] = bool(torch.equal(x_cpu, x_t[:args.cpu_steps].cpu()))
 
     line = {
         "metric": "synaptic events/s at 140k-neuron connectome",
@@ -429,14 +454,111 @@ def main():
                    "weights": f"lognormal(mu={args.w_mu}, sigma=0.5), inhibitory x4",
                    "accumulation": args.mode, "build_s": round(t_build, 2)},
         "roofline": roofline,
+        "calls": calls,
         "cpu_baseline": cpu,
         "conn_build": {"device_ms": round(conn_build_ms, 2), "edges": nnz,
                        "what": "delay expansion, coalescing and CSR/CSC layout "
                                "(btx_conn_build)"},
         "serving": serving,
         "training": training,
+        "activity_sweep": sweep,
+        "plain_4096": c0,
     }
     print(json.dumps(line), flush=True)
+
+
+SWEEP_PROBS = (0.001, 0.002, 0.005, 0.01, 0.02, 0.05, 0.1)
+
+
+def activity_sweep(args, model, eng, deg, dev):
+    """The step kernel across activity levels, as the reference's harness
+    sweeps the spike rate of its product's input (0.1 % .. 10 %): each lane's
+    neurons receive a one-step kick with probability p per step (in-kernel
+    drive, raster only), 20 warm-up and 50 timed steps per level from the
+    state the headline run left.  The refractory period (2 ms = 20 steps)
+    caps activity near 5 %; the realised active fraction is reported."""
+    from btorch_amd.models import environ
+    from btorch_amd.models.functional import prepare_sparse_modules
+
+    n, lanes = args.n, args.lanes
+    seeds = (np.arange(lanes, dtype=np.int64) * 104729 + 77 + args.seed) & 0x7FFFFFFFFFFFFFFF
+    out = []
+    t_clock = 0
+    with environ.context(dt=0.1), torch.no_grad(), prepare_sparse_modules(model, lanes):
+        for prob in SWEEP_PROBS:
+            drive = {"seed": torch.from_numpy(seeds).to(dev),
+                     "prob": torch.full((lanes,), prob, dtype=torch.float32, device=dev),
+                     "t0": torch.full((lanes,), t_clock, dtype=torch.int64, device=dev),
+                     "kick": args.kick}
+            r_w, _ = eng.run(None, 20, lanes, set(), drive=drive)
+            drive["t0"] += 20
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            r, _ = eng.run(None, 50, lanes, set(), drive=drive)
+            e1.record()
+            torch.cuda.synchronize()
+            t_clock += 70
+            ms = e0.elapsed_time(e1)
+            tables = delivery_table(torch.cat([r_w, r], 0), n, deg)
+            ev, _ = window_events(tables, 20, 70)
+            spikes = float(unpack_count(r))
+            out.append({"drive_prob": prob, "active_fraction": spikes / (50 * lanes * n),
+                        "events_per_step": ev / 50, "us_per_step": 1e3 * ms / 50,
+                        "events_per_s": ev / (ms * 1e-3)})
+    return out
+
+
+def c0_bench(args, dev):
+    """The reference's CPU-runnable configuration at full size: N = 4096 LIF
+    (80/20 E/I, ER p = 0.02, lognormal(-1, 0.5), inhibitory x4), exponential
+    synapse 5 ms, 1000 Poisson channels at 10 Hz through a fixed 0.05-density
+    projection (neuron_inp_module), B = 1, T = 1000, dt = 0.1 ms.  The GPU
+    runs model(x) (projection and the fused engine); the CPU oracle runs the
+    same T = 1000 steps, and the two rasters are compared."""
+    from btorch_amd import workloads
+    from btorch_amd.models import LIF, ExponentialPSC, RecurrentNN, SparseConn, environ, functional
+    from oracle import rsnn
+
+    n, n_in, B, T = 4096, 1000, 1, 1000
+    c = workloads.plain_lif_net(n=n, p=0.02, seed=0)
+    conn = c.matrix()
+    win = workloads.input_weights(n_in, n, 0.05, 0)
+    model = RecurrentNN(LIF(n, tau=20.0, tau_ref=2.0), ExponentialPSC(n, 5.0, SparseConn(conn)),
+                        neuron_inp_module=SparseConn(win, enforce_dale=False), step_mode="m",
+                        update_state_names=[]).to(dev)
+    xin = torch.from_numpy(workloads.poisson_input(T, B, n_in, 10.0, 0.1, seed=0))
+    xd = xin.to(dev)
+    rp = model.synapse.linear._csr_row_ptr
+    deg_n = (rp[1:] - rp[:-1]).to(torch.float64)
+    res = {}
+    with environ.context(dt=0.1), torch.no_grad():
+        for rep in range(2):  # the second call is timed
+            functional.init_net_state(model, batch_size=B, device=dev, dtype=torch.float32)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            z, _ = model(xd)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+    ev = float((z.reshape(T, n).to(torch.float64) @ deg_n).sum())
+    res["gpu"] = {"s": el, "steps_per_s": T / el, "events_per_s": ev / el,
+                  "spikes": int(z.sum())}
+    threads = len(os.sched_getaffinity(0))
+    torch.set_num_threads(threads)
+    x_proj = rsnn.NativeSparse(win)(xin.reshape(T * B, n_in)).reshape(T, B, n)
+    net = rsnn.Net(n, conn, rsnn.LIFParams.make(n, tau=20.0, tau_ref=2.0), torch.tensor(5.0), 0.1)
+    t0 = time.perf_counter()
+    ref = rsnn.simulate(net, x_proj)
+    el_cpu = time.perf_counter() - t0
+    res["cpu_baseline"] = {"value": ev / el_cpu, "unit": "events/s", "cores": threads,
+                           "kind": "port", "s": el_cpu, "steps_per_s": T / el_cpu,
+                           "sample": "the full T = 1000 run (torch-CPU oracle, input "
+                                     "projection included)"}
+    res["raster_bit_exact_vs_cpu"] = bool(torch.equal(z.cpu().reshape(T, B, n), ref["z"]))
+    res["workload"] = ("N=4096 LIF, ER p=0.02 (~335k synapses), 1000 Poisson inputs at 10 Hz, "
+                       "B=1, T=1000, dt=0.1 ms")
+    res["events"] = ev
+    return res
 
 
 def _private_net(n, seed, dev):
@@ -587,10 +709,9 @@ def train_bench(args, dev):
             "grads": "synaptic weights (13.4M), tau_m (16384), readout"}
 
 
-def cpu_baseline(args, conn, snap, x_cpu, r_t, deg, tables, W):
+def cpu_baseline(args, conn, snap, x_cpu, gpu_z, tables, W):
     """Time the CPU oracle on the first ``cpu_steps`` timed steps from the
     same state; check its raster against the GPU's."""
-    from btorch_amd.models._engine import unpack_bits
     from oracle import rsnn
 
     n, D = args.n, args.delay_max + 1
@@ -608,7 +729,6 @@ def cpu_baseline(args, conn, snap, x_cpu, r_t, deg, tables, W):
     t0 = time.perf_counter()
     out = rsnn.simulate(net, x_cpu, state=snap)
     el = time.perf_counter() - t0
-    gpu_z = unpack_bits(r_t[:S], n).cpu()
     ev, _ = window_events(tables, W, W + S)
     return {"value": ev / el, "unit": "events/s", "cores": threads, "kind": "port",
             "sample": f"{S} steps of the same workload from the timed region's entry state "

@@ -255,3 +255,29 @@ def spmv_weight_grad(layout: CsrLayout, x2d: torch.Tensor | None, active: torch.
         active.data_ptr(), counts.data_ptr(), lanes, active.shape[1],
         None if x2d is None else x2d.data_ptr(), 0 if x2d is None else x2d.shape[1],
         g.data_ptr(), gw_csr.data_ptr(), _lib.stream_ptr()), "btx_spmv_weight_grad")
+
+
+def poisson_fill(seed: torch.Tensor, prob: torch.Tensor, t0: torch.Tensor, kick: float, T: int,
+                 n: int, out: torch.Tensor | None = None) -> torch.Tensor:
+    """(T, lanes, n) float32 Poisson input: ``kick`` where the fused step's
+    in-kernel drive (per-lane seed, rate and clock ``t0``) would fire, else
+    0 (``btx_poisson_fill``).  A run given this tensor as x produces the same
+    bits as a run given the drive."""
+    _lib.require_hip(seed, prob, t0)
+    lanes = seed.shape[0]
+    for t_, dt_ in ((seed, torch.int64), (prob, torch.float32), (t0, torch.int64)):
+        if t_.shape != (lanes,) or t_.dtype != dt_ or not t_.is_contiguous():
+            raise ValueError("seed/prob/t0 must be contiguous (lanes,) int64/float32/int64")
+    if out is None:
+        out = torch.empty((T, lanes, n), dtype=torch.float32, device=seed.device)
+    if out.shape != (T, lanes, n) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out must be contiguous float32 (T, lanes, n)")
+    # one launch covers at most 65535 steps
+    for s0 in range(0, T, 65535):
+        s1 = min(T, s0 + 65535)
+        t0s = t0 + s0 if s0 else t0
+        _lib.check(_lib.load().btx_poisson_fill(seed.data_ptr(), prob.data_ptr(), t0s.data_ptr(),
+                                                float(kick), s1 - s0, lanes, n,
+                                                out[s0:s1].data_ptr(), _lib.stream_ptr()),
+                   "btx_poisson_fill")
+    return out

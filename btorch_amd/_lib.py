@@ -127,6 +127,7 @@ class RsnnPlan(ctypes.Structure):
         ("rec_syn_h2", _vp),
         ("workspace", _vp),
         ("workspace_bytes", ctypes.c_size_t),
+        ("tables_ready", _c_int),
     ]
 
 
@@ -185,6 +186,7 @@ _SIGNATURES = {
         [_vp, _vp, _c_i64, _c_i64, _vp, _vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp, _vp],
     ),
     "btx_rsnn_workspace_bytes": (ctypes.c_size_t, [_c_i64, _c_i64, _c_int, _c_int]),
+    "btx_poisson_fill": (_c_int, [_vp, _vp, _vp, _c_f, _c_i64, _c_i64, _c_i64, _vp, _vp]),
     "btx_rsnn_run": (_c_int, [ctypes.POINTER(RsnnPlan), _c_i64, _vp]),
     "btx_rsnn_run_timed": (_c_int, [ctypes.POINTER(RsnnPlan), _c_i64, _vp,
                                     ctypes.POINTER(ctypes.c_double),

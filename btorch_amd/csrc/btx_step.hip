@@ -1167,6 +1167,30 @@ __global__ __launch_bounds__(256) void k_syn_decay(StepArgs a, int64_t m) {
     else a.psc[i] = psc;
 }
 
+// The in-kernel drive as a dense tensor: one thread per (lane, neuron pair)
+// of step blockIdx.y, the same numbers poisson_drive4 draws.
+__global__ __launch_bounds__(256) void k_poisson_fill(const uint64_t* __restrict__ seed,
+                                                      const float* __restrict__ prob,
+                                                      const int64_t* __restrict__ t0, float kick,
+                                                      int64_t lanes, int64_t n,
+                                                      float* __restrict__ x) {
+    const int64_t t = blockIdx.y;
+    const int64_t npair = (n + 1) >> 1;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= lanes * npair) return;
+    const int64_t lane = g / npair, q = g - lane * npair;
+    const uint64_t step = (uint64_t)(t0[lane] + t);
+    const uint64_t stream = mix64(seed[lane] ^ (step * 0x100000001b3ull));
+    const uint64_t h = mix64(stream + (uint64_t)(q + 1) * 0x9e3779b97f4a7c15ull);
+    const float pr = prob[lane];
+    const float u0 = (float)(h >> 40) * (1.0f / 16777216.0f);
+    const float u1 = (float)((h >> 16) & 0xffffffull) * (1.0f / 16777216.0f);
+    float* row = x + (t * lanes + lane) * n;
+    const int64_t j = 2 * q;
+    row[j] = u0 < pr ? kick : 0.f;
+    if (j + 1 < n) row[j + 1] = u1 < pr ? kick : 0.f;
+}
+
 // Per-run tables, written from the launch arguments by one workgroup (a
 // run-time index into an array of the argument block would make the compiler
 // copy the whole block to scratch memory).
@@ -1382,6 +1406,20 @@ BTX_API int btx_syn_decay(const btx_rsnn_plan* p, void* stream) {
     return check_launch("btx_syn_decay");
 }
 
+BTX_API int btx_poisson_fill(const uint64_t* seed, const float* prob, const int64_t* t0,
+                             float kick, int64_t T, int64_t lanes, int64_t n, float* x,
+                             void* stream) {
+    if (T <= 0 || lanes <= 0 || n <= 0) return 0;
+    if (seed == nullptr || prob == nullptr || t0 == nullptr || x == nullptr)
+        return fail("btx_poisson_fill: null pointer");
+    if (T > 65535) return fail("btx_poisson_fill: T exceeds 65535 steps per call");
+    const int64_t work = lanes * ((n + 1) >> 1);
+    if ((work + 255) / 256 > 0x7fffffff) return fail("btx_poisson_fill: too many lanes x neurons");
+    hipLaunchKernelGGL(k_poisson_fill, dim3((unsigned)((work + 255) / 256), (unsigned)T),
+                       dim3(256), 0, (hipStream_t)stream, seed, prob, t0, kick, lanes, n, x);
+    return check_launch("btx_poisson_fill");
+}
+
 BTX_API size_t btx_rsnn_workspace_bytes(int64_t lanes, int64_t n, int slots, int ordered) {
     (void)slots;
     return engine_ws(lanes, n, ordered).total;
@@ -1554,7 +1592,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         tb.bin_delay[j] = p->bin_delay[j];
         tb.bin_sub[j] = p->bin_sub[j];
     }
-    hipLaunchKernelGGL(k_set_tables, dim3(1), dim3(64), 0, s, tb, ctl);
+    if (!p->tables_ready) hipLaunchKernelGGL(k_set_tables, dim3(1), dim3(64), 0, s, tb, ctl);
     if (p->ring_rebuild)
         hipLaunchKernelGGL(k_ring_from_words,
                            dim3((unsigned)nchunks, (unsigned)lanes, (unsigned)R),

@@ -80,6 +80,10 @@ def _check_supported(spec, lanes):
             raise NotImplementedError(f"gradient w.r.t. LIF.{name} is not provided")
     if lanes > 64:
         raise NotImplementedError("the fused adjoint supports at most 64 lanes")
+    # a surrogate without an adjoint kernel (or a non-spiking forward)
+    # trains through the per-step loop, whose steps differentiate any
+    # surrogate in torch ops
+    _surrogate_code(neuron.surrogate_function)
 
 
 def _delayed(spec) -> bool:
@@ -368,7 +372,14 @@ def run_with_grad(rnn, eng, x):
     _surrogate_code(neuron.surrogate_function)
     eng.spec_dt = float(environ.get("dt"))
     names = rnn._record_names()
-    want = rnn._wanted_records(names, spec) - {"z", "history"}
+    # every requested state maps onto an engine record (NotImplementedError
+    # otherwise, as the no-grad path); the history memory is rebuilt from the
+    # produced spikes
+    want_all = rnn._wanted_records(names, spec)
+    want = want_all - {"z", "history"}
+    hist0 = (spec.history.history.detach().clone()
+             if spec.history is not None and "history" in want_all else None)
+    cursor_rec = spec.history.cursor if spec.history is not None else 0
     chunk = T
     if rnn.grad_checkpoint:
         chunk = rnn.chunk_size or (T if rnn.unroll is False else int(rnn.unroll))
@@ -408,12 +419,12 @@ def run_with_grad(rnn, eng, x):
     recs = {k: torch.cat(vals) for k, vals in sink.items()}
     states = {}
     for key in names:
-        if key == "neuron.v":
-            states[key] = recs["v"].reshape(T, *shape)
-        elif key == "neuron.refractory":
-            states[key] = recs["refractory"].reshape(T, *shape)
-        elif key == "neuron.v_pre_spike":
-            states[key] = recs["v_pre"].reshape(T, *shape)
-        elif key in spec.psc_keys:
-            states[key] = recs["psc"].reshape(T, *shape)
+        rec = rnn._record_of(key, spec)
+        if rec == "history":
+            from ._engine import pack_bits
+            words = pack_bits(z.detach().reshape(T, lanes, n))
+            seq = eng.history_sequence(words, hist0, T, lanes, n, cursor_rec)
+            states[key] = seq.reshape(T, *spec.history.history.shape)
+        else:
+            states[key] = recs[rec].reshape(T, *shape)
     return z, states

@@ -25,6 +25,7 @@ delivers at the end of step t+d.
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass, field
 
 import torch
@@ -150,12 +151,41 @@ def unpack_bits(words: torch.Tensor, n: int) -> torch.Tensor:
     return bits.reshape(*words.shape[:-1], words.shape[-1] * 64)[..., :n].to(torch.float32)
 
 
+def _tkey(t):
+    """Identity and in-place version of a tensor (None passes through)."""
+    if t is None or not torch.is_tensor(t):
+        return t
+    return (id(t), t._version, t.data_ptr())
+
+
+class _RingState:
+    """The device ring a sequence of runs shares: ring_d + 1 slots of spike
+    lists, per-segment counts and words, the slot the next run starts at, and
+    the (tensor, version, cursor) of the module history it last matched."""
+
+    def __init__(self, ring_d, lanes, n, dev):
+        self.idx, self.cnt, self.words = Engine._ring_buffers(ring_d, lanes, n, dev)
+        self.R = ring_d + 1
+        self.cursor = ring_d          # device slot of the next run's first step
+        self.token = None             # module history the ring holds, or None
+        self.fresh = True             # lists not built yet
+
+
 class Engine:
-    """Runs a matched cell for T steps; owns the ring and the workspace."""
+    """Runs a matched cell for T steps; owns the ring and the workspace.
+
+    Across runs it keeps, per lane count, a plan template (parameters,
+    coefficients and connectivity, rebuilt when a parameter tensor, the
+    weights, dt or the configuration changes), the workspace with its
+    delay-bin tables, and the device ring, which the next run continues
+    from instead of rebuilding it from the module's dense history."""
 
     def __init__(self, spec: CellSpec):
         self.spec = spec
         self._ws: dict = {}
+        self._tmpl: dict = {}
+        self._rings: dict = {}
+        self._tables_ok: set = set()
 
     def _workspace(self, lanes, n, slots, ordered, device):
         key = (lanes, n, slots, ordered, str(device))
@@ -164,23 +194,88 @@ class Engine:
             nbytes = _lib.load().btx_rsnn_workspace_bytes(lanes, n, slots, int(ordered))
             ws = torch.zeros(nbytes, dtype=torch.uint8, device=device)
             self._ws = {key: ws}
+            self._tables_ok = set()
         return ws
 
-    def _base_plan(self, lanes: int, keep: list):
-        """Plan fields shared by every run: parameters, state, connectivity."""
+    # -- plan -----------------------------------------------------------------
+    _SCALAR_ATTRS = ("hard_reset", "_use_refractory", "n_Iasc")
+
+    def _static_key(self, lanes: int, dev) -> tuple:
+        """What the plan template depends on besides the state tensors."""
+        from . import environ
+        sp = self.spec
+        conn = sp.conn
+        cfg = conn.sparse_config
+        parts = [lanes, str(dev), float(environ.get("dt")), bool(torch.is_grad_enabled()),
+                 int(cfg.ordered), int(cfg.slots), int(getattr(cfg, "bin_cap", 0)),
+                 id(conn._csr_row_ptr), id(conn._csc_ptr), conn.weight_key(),
+                 id(conn._prepared_sparse_weight)]
+        for m in (sp.neuron, sp.exp_psc):
+            mem = getattr(m, "_memories_rv", {})
+            for name, t in m._parameters.items():
+                parts.append((name, _tkey(t)))
+            for name, t in m._buffers.items():
+                if name not in mem:
+                    parts.append((name, _tkey(t)))
+            for name in self._SCALAR_ATTRS:
+                parts.append(getattr(m, name, None))
+        return tuple(parts)
+
+    def _template(self, lanes: int):
+        """(template plan, tensors it points into): parameters,
+        coefficients, connectivity and bins, cached per lane count."""
         sp = self.spec
         neuron, syn, conn = sp.neuron, sp.exp_psc, sp.conn
-        n = neuron.size
         dev = neuron.v.device
-        _lib.require_hip(neuron.v)
+        key = self._static_key(lanes, dev)
+        hit = self._tmpl.get(lanes)
+        if hit is not None and hit[0] == key:
+            return hit[1], hit[2]
+        keep: list = []
         cfg = conn.sparse_config
         p = _plan.neuron_plan(neuron, lanes, dev, keep)
         p.ordered = int(cfg.ordered)
         p.slots = int(cfg.slots)
         p.bin_cap = int(getattr(cfg, "bin_cap", 0))
         syn.synapse_plan(p, dev, keep)
-        # state
+        w = conn.packed_weight().detach()
+        if w.dtype != torch.float32:
+            raise TypeError("the fused engine supports float32 weights only")
+        keep.append(w)
+        p.row_ptr = conn._csr_row_ptr.data_ptr()
+        p.col = conn._csr_col.data_ptr()
+        p.w = w.data_ptr()
+        p.nnz = conn.nnz
+        p.csr_d = sp.csr_d
+        w_dst = conn.effective_weight_f32()
+        keep.append(w_dst)
+        keep += [conn._csr_row_ptr, conn._csr_col, conn._csc_ptr, conn._csc_src]
+        p.csc_ptr = conn._csc_ptr.data_ptr()
+        p.csc_src = conn._csc_src.data_ptr()
+        p.csc_w = w_dst.data_ptr()
+        p.ring_d = sp.ring_d
+        p.nbins = len(sp.bins)
+        for j, (age, sub) in enumerate(sp.bins):
+            p.bin_delay[j] = age
+            p.bin_sub[j] = sub
+        self._tmpl[lanes] = (key, p, keep)
+        return p, keep
+
+    def _base_plan(self, lanes: int, keep: list):
+        """A plan for one run: the cached template plus the state pointers
+        (checked every run: a module may be given new state tensors)."""
+        sp = self.spec
+        neuron, syn = sp.neuron, sp.exp_psc
+        n = neuron.size
+        dev = neuron.v.device
+        _lib.require_hip(neuron.v)
+        tmpl, tkeep = self._template(lanes)
+        keep += tkeep
+        p = _lib.RsnnPlan()
+        ctypes.memmove(ctypes.addressof(p), ctypes.addressof(tmpl), ctypes.sizeof(p))
         states = [(neuron, "v"), (syn, "psc")] + [(syn, k) for k in getattr(syn, "_STATES", ())]
+        if isinstance(neuron, TwoCompartmentGLIF):
+            states += [(neuron, k) for k in ("i_a", "i_bap", "theta_th")]
         for mod, name in states:
             t = getattr(mod, name)
             if t is None or t.numel() != lanes * n or t.dtype != torch.float32:
@@ -195,31 +290,25 @@ class Engine:
             t = getattr(syn, k)
             setattr(p, f"syn_h{j + 1}", t.data_ptr())
             keep.append(t)
+        if isinstance(neuron, TwoCompartmentGLIF):
+            for k in ("i_a", "i_bap", "theta_th"):
+                t = getattr(neuron, k)
+                setattr(p, k, t.data_ptr())
+                keep.append(t)
+        if isinstance(neuron, GLIF3):
+            m = neuron.n_Iasc
+            ia = neuron.Iasc
+            if ia is None or ia.numel() != lanes * n * m or ia.dtype != torch.float32:
+                raise RuntimeError(f"GLIF3.Iasc is not initialised for {lanes} lanes x {n} "
+                                   "neurons")
+            if not ia.is_contiguous():
+                neuron.Iasc = ia = ia.contiguous()
+            p.iasc = ia.data_ptr()
+            keep.append(ia)
         if getattr(neuron, "_use_refractory", False):
-            if not neuron.refractory.is_contiguous():
-                neuron.refractory = neuron.refractory.contiguous()
-            p.refractory = neuron.refractory.data_ptr()
-            keep.append(neuron.refractory)
-        # connectivity
-        w = conn.packed_weight().detach()
-        if w.dtype != torch.float32:
-            raise TypeError("the fused engine supports float32 weights only")
-        keep.append(w)
-        p.row_ptr = conn._csr_row_ptr.data_ptr()
-        p.col = conn._csr_col.data_ptr()
-        p.w = w.data_ptr()
-        p.nnz = conn.nnz
-        p.csr_d = sp.csr_d
-        w_dst = conn._get_effective_weight().detach().to(torch.float32).contiguous()
-        keep.append(w_dst)
-        p.csc_ptr = conn._csc_ptr.data_ptr()
-        p.csc_src = conn._csc_src.data_ptr()
-        p.csc_w = w_dst.data_ptr()
-        p.ring_d = sp.ring_d
-        p.nbins = len(sp.bins)
-        for j, (age, sub) in enumerate(sp.bins):
-            p.bin_delay[j] = age
-            p.bin_sub[j] = sub
+            ref = _plan.lane_state(neuron, "refractory", tuple(neuron.v.shape))
+            p.refractory = ref.data_ptr()
+            keep.append(ref)
         return p, n, dev
 
     @staticmethod
@@ -260,6 +349,95 @@ class Engine:
             p.rec_sample = sample.data_ptr()
             p.n_sample = sample.shape[2]
 
+    # -- delay ring -----------------------------------------------------------
+    def _ring(self, lanes, n, dev) -> _RingState:
+        key = (lanes, str(dev))
+        ring = self._rings.get(key)
+        if ring is None:
+            ring = _RingState(self.spec.ring_d, lanes, n, dev)
+            self._rings = {key: ring}
+        return ring
+
+    def _hist_token(self, hist):
+        buf = hist._buffers.get("history")
+        return None if buf is None else (id(buf), buf._version, hist.cursor)
+
+    def _prepare_ring(self, ring: _RingState, lanes, n, dev, keep) -> int:
+        """Make the ring hold the module's history; returns ring_rebuild."""
+        sp = self.spec
+        hist = sp.history
+        ring_d = sp.ring_d
+        if hist is None:
+            # no delay memory: only slots this run writes are read
+            rebuild = 1 if ring.fresh else 0
+            ring.fresh = False
+            return rebuild
+        fn = hist.deferred_writeback()
+        if fn is not None and getattr(fn, "ring", None) is ring:
+            # the module's history is the state this ring left: continue
+            hist.discard_writeback()
+            return 0
+        if fn is not None:
+            hist._sync_memories()  # another engine's deferred state
+        if not ring.fresh and ring.token is not None and ring.token == self._hist_token(hist):
+            return 0
+        # rebuild from the dense memory: device slot ring_d - 1 - d holds
+        # delay d; the run starts at slot ring_d
+        h = hist.history
+        if h is None or h.numel() != lanes * ring_d * n:
+            raise RuntimeError("SpikeHistory is not initialised for this batch")
+        h = h.reshape(lanes, ring_d, n).to(torch.float32).contiguous()
+        keep.append(h)
+        cursor = hist.cursor
+        j = torch.arange(ring_d, dtype=torch.long, device=dev)
+        order = (j + cursor) % ring_d if hist.use_circular_buffer else ring_d - 1 - j
+        from .._kernels import pack_rows
+        words = pack_rows(h.reshape(lanes * ring_d, n)).reshape(lanes, ring_d, -1)
+        ring.words[:ring_d] = words.index_select(1, order).transpose(0, 1)
+        ring.words[ring_d:] = 0
+        ring.cnt.zero_()
+        ring.cursor = ring_d
+        ring.fresh = False
+        return 1
+
+    def _defer_history(self, ring: _RingState, lanes, n, T) -> None:
+        """After a run: the module's history is what the ring holds; write it
+        back only when it is read."""
+        hist = self.spec.history
+        if hist is None:
+            return
+        cursor = (hist.cursor + T) % hist.max_delay if hist.use_circular_buffer else 0
+        ring_d, R = self.spec.ring_d, ring.R
+        dev_cursor = ring.cursor
+        engine = self
+
+        class _Writeback:
+            def __call__(self_, h):
+                buf = h._buffers.get("history")
+                if buf is None or buf.numel() != lanes * ring_d * n:
+                    raise RuntimeError("SpikeHistory changed shape while its delay history was "
+                                       "held on the device")
+                if not (buf.is_contiguous() and buf.dtype == torch.float32):
+                    buf = torch.empty(buf.shape, dtype=torch.float32, device=buf.device)
+                    h._buffers["history"] = buf
+                # module slot j holds delay d(j); delay d is device slot
+                # (dev_cursor - 1 - d) mod R
+                j = torch.arange(ring_d, dtype=torch.long, device=buf.device)
+                d = (cursor - 1 - j) % ring_d if h.use_circular_buffer else j
+                slots = (dev_cursor - 1 - d) % R
+                words = ring.words.index_select(0, slots).transpose(0, 1).contiguous()
+                from .._kernels import unpack_rows
+                unpack_rows(words.reshape(lanes * ring_d, -1), n,
+                            out=buf.view(lanes * ring_d, n))
+                if h.use_circular_buffer:
+                    h._buffers["_cursor"].fill_(cursor)
+                ring.token = engine._hist_token(h)
+
+        fn = _Writeback()
+        fn.ring = ring
+        hist.defer_writeback(fn, cursor)
+
+    # -- run --------------------------------------------------------------------
     def run(self, x: torch.Tensor | None, T: int, lanes: int, record: set[str],
             teacher: torch.Tensor | None = None, timings: dict | None = None,
             drive: dict | None = None, counts: torch.Tensor | None = None,
@@ -287,37 +465,12 @@ class Engine:
                 raise ValueError("apical input must be contiguous float32 (T, lanes, n)")
             p.x_apical = x_apical.data_ptr()
             p.x_apical_step_stride = lanes * n
-        # delay ring (ring_d + 1 slots): the history memory's spike words go
-        # into slots 0..ring_d-1 (slot ring_d - k holds the step k steps
-        # before the run) and the run starts at slot ring_d; the lists are
-        # rebuilt on the device from the words
-        ring_d = sp.ring_d
-        ring_idx, ring_cnt, ring_words = self._ring_buffers(ring_d, lanes, n, dev)
-        cursor = 0
-        hist = sp.history
-        if hist is not None:
-            h = hist.history
-            if h is None or h.numel() != lanes * ring_d * n:
-                raise RuntimeError("SpikeHistory is not initialised for this batch")
-            h = h.reshape(lanes, ring_d, n).to(torch.float32).contiguous()
-            keep.append(h)
-            cursor = hist.cursor
-            # circular: module slot of delay d is (cursor - 1 - d) mod D, so
-            # device slot j = ring_d - 1 - d maps to module slot (cursor + j) mod D;
-            # cat: module index d is delay d
-            j = torch.arange(ring_d, dtype=torch.long, device=dev)
-            order = (j + cursor) % ring_d if hist.use_circular_buffer else ring_d - 1 - j
-            from .._kernels import pack_rows
-            # one launch packs every (lane, slot) row; the slots are then
-            # put in device order by a small gather of the words
-            words = pack_rows(h.reshape(lanes * ring_d, n)).reshape(lanes, ring_d, -1)
-            ring_words[:ring_d] = words.index_select(1, order).transpose(0, 1)
-        keep += [ring_idx, ring_cnt, ring_words]
-        p.ring_idx = ring_idx.data_ptr()
-        p.ring_cnt = ring_cnt.data_ptr()
-        p.ring_words = ring_words.data_ptr()
-        p.ring_cursor = ring_d
-        p.ring_rebuild = 1  # ring lists from ring_words before the first step
+        ring = self._ring(lanes, n, dev)
+        p.ring_rebuild = self._prepare_ring(ring, lanes, n, dev, keep)
+        p.ring_idx = ring.idx.data_ptr()
+        p.ring_cnt = ring.cnt.data_ptr()
+        p.ring_words = ring.words.data_ptr()
+        p.ring_cursor = ring.cursor
         self._set_outputs(p, T, lanes, n, drive, counts, sample)
         nwords = (n + 63) // 64
         raster = torch.empty((T, lanes, nwords), dtype=torch.int64, device=dev)
@@ -351,11 +504,12 @@ class Engine:
         ws = self._workspace(lanes, n, p.slots, p.ordered, dev)
         p.workspace = ws.data_ptr()
         p.workspace_bytes = ws.numel()
+        tables = (ws.data_ptr(), tuple(sp.bins))
+        p.tables_ready = 1 if tables in self._tables_ok else 0
         stream = _lib.stream_ptr()
         if timings is None:
             _lib.check(_lib.load().btx_rsnn_run(p, T, stream), "btx_rsnn_run")
         else:
-            import ctypes
             ms = (ctypes.c_double * 4)()
             calls = (ctypes.c_int64 * 4)()
             cap = T + 1
@@ -365,7 +519,9 @@ class Engine:
             for k, name in enumerate(("step", "reduce", "push", "finish")):
                 timings[name] = (float(ms[k]), int(calls[k]))
                 timings[name + "_each"] = [per[k * cap + i] for i in range(min(calls[k], cap))]
-        self._writeback_history(raster, T, lanes, n, cursor)
+        self._tables_ok.add(tables)
+        ring.cursor = (ring.cursor + T) % ring.R
+        self._defer_history(ring, lanes, n, T)
         return raster, recs
 
     def resident(self, lanes: int, T: int, drive: dict | None = None,
@@ -377,33 +533,6 @@ class Engine:
         in a HIP graph.  T must be a multiple of the ring depth, so that every
         launch starts at the same ring slot."""
         return ResidentRun(self, lanes, T, drive, counts, sample)
-
-    def _writeback_history(self, raster, T, lanes, n, cursor):
-        hist = self.spec.history
-        if hist is None:
-            return
-        from .._kernels import unpack_rows
-        D = self.spec.ring_d
-        h = hist.history
-        hv = h.reshape(lanes, D, n)
-        take = min(T, D)
-        # the last `take` steps' spikes, unpacked by one launch: (take, lanes, n)
-        last = unpack_rows(raster[T - take:T].reshape(take * lanes, -1), n).reshape(take, lanes, n)
-        if hist.use_circular_buffer:
-            if not hv.is_contiguous():
-                hv = hv.contiguous()
-            slots = (torch.arange(take, dtype=torch.long, device=hv.device)
-                     + (cursor + T - take)) % D
-            hv.index_copy_(1, slots, last.transpose(0, 1))
-            hist.history = hv.reshape(h.shape)
-            hist.set_cursor(cursor + T)
-            return
-        # cat mode: index d holds delay d, i.e. step T-1-d
-        new = torch.empty_like(hv)
-        new[:, :take] = last.flip(0).transpose(0, 1)
-        if take < D:
-            new[:, take:] = hv[:, : D - take]
-        hist.history = new.reshape(h.shape)
 
     def history_sequence(self, raster, hist0, T, lanes, n, cursor):
         """(T, lanes, D, n) history memory after every step (for recording)."""

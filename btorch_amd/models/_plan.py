@@ -23,6 +23,34 @@ def param_vec(t, n: int, shape: tuple[int, ...], device) -> tuple[torch.Tensor, 
     return torch.broadcast_to(t, shape).reshape(n).contiguous(), 1
 
 
+def lane_state(module, name: str, shape: tuple[int, ...]) -> torch.Tensor:
+    """The memory ``name`` of ``module`` as a contiguous fp32 tensor of
+    exactly ``shape`` (the lanes x neurons a launch covers), stored back on
+    the module.  A memory with fewer batch dimensions (an unbatched state
+    meeting a batched input) is broadcast into a fresh copy; one that cannot
+    broadcast, or does not hold ``prod(shape)`` elements afterwards, raises
+    before any kernel sees it."""
+    t = getattr(module, name, None)
+    if t is None:
+        raise RuntimeError(f"{type(module).__name__}.{name} is not initialised "
+                           "(call init_net_state)")
+    if tuple(t.shape) != tuple(shape):
+        try:
+            t = torch.broadcast_to(t, shape)
+        except RuntimeError as e:
+            raise RuntimeError(f"{type(module).__name__}.{name} of shape {tuple(t.shape)} does "
+                               f"not match the launch shape {tuple(shape)}") from e
+    t = t.contiguous()
+    n = 1
+    for d in shape:
+        n *= int(d)
+    if t.numel() != n:
+        raise RuntimeError(f"{type(module).__name__}.{name} holds {t.numel()} values, the "
+                           f"launch covers {n}")
+    setattr(module, name, t)
+    return t
+
+
 def exp_euler_coeff(tau_syn: torch.Tensor, dt: float) -> torch.Tensor:
     """k = expm1(dt*l)/l with l = -1/tau_s, evaluated by torch on the CPU in
     fp32 exactly as exp_euler_step does (btorch/models/ode.py:50 with the

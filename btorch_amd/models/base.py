@@ -383,15 +383,23 @@ class MemoryModule(StepModule, torch.nn.Module):
                 )
             buf.copy_(v)
 
+    def _sync_memories(self) -> None:
+        """Bring every memory buffer up to date.  A memory may be kept
+        elsewhere between runs (the fused engine holds a delay history on the
+        device) and written back here only when it is read."""
+
     def memories(self):
+        self._sync_memories()
         for name in self._memories_rv:
             yield self._buffers[name]
 
     def named_memories(self):
+        self._sync_memories()
         for name in self._memories_rv:
             yield name, self._buffers[name]
 
     def detach(self):
+        self._sync_memories()
         for key in self._memories_rv:
             buf = self._buffers.get(key)
             if buf is not None:
@@ -399,6 +407,7 @@ class MemoryModule(StepModule, torch.nn.Module):
 
     @property
     def _memories(self):
+        self._sync_memories()
         return {name: self._buffers[name] for name in self._memories_rv}
 
     @_memories.setter

@@ -86,6 +86,27 @@ def named_hidden_states(mod: nn.Module, names: Sequence[str] | None = None,
 named_memory_values = filter_hidden_states = named_hidden_states
 
 
+def hidden_state_names(mod: nn.Module, names: Sequence[str] | None = None,
+                       allow_buffer: bool = False) -> list[str]:
+    """The keys ``named_hidden_states`` would return, without reading any
+    memory (so memories held elsewhere between runs are not written back)."""
+    wanted = _strip_self(names) if names is not None else None
+    out = []
+    for name, m in mod.named_modules():
+        if isinstance(m, base.MemoryModule):
+            keys = list(m._memories_rv)
+        elif allow_buffer:
+            keys = list(m._buffers)
+        else:
+            continue
+        prefix = "" if name == "" else f"{name}."
+        for k in keys:
+            key = prefix + k
+            if wanted is None or name in wanted or key in wanted:
+                out.append(key)
+    return out
+
+
 def set_hidden_states(mod: nn.Module, hidden_states: dict[str, Any], allow_buffer: bool = False,
                       inplace: bool = False):
     """Assign memories from a dotted dict (or ``{"module": {"v": ...}}``)."""
@@ -141,6 +162,6 @@ def detach_net(net: nn.Module):
 
 __all__ = [
     "init_net_state", "reset_net", "reset_net_state", "prepare_sparse_modules",
-    "named_hidden_states", "filter_hidden_states", "set_hidden_states",
+    "named_hidden_states", "filter_hidden_states", "hidden_state_names", "set_hidden_states",
     "named_memory_reset_values", "detach_net",
 ]

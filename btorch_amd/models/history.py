@@ -7,8 +7,13 @@ writes the newest spikes at ``_cursor`` and reads delay d from slot
 the rows of ``expand_conn_for_delays``.
 
 Inside ``RecurrentNN`` the history is replaced by a ring of compacted spike
-lists on the device (no (B, N*D) matrix is ever materialised); this module
-holds the dense form at run boundaries and serves standalone use.
+lists on the device (no (B, N*D) matrix is ever materialised).  Between runs
+the ring stays on the device and the next run continues from it; the dense
+memory is written back from the ring only when something reads it (the
+``history`` or ``_cursor`` attribute, ``memories()``, ``state_dict``,
+``.to()``), so a run followed by another run never pays for the (B, D, N)
+write.  Assigning the memory, ``set_cursor``, ``reset`` and ``init_state``
+end that deferral first, so the module's own value always wins.
 """
 
 from __future__ import annotations
@@ -29,27 +34,89 @@ class SpikeHistory(MemoryModule):
         self.register_memory("history", torch.tensor(0.0, dtype=torch.float32),
                              (self.max_delay, *self.n_neuron))
         self._cursor_host = 0
+        # deferred write-back of the memory (see the module doc): a callable
+        # that fills the ``history`` buffer and the device cursor, or None
+        self._deferred = None
         if use_circular_buffer:
             self.register_buffer("_cursor", torch.tensor(0, dtype=torch.long), persistent=False)
 
+    # -- deferred write-back ------------------------------------------------
+    def _sync_memories(self) -> None:
+        fn = self.__dict__.get("_deferred")
+        if fn is not None:
+            self.__dict__["_deferred"] = None
+            fn(self)
+
+    def defer_writeback(self, fn, cursor: int) -> None:
+        """The memory now lives elsewhere: ``fn(self)`` writes it (and the
+        device cursor) back when it is read.  ``cursor`` is the host cursor
+        after the run that produced it."""
+        self.__dict__["_deferred"] = fn
+        self._cursor_host = int(cursor) % self.max_delay
+
+    def deferred_writeback(self):
+        """The pending write-back callable, or None."""
+        return self.__dict__.get("_deferred")
+
+    def discard_writeback(self) -> None:
+        self.__dict__["_deferred"] = None
+
+    def __getattr__(self, name):
+        if name in ("history", "_cursor"):
+            self._sync_memories()
+        return super().__getattr__(name)
+
+    def __setattr__(self, name, value):
+        if name in ("history", "_cursor") and "_deferred" in self.__dict__:
+            # an assignment replaces what the deferred write-back would bring
+            if name == "_cursor":
+                self._sync_memories()
+            else:
+                self.__dict__["_deferred"] = None
+        super().__setattr__(name, value)
+
+    def named_buffers(self, *args, **kwargs):
+        self._sync_memories()
+        return super().named_buffers(*args, **kwargs)
+
+    def _apply(self, fn, recurse=True):
+        self._sync_memories()
+        return super()._apply(fn, recurse=recurse)
+
+    def _save_to_state_dict(self, destination, prefix, keep_vars):
+        self._sync_memories()
+        super()._save_to_state_dict(destination, prefix, keep_vars)
+
+    def _load_from_state_dict(self, state_dict, prefix, *args, **kwargs):
+        self._sync_memories()
+        super()._load_from_state_dict(state_dict, prefix, *args, **kwargs)
+
+    # -- cursor ---------------------------------------------------------------
     @property
     def cursor(self) -> int:
         """Host copy of the circular cursor (0 in cat mode)."""
         return self._cursor_host if self.use_circular_buffer else 0
 
     def set_cursor(self, value: int) -> None:
+        self._sync_memories()
         self._cursor_host = int(value) % self.max_delay
         if self.use_circular_buffer:
             self._cursor.fill_(self._cursor_host)
 
     def init_state(self, batch_size=None, dtype=None, device=None, persistent=False,
                    skip_mem_name=()):
+        if "history" in tuple(skip_mem_name):
+            self._sync_memories()
+        self.discard_writeback()
         super().init_state(batch_size, dtype, device, persistent, tuple(skip_mem_name))
         if self.use_circular_buffer:
             self._cursor = torch.tensor(0, dtype=torch.long, device=device)
         self._cursor_host = 0
 
     def reset(self, batch_size=None, dtype=None, device=None, skip_mem_name=(), inplace=False):
+        if "history" in tuple(skip_mem_name):
+            self._sync_memories()
+        self.discard_writeback()
         super().reset(batch_size, dtype, device, tuple(skip_mem_name), inplace=inplace)
         if self.use_circular_buffer:
             if inplace:

@@ -89,6 +89,9 @@ class BaseSparseConn(nn.Module):
         self._prepared_sparse_weight: torch.Tensor | None = None
         self._sparse_prepare_depth = 0
         self._workspaces: dict[int, torch.Tensor] = {}
+        # no-grad packed (CSR order) and destination-order weights, reused
+        # while the tensors they derive from are unchanged (weight_key)
+        self._weight_cache: dict[str, tuple] = {}
         prebuilt = None
         if isinstance(conn, _kernels.BuiltConn):
             # built on the device by btx_conn_build (see ``from_edges``)
@@ -173,6 +176,7 @@ class BaseSparseConn(nn.Module):
         self._prepared_sparse_weight = None
         self._sparse_prepare_depth = 0
         self._workspaces = {}
+        self._weight_cache = {}
         return super()._apply(fn, recurse=recurse)
 
     def _load_from_state_dict(self, state_dict, prefix, local_metadata, strict, missing_keys,
@@ -181,6 +185,7 @@ class BaseSparseConn(nn.Module):
                                       unexpected_keys, error_msgs)
         self._prepared_sparse_weight = None
         self._sparse_prepare_depth = 0
+        self._weight_cache = {}
         self._rebuild_layout()
 
     # -- weights -------------------------------------------------------------
@@ -190,19 +195,61 @@ class BaseSparseConn(nn.Module):
     def _get_effective_weight(self) -> torch.Tensor:
         raise NotImplementedError
 
+    _LAYOUT_BUFFERS = ("_csr_row_ptr", "_csr_col", "_csr_perm", "_csc_ptr", "_csc_src")
+
+    def weight_key(self) -> tuple:
+        """Identity, storage and in-place version of every parameter and
+        buffer the effective weight derives from.  Writes through
+        ``tensor.data`` keep the version, so code that edits weights that
+        way assigns ``.data`` (a new storage) or calls ``touch_weights()``."""
+        parts = [self._weight_epoch]
+        for name, t in self._parameters.items():
+            parts.append((name, None if t is None else (id(t), t._version, t.data_ptr())))
+        for name, t in self._buffers.items():
+            if name not in self._LAYOUT_BUFFERS:
+                parts.append((name, None if t is None else (id(t), t._version, t.data_ptr())))
+        return tuple(parts)
+
+    _weight_epoch = 0
+
+    def touch_weights(self) -> None:
+        """Drop the cached packed weights (after an edit ``weight_key`` cannot see)."""
+        self._weight_epoch += 1
+        self._weight_cache = {}
+
+    def _cached_weight(self, kind: str, make) -> torch.Tensor:
+        key = (self.weight_key(), id(self._csr_perm))
+        hit = self._weight_cache.get(kind)
+        if hit is not None and hit[0] == key:
+            return hit[1]
+        with torch.no_grad():
+            w = make()
+        self._weight_cache[kind] = (key, w)
+        return w
+
+    def effective_weight_f32(self) -> torch.Tensor:
+        """Detached float32 effective weights in the coalesced (dst, src) order."""
+        return self._cached_weight(
+            "dst", lambda: self._get_effective_weight().detach().to(torch.float32).contiguous())
+
+    def _pack(self) -> torch.Tensor:
+        if torch.is_grad_enabled():
+            w = self._get_effective_weight()
+            return w.index_select(0, self._csr_perm).contiguous()
+        return self._cached_weight(
+            "csr", lambda: self._get_effective_weight().index_select(0, self._csr_perm)
+            .contiguous())
+
     def packed_weight(self) -> torch.Tensor:
         """Effective weights in CSR order (``magnitude[perm]``)."""
         if self._prepared_sparse_weight is not None:
             return self._prepared_sparse_weight
-        w = self._get_effective_weight()
-        return w.index_select(0, self._csr_perm).contiguous()
+        return self._pack()
 
     def prepare_sparse(self, batch_size: int | None = None) -> None:
         """Pack weights once for a multi-step run (reference-counted)."""
         if self._prepared_sparse_weight is None:
-            w = self._get_effective_weight()
-            packed = w.index_select(0, self._csr_perm).contiguous()
-            self._prepared_sparse_weight = packed if torch.is_grad_enabled() else packed.detach()
+            self._prepared_sparse_weight = self._pack()
         if batch_size is not None and self._csr_col.is_cuda:
             self._workspace_for(int(batch_size))
         self._sparse_prepare_depth += 1

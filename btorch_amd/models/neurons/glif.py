@@ -45,9 +45,8 @@ class _GLIF3Step(torch.autograd.Function):
         plan.iasc = ia_new.data_ptr()
         ref_in = None
         if neuron._use_refractory:
-            ref = neuron.refractory.contiguous()
+            ref = _plan.lane_state(neuron, "refractory", tuple(v_new.shape))
             ref_in = ref.clone()
-            neuron.refractory = ref
             plan.refractory = ref.data_ptr()
         z = torch.empty_like(xb)
         v_pre = torch.empty_like(xb)
@@ -185,8 +184,7 @@ class GLIF3(BaseNode):
         self.v = self.v.contiguous()
         plan.v = self.v.data_ptr()
         if self._use_refractory:
-            self.refractory = self.refractory.contiguous()
-            plan.refractory = self.refractory.data_ptr()
+            plan.refractory = _plan.lane_state(self, "refractory", tuple(self.v.shape)).data_ptr()
         z = torch.empty_like(x)
         v_pre = torch.empty_like(x) if self.pre_spike_v else None
         _lib.check(_lib.load().btx_lif_step(plan, x.data_ptr(), z.data_ptr(),

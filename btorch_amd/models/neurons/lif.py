@@ -126,9 +126,10 @@ class LIF(BaseNode):
         plan.v = v.data_ptr()
         ref_in = None
         if self._use_refractory:
-            ref = self.refractory.contiguous()
+            # the counter covers exactly the lanes the launch updates (an
+            # unbatched counter meeting a batched input gets its own copy)
+            ref = _plan.lane_state(self, "refractory", tuple(v.shape))
             ref_in = ref.clone()
-            self.refractory = ref
             plan.refractory = ref.data_ptr()
         z = torch.empty_like(x)
         v_pre = torch.empty_like(x) if want_pre else None

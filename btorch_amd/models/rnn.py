@@ -22,7 +22,7 @@ import torch.nn as nn
 
 from . import base
 from ._engine import Engine, match_cell
-from .functional import filter_hidden_states, named_hidden_states, prepare_sparse_modules
+from .functional import filter_hidden_states, hidden_state_names, prepare_sparse_modules
 from .linear import BaseSparseConn
 
 
@@ -43,11 +43,39 @@ class RecurrentNNAbstract(base.MemoryModule):
         self.grad_checkpoint = grad_checkpoint
         self.save_grad_history = save_grad_history
         self.grad_state_names = grad_state_names
+        self._grad_history: dict[str, list] = {}
         self.cudagraph = cudagraph
         self.cudagraph_warmup = cudagraph_warmup
+        self._graph_runner = None
 
     def single_step_forward(self, *args, **kwargs):
         raise NotImplementedError
+
+    # -- gradient history (rnn.py:86-104, 319-326, 407-413) ------------------
+    def _should_save_grad(self, state_name: str) -> bool:
+        if not self.save_grad_history:
+            return False
+        return self.grad_state_names is None or state_name in self.grad_state_names
+
+    def get_grad_history(self) -> dict[str, list]:
+        """dL/d(state after step t) of every saved state, filled by backward."""
+        return self._grad_history
+
+    def clear_grad_history(self) -> None:
+        self._grad_history = {}
+
+    def _hook_grad_history(self, states: dict[str, list], T: int) -> None:
+        """Hooks on the per-step state tensors that record their gradient."""
+        for name, tensors in states.items():
+            if not self._should_save_grad(name):
+                continue
+            slots = self._grad_history.setdefault(name, [None] * T)
+            for t, tensor in enumerate(tensors):
+                if torch.is_tensor(tensor) and tensor.requires_grad:
+                    def hook(g, slots=slots, t=t):
+                        slots[t] = g.detach().clone()
+                        return g
+                    tensor.register_hook(hook)
 
     def _detect_loop_args(self, *args):
         if len(args) == 1:
@@ -80,6 +108,10 @@ class RecurrentNNAbstract(base.MemoryModule):
 
     def _multi_step_loop(self, *args, loop_args=None, **kwargs):
         T, loop_args, _, large, nchunks = self._chunk_plan(*args, loop_args=loop_args)
+        if self.save_grad_history:
+            self._grad_history = {}
+            if self.grad_state_names:
+                self._grad_history = {k: [None] * T for k in self.grad_state_names}
         zs, states = [], {}
         for c in range(nchunks):
             lo, hi = c * large, min(T, (c + 1) * large)
@@ -94,6 +126,8 @@ class RecurrentNNAbstract(base.MemoryModule):
                 zs.append(z)
                 for k, v in st.items():
                     states.setdefault(k, []).append(v)
+        if self.save_grad_history and torch.is_grad_enabled():
+            self._hook_grad_history(states, T)
         return torch.stack(zs, 0), {k: torch.stack(v, 0) for k, v in states.items()}
 
 
@@ -152,8 +186,22 @@ class RecurrentNN(RecurrentNNAbstract):
                                        allow_buffer=self.allow_buffer)
 
     # -- fused path ----------------------------------------------------------
+    def _engine_identity(self) -> tuple:
+        """Modules and layout tensors a matched CellSpec froze: a change of
+        any of them (a new connection, a reloaded layout, a new history
+        module) matches the cell again."""
+        syn = self.synapse
+        inner = getattr(syn, "psc_module", None) or getattr(syn, "base_psc", None)
+        conn = getattr(syn, "linear", None) or getattr(inner, "linear", None)
+        hist = getattr(syn, "history", None)
+        nrn = self.neuron
+        groups = tuple(id(g) for g in nrn.groups()) if hasattr(nrn, "groups") else ()
+        return (id(nrn), groups, id(syn), id(inner), id(self.syn_inp_module), id(conn),
+                id(getattr(conn, "_csr_row_ptr", None)), id(hist),
+                id(getattr(self, "synapse_apical", None)))
+
     def fused_engine(self) -> Engine | None:
-        key = (id(self.neuron), id(self.synapse), id(self.syn_inp_module))
+        key = self._engine_identity()
         if self._engine is None or self._engine_key != key:
             spec = match_cell(self)
             self._engine = Engine(spec) if spec is not None else None
@@ -161,27 +209,43 @@ class RecurrentNN(RecurrentNNAbstract):
         return self._engine
 
     def _record_names(self) -> list[str]:
-        names = list(named_hidden_states(self, self.update_state_names,
-                                         allow_buffer=self.allow_buffer).keys())
-        return names
+        return hidden_state_names(self, self.update_state_names, allow_buffer=self.allow_buffer)
+
+    def _graph(self, eng, x: torch.Tensor):
+        """The graph runner when ``cudagraph`` is set (checks first), else None."""
+        if not self.cudagraph:
+            return None
+        from .cudagraph import GraphRunner
+        GraphRunner.check_supported(self, x)
+        if eng is None:
+            raise NotImplementedError("cudagraph=True replays the fused engine; this cell is "
+                                      "not one it runs")
+        if self._graph_runner is None:
+            self._graph_runner = GraphRunner(warmup=self.cudagraph_warmup)
+        return self._graph_runner
 
     def _multi_step_loop(self, *args, loop_args=None, **kwargs):
         eng = self.fused_engine()
         x = args[0] if args else None
         if (eng is None or len(args) != 1 or kwargs or not torch.is_tensor(x)
                 or x.device.type != "cuda"):
+            if self.cudagraph:
+                raise NotImplementedError("cudagraph=True replays the fused engine over one "
+                                          "input sequence on a HIP device")
             return super()._multi_step_loop(*args, loop_args=loop_args, **kwargs)
         from ._adjoint import fused_adjoint_covers, needs_grad, run_with_grad
 
+        graph = self._graph(eng, x)
         if needs_grad(self, x):
             lanes = x[0].numel() // max(1, x.shape[-1]) if x.ndim > 1 else 1
-            if fused_adjoint_covers(eng.spec, lanes):
+            if fused_adjoint_covers(eng.spec, lanes) and not self.save_grad_history:
                 return run_with_grad(self, eng, x)
             # autograd through the standalone HIP steps (which raise for
-            # what they cannot differentiate)
+            # what they cannot differentiate); per-step states carry the
+            # gradient hooks of save_grad_history
             return super()._multi_step_loop(*args, loop_args=loop_args, **kwargs)
         with torch.no_grad():
-            return self._fused_forward(eng, x)
+            return self._fused_forward(eng, x, graph=graph)
 
     def _input_sequence(self, x: torch.Tensor, n: int):
         T = x.shape[0]
@@ -193,7 +257,8 @@ class RecurrentNN(RecurrentNNAbstract):
         x = x.reshape(T, -1, n) if x.numel() else x
         return x.to(torch.float32).contiguous(), lead
 
-    def _fused_forward(self, eng: Engine, x: torch.Tensor, x_apical: torch.Tensor | None = None):
+    def _fused_forward(self, eng: Engine, x: torch.Tensor, x_apical: torch.Tensor | None = None,
+                       graph=None):
         sp = eng.spec
         n = sp.neuron.size
         T = x.shape[0]
@@ -208,7 +273,12 @@ class RecurrentNN(RecurrentNNAbstract):
         hist0 = sp.history.history.clone() if (sp.history is not None and
                                                "history" in want) else None
         cursor0 = sp.history.cursor if sp.history is not None else 0
-        raster, recs = eng.run(xs, T, lanes, want, x_apical=xa)
+        if graph is not None:
+            if xa is not None:
+                raise NotImplementedError("cudagraph=True replays runs without an apical input")
+            raster, recs = graph.run(eng, xs, want)
+        else:
+            raster, recs = eng.run(xs, T, lanes, want, x_apical=xa)
         return self._assemble(eng, names, recs, raster, hist0, cursor0, T, lanes, n)
 
     # neuron memories the engine records, by name within the neuron module
@@ -313,20 +383,25 @@ class ApicalRecurrentNN(RecurrentNN):
                    and (x_apical is None or (torch.is_tensor(x_apical)
                                              and x_apical.shape[0] == x.shape[0])))
         if not fusable:
+            if self.cudagraph:
+                raise NotImplementedError("cudagraph=True replays the fused engine; this call "
+                                          "is not one it runs")
             return RecurrentNNAbstract._multi_step_loop(self, *args, loop_args=loop_args,
                                                         **kwargs)
+        graph = self._graph(eng, x)
         from ._adjoint import fused_adjoint_covers, needs_grad, run_with_grad
 
         if needs_grad(self, x) or (x_apical is not None and x_apical.requires_grad
                                    and torch.is_grad_enabled()):
             lanes = x[0].numel() // max(1, x.shape[-1]) if x.ndim > 1 else 1
-            if x_apical is None and fused_adjoint_covers(eng.spec, lanes):
+            if (x_apical is None and fused_adjoint_covers(eng.spec, lanes)
+                    and not self.save_grad_history):
                 return run_with_grad(self, eng, x)
             # autograd through the standalone HIP steps
             return RecurrentNNAbstract._multi_step_loop(self, *args, loop_args=loop_args,
                                                         **kwargs)
         with torch.no_grad():
-            return self._fused_forward(eng, x, x_apical)
+            return self._fused_forward(eng, x, x_apical, graph=graph)
 
 
 class SomaApicalRecurrentNN(ApicalRecurrentNN):

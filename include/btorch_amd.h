@@ -382,9 +382,23 @@ typedef struct btx_rsnn_plan {
     float* rec_syn_h2;    /* [T][lanes][n] or NULL */
     /* workspace from btx_rsnn_workspace_bytes(), zero-filled once */
     void* workspace;      size_t workspace_bytes;
+    /* 1 = the workspace already holds this plan's delay-bin tables (an
+     * earlier run with the same workspace, nbins, bin_delay and bin_sub
+     * wrote them), so the run skips writing them; 0 = write them first */
+    int tables_ready;
 } btx_rsnn_plan;
 
 size_t btx_rsnn_workspace_bytes(int64_t lanes, int64_t n, int slots, int ordered);
+
+/* The in-kernel Poisson drive of btx_rsnn_plan as a dense input tensor:
+ * x[t, b, j] = kick when the uniform of neuron j of lane b at local step
+ * t0[b] + t falls below prob[b], else 0 (the same SplitMix64 numbers the
+ * fused step draws), for t in [0, T).  A run given this x and no drive
+ * produces the same bits as a run given the drive.  Input generator of the
+ * benchmark workloads (Poisson background as in btorch/datasets/noise.py);
+ * it has no counterpart on the reference's step path. */
+int btx_poisson_fill(const uint64_t* seed, const float* prob, const int64_t* t0, float kick,
+                     int64_t T, int64_t lanes, int64_t n, float* x, void* stream);
 
 /* Run steps [0, n_steps) of the plan.  Records index t in [0, n_steps).
  * State (v, refractory, psc, ring) is final on return (stream order). */

@@ -545,12 +545,14 @@ SUPERSPIKE_K = np.sqrt(2.0) - 1.0
 
 
 def surrogate_derivative(kind: str, u, alpha: float, damping: float):
-    """sigmoid.py:22-29, superspike.py:25-31, erf.py:22-28."""
+    """sigmoid.py:22-29, superspike.py:25-31, erf.py:22-28, atan.py:22-27."""
     if kind == "sigmoid":
         s = torch.sigmoid(SIGMOID_K * alpha * u)
         return damping * 4.0 * s * (1.0 - s)
     if kind == "superspike":
         return damping / (1.0 + SUPERSPIKE_K * alpha * u.abs()) ** 2
+    if kind == "atan":
+        return damping / (1.0 + (alpha * u) ** 2)
     return damping * torch.exp(-np.log(2.0) * (alpha * u) ** 2)
 
 
@@ -560,6 +562,8 @@ def surrogate_primitive(kind: str, u, alpha: float):
     if kind == "superspike":
         ka = SUPERSPIKE_K * alpha
         return torch.where(u >= 0.0, 1.0 - 0.5 / (1.0 + ka * u), 0.5 / (1.0 - ka * u))
+    if kind == "atan":  # atan.py:10-12
+        return 0.5 + torch.atan(alpha * u) / np.pi
     return torch.special.erfc(-np.sqrt(np.log(2.0)) * alpha * u) / 2.0
 
 

@@ -380,3 +380,68 @@ def test_fused_adjoint_coverage_decision():
     trainable_vth = match_cell(RecurrentNN(LIF(n, trainable_param={"v_threshold"}),
                                            ExponentialPSC(n, 5.0, SparseConn(m))))
     assert not fused_adjoint_covers(trainable_vth, 8)
+
+
+# ---------------------------------------------------------------------------
+# state shapes, engine identity, deferred history write-back (host logic)
+# ---------------------------------------------------------------------------
+
+def test_lane_state_broadcasts_or_raises_before_any_launch():
+    from btorch_amd.models import LIF, _plan
+    n = 10
+    lif = LIF(n, tau_ref=2.0)
+    lif.init_state()
+    assert lif.refractory.shape == (n,)
+    ref = _plan.lane_state(lif, "refractory", (4, n))
+    assert ref.shape == (4, n) and ref.is_contiguous() and lif.refractory is ref
+    lif.refractory = torch.zeros(3, n)
+    with pytest.raises(RuntimeError, match="does not match"):
+        _plan.lane_state(lif, "refractory", (4, n))
+    lif.refractory = None
+    with pytest.raises(RuntimeError, match="not initialised"):
+        _plan.lane_state(lif, "refractory", (4, n))
+
+
+def test_engine_identity_follows_connection_and_layout():
+    import scipy.sparse
+    from btorch_amd.models import ExponentialPSC, LIF, RecurrentNN, SparseConn
+    n = 12
+    m = scipy.sparse.random(n, n, density=0.3, random_state=0, dtype=np.float32)
+    model = RecurrentNN(LIF(n), ExponentialPSC(n, 5.0, SparseConn(m)), step_mode="m")
+    k0 = model._engine_identity()
+    model.synapse.linear = SparseConn(m)
+    k1 = model._engine_identity()
+    assert k1 != k0
+    model.synapse.linear.load_state_dict(model.synapse.linear.state_dict())
+    assert model._engine_identity() != k1  # the layout was rebuilt on load
+
+
+def test_spike_history_deferred_writeback_semantics():
+    from btorch_amd.models import SpikeHistory
+    h = SpikeHistory(5, max_delay_steps=3)
+    h.init_state(batch_size=2, persistent=True)
+    calls = []
+
+    def wb(mod):
+        calls.append(1)
+        mod._buffers["history"].fill_(1.0)
+
+    # a read writes back once, at the read
+    h.defer_writeback(wb, cursor=2)
+    assert h.cursor == 2 and not calls
+    assert float(h.history.sum()) == 2 * 3 * 5 and len(calls) == 1
+    h.history  # noqa: B018
+    assert len(calls) == 1
+    # state_dict and memories() write back; assigning the memory discards it
+    h.defer_writeback(wb, cursor=0)
+    h._buffers["history"].zero_()
+    assert float(h.state_dict()["history"].sum()) == 30 and len(calls) == 2
+    h.defer_writeback(wb, cursor=0)
+    h.history = torch.zeros(2, 3, 5)
+    assert float(h.history.sum()) == 0 and len(calls) == 2
+    h.defer_writeback(wb, cursor=1)
+    h.set_cursor(2)
+    assert len(calls) == 3 and h.cursor == 2
+    h.defer_writeback(wb, cursor=1)
+    h.reset()
+    assert float(h.history.sum()) == 0 and len(calls) == 3

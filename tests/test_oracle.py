@@ -575,3 +575,158 @@ def test_oracle_product_matches_committed_sparse_mm_fixture():
         return res
     assert np.array_equal(seq(lambda e: e), out)
     assert not np.array_equal(seq(lambda e: e[::-1]), out)
+
+
+# ---------------------------------------------------------------------------
+# surrogates: the reference's own invariants (tests/models/test_surrogate.py:
+# 82-116) -- g(0) = damping and half width at half maximum = 1/alpha -- and
+# the primitive's derivative equal to the stated derivative
+# ---------------------------------------------------------------------------
+
+SURROGATE_KINDS = ("sigmoid", "superspike", "erf", "atan")
+ALPHAS = (1.0, 2.0, 4.0, 25.0 / (math.sqrt(2.0) - 1.0))
+
+
+def _hwhm(g, alpha):
+    """Largest u >= 0 with g(u) >= g(0)/2, by bisection on [0, 10/alpha]."""
+    g0 = float(g(torch.zeros((), dtype=torch.float64)))
+    lo, hi = 0.0, 10.0 / alpha
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if float(g(torch.tensor(mid, dtype=torch.float64))) >= 0.5 * g0:
+            lo = mid
+        else:
+            hi = mid
+    return lo
+
+
+@pytest.mark.parametrize("kind", SURROGATE_KINDS)
+@pytest.mark.parametrize("alpha", ALPHAS)
+def test_oracle_surrogate_unit_peak_and_hwhm(kind, alpha):
+    for damping in (1.0, 0.5):
+        def g(u):
+            return rsnn.surrogate_derivative(kind, u, alpha, damping)
+        assert abs(float(g(torch.zeros((), dtype=torch.float64))) - damping) < 1e-12
+        assert abs(_hwhm(g, alpha) - 1.0 / alpha) < 1e-9 / alpha
+        # symmetric, decreasing away from the threshold
+        u = torch.linspace(0.0, 5.0 / alpha, 101, dtype=torch.float64)
+        assert torch.allclose(g(u), g(-u), rtol=1e-12, atol=0)
+        assert bool((g(u)[1:] <= g(u)[:-1]).all())
+
+
+@pytest.mark.parametrize("kind", SURROGATE_KINDS)
+@pytest.mark.parametrize("alpha", ALPHAS)
+def test_oracle_surrogate_primitive_integrates_to_derivative(kind, alpha):
+    # the primitive (the non-spiking forward) has the derivative's shape:
+    # d/du primitive(u) (autograd, float64), divided by its value at the
+    # threshold, equals the unit-peak derivative (the reference normalises
+    # the gradient to 1 at threshold, not the primitive's slope);
+    # primitive(0) = 1/2 and it runs from 0 to 1
+    u = torch.linspace(-4.0 / alpha, 4.0 / alpha, 401, dtype=torch.float64, requires_grad=True)
+    p = rsnn.surrogate_primitive(kind, u, alpha)
+    (gp,) = torch.autograd.grad(p.sum(), u)
+    d = rsnn.surrogate_derivative(kind, u.detach(), alpha, 1.0)
+    assert torch.allclose(gp / gp[200], d, rtol=1e-10, atol=1e-12)
+    # its peak slope is the closed form of each primitive
+    slope0 = {"sigmoid": rsnn.SIGMOID_K * alpha / 4.0,
+              "superspike": 0.5 * rsnn.SUPERSPIKE_K * alpha,
+              "erf": math.sqrt(math.log(2.0)) * alpha / math.sqrt(math.pi),
+              "atan": alpha / math.pi}[kind]
+    assert abs(float(gp[200]) - slope0) < 1e-10 * slope0
+    zero = torch.zeros((), dtype=torch.float64)
+    assert abs(float(rsnn.surrogate_primitive(kind, zero, alpha)) - 0.5) < 1e-12
+    big = torch.tensor([-1e7 / alpha, 1e7 / alpha], dtype=torch.float64)
+    ends = rsnn.surrogate_primitive(kind, big, alpha)
+    assert float(ends[0]) < 1e-6 and float(ends[1]) > 1 - 1e-6
+
+
+def test_oracle_surrogate_mistakes_are_caught():
+    # a dropped constant (slope alpha instead of the HWHM-consistent scale)
+    # moves the half width: the invariant above would fail
+    alpha = 4.0
+    wrong_sig = _hwhm(lambda u: 4 * torch.sigmoid(alpha * u) * (1 - torch.sigmoid(alpha * u)),
+                      alpha)
+    wrong_ss = _hwhm(lambda u: 1.0 / (1.0 + alpha * u.abs()) ** 2, alpha)
+    wrong_erf = _hwhm(lambda u: torch.exp(-(alpha * u) ** 2), alpha)
+    for hw in (wrong_sig, wrong_ss, wrong_erf):
+        assert abs(hw - 1.0 / alpha) > 1e-2 / alpha
+
+
+@pytest.mark.parametrize("kind", SURROGATE_KINDS)
+def test_product_surrogate_classes_equal_oracle(kind):
+    # the package's surrogate classes (used by the standalone steps' torch
+    # backward) agree with the pinned oracle forms
+    from btorch_amd.models import surrogate
+    cls = {"sigmoid": surrogate.Sigmoid, "superspike": surrogate.SuperSpike,
+           "erf": surrogate.Erf, "atan": surrogate.ATan}[kind]
+    for alpha in ALPHAS:
+        fn = cls(alpha=alpha, damping_factor=0.7)
+        u = torch.linspace(-3.0 / alpha, 3.0 / alpha, 257, dtype=torch.float64)
+        g = torch.ones_like(u)
+        got = fn.derivative(u, g, fn.damping_factor)
+        ref = rsnn.surrogate_derivative(kind, u, alpha, 0.7)
+        assert torch.allclose(got, ref, rtol=1e-12, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# step composition: hand-derived known answers of the reference's end-to-end
+# test (tests/models/test_delayed_psc.py:384-450): identity weights, x = 2,
+# dt = 1, LIF tau 20, threshold 1, soft reset, exponential synapse tau 2
+# (a = e^-0.5 per step).  Each step: I_t = psc_{t-1} + x ; v <- v(1 - dt/tau)
+# + dt*I, fire at v >= 1, v -= 1 ; psc_t = a*psc_{t-1} + z_{t-d}.
+# ---------------------------------------------------------------------------
+
+A_HALF = math.exp(-0.5)
+# undelayed synapse (d = 0): psc_t = a psc_{t-1} + z_t
+KNOWN_V0 = [1.0, 2.95, 4.8025 + A_HALF]
+KNOWN_PSC0 = [1.0, 1.0 + A_HALF, 1.0 + A_HALF + A_HALF ** 2]
+# delay of two steps (DelayedPSC(max_delay_steps=2); HeterSynapsePSC with
+# every delay 2): psc_t = a psc_{t-1} + z_{t-2}
+KNOWN_V2 = [1.0, 1.95, 2.8525, 4.709875, 6.47438125 + A_HALF]
+KNOWN_PSC2 = [0.0, 0.0, 1.0, 1.0 + A_HALF, 1.0 + A_HALF + A_HALF ** 2]
+
+
+def _eye_net(n, mode, D):
+    eye = scipy.sparse.coo_array(np.eye(n, dtype=np.float32))
+    if mode == "hetero":
+        # every synapse delayed by D - 1 steps: row pre*D + (D - 1)
+        conn = rsnn.expand_conn_for_delays(eye, np.full(n, D - 1), D)
+        kw = dict(delay_mode="hetero", max_delay_steps=D, circular=True)
+    elif mode == "fixed":
+        conn, kw = eye, dict(delay_mode="fixed", max_delay_steps=D, circular=True)
+    else:
+        conn, kw = eye, {}
+    lif = rsnn.LIFParams.make(n, tau=20.0)
+    return rsnn.Net(n, conn, lif, torch.tensor(2.0), 1.0, **kw)
+
+
+@pytest.mark.parametrize("mode,D,v_ref,psc_ref", [
+    ("none", 1, KNOWN_V0, KNOWN_PSC0),
+    ("fixed", 2, KNOWN_V2, KNOWN_PSC2),
+    ("hetero", 3, KNOWN_V2, KNOWN_PSC2),
+])
+def test_step_composition_known_answers(mode, D, v_ref, psc_ref):
+    n, B = 4, 2
+    T = len(v_ref)
+    x = torch.full((T, B, n), 2.0, dtype=torch.float64)
+    out = rsnn.simulate(_eye_net(n, mode, D), x, dtype=torch.float64)
+    assert bool((out["z"] == 1).all())
+    v = out["v"][:, 0, 0].tolist()
+    psc = out["psc"][:, 0, 0].tolist()
+    assert np.allclose(v, v_ref, rtol=0, atol=1e-12), (v, v_ref)
+    assert np.allclose(psc, psc_ref, rtol=0, atol=1e-12), (psc, psc_ref)
+    assert torch.equal(out["v"], out["v"][:, :1, :1].expand_as(out["v"]))
+
+
+def test_step_composition_mistakes_fail_the_known_answers():
+    # plausible misreadings of rnn.py:517-528 give other numbers: the neuron
+    # reading psc after this step's push, or the push delivering one step late
+    v_ref, psc_ref = KNOWN_V0, KNOWN_PSC0
+    a = A_HALF
+    # neuron reads the freshly pushed psc: I_0 = 1 + 2
+    v_wrong = 0.0 * 0.95 + 3.0 - 1.0
+    assert abs(v_wrong - v_ref[0]) > 0.5
+    # push before decay: psc_1 = a*(1 + 1) instead of a + 1
+    assert abs(a * 2.0 - psc_ref[1]) > 0.1
+    # off-by-one delay: the d = 2 trace would equal the d = 3 one
+    assert KNOWN_PSC2[2] != 0.0
