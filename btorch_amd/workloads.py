@@ -189,3 +189,25 @@ def sheet_net_device(side: int = 1000, mean_degree: float = 1000.0, sigma: float
         w[s:s + chunk] = torch.where(inh, -inh_scale * mag, mag)
     delay = torch.randint(delay_lo, delay_hi + 1, (E,), generator=g, device=device)
     return pre, post, w, delay
+
+
+def connectome_140k(n: int = 140_000, k_min: int = 100, k_max: int = 5000, delay_max: int = 20,
+                    seed: int = 1, w_mu: float = -3.7, device="cuda", sparse_backend="native"):
+    """The 140k-neuron connectome workload: power-law out-degree (exponent
+    2.1) on [k_min, k_max] (E ~ 50M at k_min 100), 70/30 E/I,
+    lognormal(w_mu, 0.5) weights with inhibitory x4, delays U{1..delay_max}
+    (D = delay_max + 1 bins), built on the device as a delay-expanded
+    SparseConn; per-neuron tau_m ~ U(10, 30) ms.  Returns (conn, tau, nnz)."""
+    import torch
+
+    from .models import SparseConn
+
+    D = delay_max + 1
+    c = powerlaw_net_fast(n=n, k_min=k_min, k_max=k_max, exc_frac=0.7, delay_lo=1,
+                          delay_hi=delay_max, seed=seed, w_mu=w_mu)
+    edges = [torch.from_numpy(a).to(device) for a in (c.pre, c.post, c.w, c.delay)]
+    conn = SparseConn.from_edges(*edges[:3], edges[3], D, shape=(n, n),
+                                 sparse_backend=sparse_backend)
+    rng = np.random.default_rng(seed + 7)
+    tau = torch.from_numpy(rng.uniform(10.0, 30.0, n).astype(np.float32))
+    return conn, tau, c.nnz
