@@ -22,6 +22,8 @@ RING_CHUNK = 1024
 NEURON_LIF = 0
 NEURON_GLIF3 = 1
 NEURON_TWO_COMPARTMENT = 2
+NEURON_MIXED = 3
+MIXED_HARD_RESET = 4
 
 SYN_EXP = 0
 SYN_ALPHA = 1
@@ -128,6 +130,8 @@ class RsnnPlan(ctypes.Structure):
         ("workspace", _vp),
         ("workspace_bytes", ctypes.c_size_t),
         ("tables_ready", _c_int),
+        ("neuron_kinds", _vp),
+        ("mixed_models", _c_int),
     ]
 
 

@@ -98,6 +98,9 @@ constexpr int kMaxAsc = 4;
 constexpr int kLif = 0;
 constexpr int kGlif3 = 1;
 constexpr int kTwoComp = 2;
+// Several neuron models on one axis: each neuron's model (and reset mode)
+// comes from StepArgs::kinds; the state arrays cover every neuron.
+constexpr int kMixed = 3;
 // Flag on a kind: the synapse has kinetics other than the exponential one
 // (a second state h, or two conductances), chosen by StepArgs::syn.
 constexpr int kSynX = 4;
@@ -109,7 +112,8 @@ struct StepArgs {
     float dt;
     int hard;
     int use_ref;
-    int kind;              // kLif, kGlif3 or kTwoComp
+    int kind;              // kLif, kGlif3, kTwoComp or kMixed
+    const uint8_t* kinds;  // kMixed: per-neuron model code (BTX_MIXED_*)
     int n_asc;
     const float* vrest;    int64_t vrest_s;
     const float* kv;       int64_t kv_s;
@@ -446,7 +450,8 @@ template <int KIND, bool REC, bool SHR, bool SX = false>
 __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane, int64_t nidx,
                                                 int64_t i, float& psc, float wz, float xin,
                                                 float drive, float& v, float& ref, float* ia,
-                                                const PreParams& q, float tau, Syn2* s2 = nullptr) {
+                                                const PreParams& q, float tau, Syn2* s2 = nullptr,
+                                                int hard = -1) {
     const float I = SX ? syn_x_input<REC>(a, i, nidx, psc, *s2, wz, xin, drive)
                        : soma_input<REC>(a, i, psc, wz, xin, drive);
     const float vth = prm_sh<SHR>(a.vth, a.vth_s, nidx, q.vth);
@@ -481,7 +486,8 @@ __device__ __forceinline__ float lif_exp_neuron(const StepArgs& a, int64_t lane,
         ref = (r2 < 0.f) ? 0.f : r2;
         if (REC && a.rec_ref) a.rec_ref[i] = ref;
     }
-    v = a.hard ? (v1 - (v1 - vr) * s) : (v1 - (vth - vr) * s);
+    const bool hr = hard < 0 ? a.hard != 0 : hard != 0;
+    v = hr ? (v1 - (v1 - vr) * s) : (v1 - (vth - vr) * s);
     if (KIND == kGlif3) {
         const int64_t qa = nidx * a.n_asc;
 #pragma unroll
@@ -780,8 +786,9 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
         }
         return;
     }
-    constexpr bool GLIF = KIND == kGlif3;
-    constexpr bool TC = KIND == kTwoComp;
+    constexpr bool MIX = KIND == kMixed;
+    constexpr bool GLIF = KIND == kGlif3 || MIX;
+    constexpr bool TC = KIND == kTwoComp || MIX;
     const int tid = threadIdx.x;
     const int64_t seg_base = seg * kChunk;
     const int64_t n0 = seg_base + tid * 4;
@@ -795,7 +802,7 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     PreParams q;
     q.vth = ld_const(a.vth);
     q.vr = ld_const(a.vr);
-    q.cm = TC ? 1.f : ld_const(a.cm);
+    q.cm = KIND == kTwoComp ? 1.f : ld_const(a.cm);
     q.tref = a.use_ref ? ld_const(a.tref) : 0.f;
     q.tsyn = a.psc ? ld_const(a.tsyn) : 1.f;
     q.ksyn = a.psc ? ld_const(a.ksyn) : 0.f;
@@ -814,15 +821,21 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     }
     if (a.use_ref) load_lane4(a.ref, i0, room, safe, vec, r);
     if (a.x) load_lane4(a.x, i0, room, safe, vec, x);
-    if (TC) {
+    if (TC && (!MIX || a.tia != nullptr)) {
         load_lane4(a.tia, i0, room, safe, vec, ta);
         load_lane4(a.tibap, i0, room, safe, vec, tb);
         load_lane4(a.tth, i0, room, safe, vec, tt);
         if (a.xa) load_lane4(a.xa, i0, room, safe, vec, xa);
     }
+    uint32_t kind4 = 0;  // kMixed: model codes of the four neurons, a byte each
+    if (MIX) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < room) kind4 |= (uint32_t)a.kinds[n0 + k] << (8 * k);
+    }
     // the membrane time constants, per neuron (shared by the lanes) or one value
     float tau4[4] = {1.f, 1.f, 1.f, 1.f};
-    if (!TC) {
+    if (!TC || MIX) {
         if (a.tau_s) {
             load_lane4(a.tau, n0, room, 0, vec, tau4);
         } else {
@@ -835,7 +848,8 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int j = 0; j < (GLIF ? kMaxAsc : 1); ++j)
-            ia[k][j] = (GLIF && j < a.n_asc && k < room) ? a.iasc[(i0 + k) * a.n_asc + j] : 0.f;
+            ia[k][j] = (GLIF && a.iasc != nullptr && j < a.n_asc && k < room)
+                           ? a.iasc[(i0 + k) * a.n_asc + j] : 0.f;
     // the drive's hashing runs while the loads are in flight
     const uint32_t kicks = poisson_drive4(a, dargs, n0);
     float wz[4] = {0.f, 0.f, 0.f, 0.f};
@@ -850,7 +864,30 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     // general or with every parameter but tau shared and c_m == 1 (fewer
     // uniform values live through the loop)
     const bool shr = a.shared && q.cm == 1.f;
-    if (SX) {
+    if (MIX) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < room) {
+                const float drv = ((kicks >> k) & 1u) ? a.in_kick : 0.f;
+                const uint32_t code = (kind4 >> (8 * k)) & 0xffu;
+                const uint32_t model = code & BTX_MIXED_MODEL_MASK;
+                const int hard = (code & BTX_MIXED_HARD_RESET) ? 1 : 0;
+                float s;
+                if (model == BTX_NEURON_TWO_COMPARTMENT)
+                    s = two_comp_neuron<true>(a, lane, n0 + k, i0 + k, p[k], wz[k], x[k], drv,
+                                              xa[k], v[k], ta[k], tb[k], tt[k], q);
+                else if (model == BTX_NEURON_GLIF3)
+                    s = lif_exp_neuron<kGlif3, true, false>(a, lane, n0 + k, i0 + k, p[k], wz[k],
+                                                            x[k], drv, v[k], r[k], ia[k], q,
+                                                            tau4[k], nullptr, hard);
+                else
+                    s = lif_exp_neuron<kLif, true, false>(a, lane, n0 + k, i0 + k, p[k], wz[k],
+                                                          x[k], drv, v[k], r[k], ia[k], q,
+                                                          tau4[k], nullptr, hard);
+                nib |= (uint32_t)(s != 0.f) << k;
+            }
+        }
+    } else if (SX) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (k < room) {
@@ -901,7 +938,7 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
             }
         }
     }
-    if (GLIF) {
+    if (GLIF && (!MIX || a.iasc != nullptr)) {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -916,7 +953,7 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
             store_lane4(a.sh1, i0, room, vec, h1);
             if (a.syn == BTX_SYN_DUAL_EXP) store_lane4(a.sh2, i0, room, vec, h2);
         }
-        if (TC) {
+        if (TC && (!MIX || a.tia != nullptr)) {
             store_lane4(a.tia, i0, room, vec, ta);
             if (bap_live || any4(tb)) store_lane4(a.tibap, i0, room, vec, tb);
             store_lane4(a.tth, i0, room, vec, tt);
@@ -1073,7 +1110,7 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 // [np, gridDim.x) play F.
 template <int G, bool ORDERED, int KIND>
 __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArgs a, PushStep p, int np) {
-    // KIND: kLif, kGlif3 or kTwoComp, with kSynX for the alpha and
+    // KIND: kLif, kGlif3, kTwoComp or kMixed, with kSynX for the alpha and
     // dual-exponential synapses
     __shared__ StepLds lds;
     const int64_t lane = blockIdx.y;
@@ -1272,8 +1309,12 @@ static StepArgs neuron_args(const btx_rsnn_plan* p) {
     a.ref = p->refractory;
     a.kind = p->neuron_model == BTX_NEURON_GLIF3             ? kGlif3
              : p->neuron_model == BTX_NEURON_TWO_COMPARTMENT ? kTwoComp
+             : p->neuron_model == BTX_NEURON_MIXED           ? kMixed
                                                              : kLif;
-    if (a.kind == kTwoComp) {
+    const bool mixed = a.kind == kMixed;
+    if (mixed) a.kinds = p->neuron_kinds;
+    if (a.kind == kTwoComp ||
+        (mixed && (p->mixed_models & (1 << BTX_NEURON_TWO_COMPARTMENT)))) {
         a.tia = p->i_a;
         a.tibap = p->i_bap;
         a.tth = p->theta_th;
@@ -1282,7 +1323,7 @@ static StepArgs neuron_args(const btx_rsnn_plan* p) {
             a.tcs[k] = p->tc_stride[k];
         }
     }
-    if (a.kind == kGlif3) {
+    if (a.kind == kGlif3 || (mixed && (p->mixed_models & (1 << BTX_NEURON_GLIF3)))) {
         a.n_asc = p->n_asc;
         a.vrest = p->v_rest; a.vrest_s = p->v_rest_stride;
         a.kv = p->k_v; a.kv_s = p->k_v_stride;
@@ -1298,17 +1339,30 @@ static bool aligned16(const void* q) { return ((uintptr_t)q & 15) == 0; }
 
 static int check_neuron(const btx_rsnn_plan* p) {
     if (p->neuron_model != BTX_NEURON_LIF && p->neuron_model != BTX_NEURON_GLIF3 &&
-        p->neuron_model != BTX_NEURON_TWO_COMPARTMENT)
+        p->neuron_model != BTX_NEURON_TWO_COMPARTMENT && p->neuron_model != BTX_NEURON_MIXED)
         return fail("unsupported neuron model");
-    if (p->neuron_model == BTX_NEURON_TWO_COMPARTMENT) {
+    const bool mixed = p->neuron_model == BTX_NEURON_MIXED;
+    const int models = mixed ? p->mixed_models : (1 << p->neuron_model);
+    if (mixed) {
+        if (p->neuron_kinds == nullptr) return fail("a mixed population needs neuron_kinds");
+        if (models == 0 || (models & ~7) != 0)
+            return fail("mixed_models must name LIF, GLIF3 and/or two-compartment neurons");
+        if (p->syn_kind != BTX_SYN_EXP)
+            return fail("a mixed population runs with the exponential synapse");
+        if (p->tau == nullptr || p->c_m == nullptr || p->v_th == nullptr ||
+            p->v_reset == nullptr)
+            return fail("a mixed population needs tau, c_m, v_th and v_reset for every neuron");
+    }
+    if (models & (1 << BTX_NEURON_TWO_COMPARTMENT)) {
         if (p->i_a == nullptr || p->i_bap == nullptr || p->theta_th == nullptr)
             return fail("two-compartment neuron needs i_a, i_bap and theta_th state");
         for (int k = 0; k < BTX_TC_NPARAM; ++k)
             if (p->tc[k] == nullptr || p->tc_stride[k] < 0 || p->tc_stride[k] > 1)
                 return fail("two-compartment neuron needs every coefficient (stride 0 or 1)");
-        if (p->use_refractory) return fail("two-compartment neuron has no refractory state");
+        if (p->use_refractory && !mixed)
+            return fail("two-compartment neuron has no refractory state");
     }
-    if (p->neuron_model == BTX_NEURON_GLIF3 &&
+    if ((models & (1 << BTX_NEURON_GLIF3)) &&
         (p->n_asc < 1 || p->n_asc > kMaxAsc || p->v_rest == nullptr || p->k_v == nullptr ||
          p->asc_k == nullptr || p->asc_kk == nullptr || p->asc_amp == nullptr ||
          p->iasc == nullptr))
@@ -1331,7 +1385,9 @@ static int vec4_ok(const btx_rsnn_plan* p, const float* x) {
     bool ok = (p->n % 4 == 0) && aligned16(p->v) && (!p->psc || aligned16(p->psc)) &&
               (!p->use_refractory || aligned16(p->refractory)) && (!x || aligned16(x)) &&
               (p->tau == nullptr || p->tau_stride == 0 || aligned16(p->tau));
-    if (p->neuron_model == BTX_NEURON_TWO_COMPARTMENT)
+    if (p->neuron_model == BTX_NEURON_TWO_COMPARTMENT ||
+        (p->neuron_model == BTX_NEURON_MIXED &&
+         (p->mixed_models & (1 << BTX_NEURON_TWO_COMPARTMENT))))
         ok = ok && aligned16(p->i_a) && aligned16(p->i_bap) && aligned16(p->theta_th) &&
              (!p->x_apical || aligned16(p->x_apical)) &&
              (p->x_apical_step_stride % 4 == 0);
@@ -1357,13 +1413,14 @@ BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float
     a.x = x;
     a.rec_z = z;
     a.rec_v_pre = v_pre;
-    a.xa = a.kind == kTwoComp ? p->x_apical : nullptr;
+    a.xa = (a.kind == kTwoComp || a.kind == kMixed) ? p->x_apical : nullptr;
     a.vec4 = vec4_ok(p, x);
     a.rec_any = set_rec_any(a);
     const dim3 grid((unsigned)a.nchunks, (unsigned)p->lanes);
     const hipStream_t s = (hipStream_t)stream;
     if (a.kind == kGlif3) launch_neuron_only<kGlif3>(a, grid, s);
     else if (a.kind == kTwoComp) launch_neuron_only<kTwoComp>(a, grid, s);
+    else if (a.kind == kMixed) launch_neuron_only<kMixed>(a, grid, s);
     else launch_neuron_only<kLif>(a, grid, s);
     return check_launch("btx_lif_step");
 }
@@ -1490,6 +1547,7 @@ static void launch_step_k(const StepArgs& a, const PushStep& ps, int np, int gro
     }
     if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3>(a, ps, np, group, grid, s);
     else if (a.kind == kTwoComp) launch_step_g<ORDERED, kTwoComp>(a, ps, np, group, grid, s);
+    else if (a.kind == kMixed) launch_step_g<ORDERED, kMixed>(a, ps, np, group, grid, s);
     else launch_step_g<ORDERED, kLif>(a, ps, np, group, grid, s);
 }
 
@@ -1687,7 +1745,8 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
             a.red.pull.slot0 = (int)mod(a.red.pull.clock, R);
         }
         a.x = p->x ? p->x + t * p->x_step_stride : nullptr;
-        a.xa = (a.kind == kTwoComp && p->x_apical) ? p->x_apical + t * p->x_apical_step_stride
+        a.xa = ((a.kind == kTwoComp || a.kind == kMixed) && p->x_apical)
+                   ? p->x_apical + t * p->x_apical_step_stride
                                                    : nullptr;
         a.t = t;
         a.rec_sample = p->rec_sample ? p->rec_sample + t * lanes * p->n_sample : nullptr;
@@ -1702,7 +1761,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         a.rec_ref = p->rec_refractory ? p->rec_refractory + t * rec_step : nullptr;
         a.rec_v_pre = p->rec_v_pre ? p->rec_v_pre + t * rec_step : nullptr;
         a.rec_v_prev = p->rec_v_prev ? p->rec_v_prev + t * rec_step : nullptr;
-        if (a.kind == kTwoComp) {
+        if (a.kind == kTwoComp || a.kind == kMixed) {
             a.rec_ia = p->rec_i_a ? p->rec_i_a + t * rec_step : nullptr;
             a.rec_ibap = p->rec_i_bap ? p->rec_i_bap + t * rec_step : nullptr;
             a.rec_th = p->rec_theta_th ? p->rec_theta_th + t * rec_step : nullptr;

@@ -65,21 +65,108 @@ def _bin_nonempty(conn: BaseSparseConn, csr_d: int) -> list[bool]:
     return [bool(c) for c in counts.tolist()]
 
 
+class MixedState:
+    """A multi-group MixedNeuronPopulation as one neuron of the fused step:
+    population-wide state arrays (v, refractory, the two-compartment states,
+    after-spike currents) that the engine gathers from the groups' memories
+    before a run and writes back into them after it, and the groups' layout
+    and parameters for the plan (``_plan.mixed_plan``)."""
+
+    def __init__(self, pop: MixedNeuronPopulation):
+        self.pop = pop
+        self.size = pop.size
+        self.n_neuron = (pop.size,)
+        groups = pop.groups()
+        self.has_tc = any(isinstance(g, TwoCompartmentGLIF) for g in groups)
+        glifs = [g for g in groups if isinstance(g, GLIF3)]
+        self.n_asc = glifs[0].n_Iasc if glifs else 0
+        self._use_refractory = any(getattr(g, "_use_refractory", False) for g in groups
+                                   if not isinstance(g, TwoCompartmentGLIF))
+        self.v = self.refractory = self.i_a = self.i_bap = self.theta_th = self.Iasc = None
+        self._lanes = None
+
+    def modules(self) -> list:
+        return self.pop.groups()
+
+    def _buffers_for(self, lanes, dev):
+        n = self.size
+        if self._lanes != (lanes, str(dev)):
+            z = lambda *shape: torch.zeros(shape, dtype=torch.float32, device=dev)  # noqa: E731
+            self.v = z(lanes, n)
+            self.refractory = z(lanes, n) if self._use_refractory else None
+            if self.has_tc:
+                self.i_a, self.i_bap, self.theta_th = z(lanes, n), z(lanes, n), z(lanes, n)
+            self.Iasc = z(lanes, n, self.n_asc) if self.n_asc else None
+            self._lanes = (lanes, str(dev))
+
+    def _members(self):
+        """(population array, group, memory name, trailing dims) to move."""
+        out = []
+        lay = self.pop.layout
+        for g, mod in enumerate(self.pop.groups()):
+            sl = lay.span(g)
+            out.append((self.v, sl, mod, "v"))
+            if isinstance(mod, TwoCompartmentGLIF):
+                out += [(self.i_a, sl, mod, "i_a"), (self.i_bap, sl, mod, "i_bap"),
+                        (self.theta_th, sl, mod, "theta_th")]
+            elif getattr(mod, "_use_refractory", False):
+                out.append((self.refractory, sl, mod, "refractory"))
+            if isinstance(mod, GLIF3):
+                out.append((self.Iasc, sl, mod, "Iasc"))
+        return out
+
+    def gather(self, lanes: int) -> None:
+        """Population arrays from the groups' memories (v, refractory, ...)."""
+        dev = self.pop.groups()[0].v.device
+        self._buffers_for(lanes, dev)
+        for arr, sl, mod, name in self._members():
+            t = getattr(mod, name)
+            c = sl.stop - sl.start
+            want = (lanes, c) + tuple(arr.shape[2:])
+            if t is None or t.numel() != lanes * c * (arr[0, 0].numel()):
+                raise RuntimeError(f"{type(mod).__name__}.{name} is not initialised for "
+                                   f"{lanes} lanes x {c} neurons (call init_net_state)")
+            arr[:, sl] = t.reshape(want)
+
+    def scatter(self) -> None:
+        """The groups' memories from the population arrays (in place when a
+        memory is already contiguous of the right shape)."""
+        for arr, sl, mod, name in self._members():
+            t = getattr(mod, name)
+            src = arr[:, sl]
+            if t.is_contiguous() and t.numel() == src.numel():
+                t.view(src.shape).copy_(src)
+            else:
+                setattr(mod, name, src.contiguous().reshape(t.shape))
+
+
 def match_cell(rnn) -> CellSpec | None:
     """CellSpec when ``rnn`` is a cell the engine runs, else None."""
     neuron, syn = getattr(rnn, "neuron", None), getattr(rnn, "synapse", None)
     neuron_key = "neuron."
+    mixed = None
     if isinstance(neuron, MixedNeuronPopulation):
-        # one group spans the whole neuron axis: the population is that neuron
-        if len(neuron.counts) != 1:
-            return None
-        neuron_key = f"neuron.{neuron._group_names[0]}."
-        neuron = neuron.groups()[0]
+        if len(neuron.counts) == 1:
+            # one group spans the whole neuron axis: the population is that neuron
+            neuron_key = f"neuron.{neuron._group_names[0]}."
+            neuron = neuron.groups()[0]
+        else:
+            mixed = neuron
     elif isinstance(neuron, TwoCompartmentGLIF):
         return None  # its (spike, v) output is not a synapse input
     if getattr(rnn, "synapse_apical", None) is not None:
         return None
-    if not isinstance(neuron, (LIF, GLIF3, TwoCompartmentGLIF)):
+    if mixed is not None:
+        groups = mixed.groups()
+        if not all(isinstance(g, (LIF, GLIF3, TwoCompartmentGLIF)) for g in groups):
+            return None
+        nasc = {g.n_Iasc for g in groups if isinstance(g, GLIF3)}
+        if len(nasc) > 1 or (nasc and not 1 <= nasc.pop() <= 4):
+            return None
+        if any(tuple(g.n_neuron) != (g.size,) for g in groups):
+            return None
+        neuron = MixedState(mixed)
+    elif not isinstance(neuron, (LIF, GLIF3, TwoCompartmentGLIF)):
         return None
     if isinstance(neuron, GLIF3) and not 1 <= neuron.n_Iasc <= 4:
         return None
@@ -125,7 +212,7 @@ def match_cell(rnn) -> CellSpec | None:
         return None
     kin = spec.exp_psc
     if not isinstance(kin, ExponentialPSC):
-        if isinstance(neuron, TwoCompartmentGLIF):
+        if isinstance(neuron, (TwoCompartmentGLIF, MixedState)):
             return None
         prefix = spec.psc_keys[-1][: -len("psc")]
         spec.syn_keys = {prefix + k: f"syn_h{j + 1}" for j, k in enumerate(kin._STATES)}
@@ -210,7 +297,8 @@ class Engine:
                  int(cfg.ordered), int(cfg.slots), int(getattr(cfg, "bin_cap", 0)),
                  id(conn._csr_row_ptr), id(conn._csc_ptr), conn.weight_key(),
                  id(conn._prepared_sparse_weight)]
-        for m in (sp.neuron, sp.exp_psc):
+        mods = (sp.neuron.modules() if isinstance(sp.neuron, MixedState) else [sp.neuron])
+        for m in (*mods, sp.exp_psc):
             mem = getattr(m, "_memories_rv", {})
             for name, t in m._parameters.items():
                 parts.append((name, _tkey(t)))
@@ -233,7 +321,10 @@ class Engine:
             return hit[1], hit[2]
         keep: list = []
         cfg = conn.sparse_config
-        p = _plan.neuron_plan(neuron, lanes, dev, keep)
+        if isinstance(neuron, MixedState):
+            p = _plan.mixed_plan(neuron, lanes, dev, keep)
+        else:
+            p = _plan.neuron_plan(neuron, lanes, dev, keep)
         p.ordered = int(cfg.ordered)
         p.slots = int(cfg.slots)
         p.bin_cap = int(getattr(cfg, "bin_cap", 0))
@@ -274,7 +365,8 @@ class Engine:
         p = _lib.RsnnPlan()
         ctypes.memmove(ctypes.addressof(p), ctypes.addressof(tmpl), ctypes.sizeof(p))
         states = [(neuron, "v"), (syn, "psc")] + [(syn, k) for k in getattr(syn, "_STATES", ())]
-        if isinstance(neuron, TwoCompartmentGLIF):
+        tc_state = isinstance(neuron, TwoCompartmentGLIF) or getattr(neuron, "has_tc", False)
+        if tc_state:
             states += [(neuron, k) for k in ("i_a", "i_bap", "theta_th")]
         for mod, name in states:
             t = getattr(mod, name)
@@ -290,11 +382,14 @@ class Engine:
             t = getattr(syn, k)
             setattr(p, f"syn_h{j + 1}", t.data_ptr())
             keep.append(t)
-        if isinstance(neuron, TwoCompartmentGLIF):
+        if tc_state:
             for k in ("i_a", "i_bap", "theta_th"):
                 t = getattr(neuron, k)
                 setattr(p, k, t.data_ptr())
                 keep.append(t)
+        if isinstance(neuron, MixedState) and neuron.Iasc is not None:
+            p.iasc = neuron.Iasc.data_ptr()
+            keep.append(neuron.Iasc)
         if isinstance(neuron, GLIF3):
             m = neuron.n_Iasc
             ia = neuron.Iasc
@@ -305,7 +400,11 @@ class Engine:
                 neuron.Iasc = ia = ia.contiguous()
             p.iasc = ia.data_ptr()
             keep.append(ia)
-        if getattr(neuron, "_use_refractory", False):
+        if isinstance(neuron, MixedState):
+            if neuron._use_refractory:
+                p.refractory = neuron.refractory.data_ptr()
+                keep.append(neuron.refractory)
+        elif getattr(neuron, "_use_refractory", False):
             ref = _plan.lane_state(neuron, "refractory", tuple(neuron.v.shape))
             p.refractory = ref.data_ptr()
             keep.append(ref)
@@ -451,6 +550,9 @@ class Engine:
         f32 membrane of neurons 0..S-1 each step."""
         sp = self.spec
         keep: list = []
+        mixed = isinstance(sp.neuron, MixedState)
+        if mixed:
+            sp.neuron.gather(lanes)
         p, n, dev = self._base_plan(lanes, keep)
         if x is not None:
             if x.shape != (T, lanes, n) or not x.is_contiguous() or x.dtype != torch.float32:
@@ -458,7 +560,8 @@ class Engine:
             p.x = x.data_ptr()
             p.x_step_stride = lanes * n
         if x_apical is not None:
-            if p.neuron_model != _lib.NEURON_TWO_COMPARTMENT:
+            if p.neuron_model != _lib.NEURON_TWO_COMPARTMENT and not (
+                    mixed and sp.neuron.has_tc):
                 raise ValueError("an apical input needs a two-compartment neuron")
             if x_apical.shape != (T, lanes, n) or not x_apical.is_contiguous() \
                     or x_apical.dtype != torch.float32:
@@ -487,7 +590,8 @@ class Engine:
                 recs[key] = torch.empty((T, lanes, n), dtype=torch.float32, device=dev)
         for key in ("i_a", "i_bap", "theta_th"):
             if key in recs:
-                if p.neuron_model != _lib.NEURON_TWO_COMPARTMENT:
+                if p.neuron_model != _lib.NEURON_TWO_COMPARTMENT and not (
+                        mixed and sp.neuron.has_tc):
                     raise ValueError(f"'{key}' is a two-compartment state")
                 setattr(p, "rec_" + key, recs[key].data_ptr())
         p.rec_z = recs["z"].data_ptr() if "z" in recs else None
@@ -520,6 +624,8 @@ class Engine:
                 timings[name] = (float(ms[k]), int(calls[k]))
                 timings[name + "_each"] = [per[k * cap + i] for i in range(min(calls[k], cap))]
         self._tables_ok.add(tables)
+        if mixed:
+            sp.neuron.scatter()
         ring.cursor = (ring.cursor + T) % ring.R
         self._defer_history(ring, lanes, n, T)
         return raster, recs
@@ -561,6 +667,9 @@ class ResidentRun:
 
     def __init__(self, engine: Engine, lanes: int, T: int, drive, counts, sample):
         sp = engine.spec
+        if isinstance(sp.neuron, MixedState):
+            raise NotImplementedError("a resident run holds single-model state; a multi-group "
+                                      "MixedNeuronPopulation runs through Engine.run")
         if T % (sp.ring_d + 1):
             raise ValueError(f"T ({T}) must be a multiple of the device ring's "
                              f"{sp.ring_d + 1} slots")

@@ -23,6 +23,14 @@ def param_vec(t, n: int, shape: tuple[int, ...], device) -> tuple[torch.Tensor, 
     return torch.broadcast_to(t, shape).reshape(n).contiguous(), 1
 
 
+def _to_device(t: torch.Tensor, device) -> torch.Tensor:
+    """A host tensor on ``device`` (through pinned memory for a HIP device:
+    an asynchronous copy, no wait on the stream)."""
+    if torch.device(device).type == "cuda":
+        return t.pin_memory().to(device, non_blocking=True)
+    return t.to(device)
+
+
 def lane_state(module, name: str, shape: tuple[int, ...]) -> torch.Tensor:
     """The memory ``name`` of ``module`` as a contiguous fp32 tensor of
     exactly ``shape`` (the lanes x neurons a launch covers), stored back on
@@ -185,3 +193,96 @@ def neuron_plan(neuron, lanes: int, device, keep: list) -> _lib.RsnnPlan:
     if hasattr(neuron, "i_bap"):
         return tc_plan(neuron, lanes, device, keep)
     return lif_plan(neuron, lanes, device, keep)
+
+
+def mixed_plan(state, lanes: int, device, keep: list) -> _lib.RsnnPlan:
+    """Neuron fields of a plan for a multi-group MixedNeuronPopulation (the
+    engine's ``MixedState``): every parameter laid out per neuron along the
+    population axis (a group's scalar repeated over its range, values the
+    group's model does not use left at a neutral 0 or 1), a model code per
+    neuron, and the population-wide state arrays.  Exponential-Euler and
+    two-compartment coefficients come from torch on the CPU, with the ops of
+    the single-model plans above."""
+    from .neurons.glif import GLIF3
+    from .neurons.two_compartment import TwoCompartmentGLIF
+
+    pop = state.pop
+    lay = pop.layout
+    n = pop.size
+    dt = float(environ.get("dt"))
+    m = max(1, state.n_asc)
+    vec = {k: torch.zeros(n, dtype=torch.float32) for k in
+           ("tau", "v_th", "v_reset", "c_m", "tau_ref", "v_rest", "k_v")}
+    vec["tau"].fill_(1.0)
+    vec["c_m"].fill_(1.0)
+    asc = {k: torch.zeros(n, m, dtype=torch.float32) for k in ("asc_k", "asc_kk", "asc_amp")}
+    tcv = [torch.zeros(n, dtype=torch.float32) for _ in range(_lib.TC_NPARAM)]
+    kinds = torch.zeros(n, dtype=torch.uint8)
+    models = 0
+
+    def put(dst, sl, val):
+        c = sl.stop - sl.start
+        dst[sl] = torch.as_tensor(val).detach().to("cpu", torch.float32).reshape(-1).expand(c)
+
+    for g, mod in enumerate(pop.groups()):
+        sl = lay.span(g)
+        c = sl.stop - sl.start
+        put(vec["v_th"], sl, mod.v_threshold)
+        put(vec["v_reset"], sl, mod.v_reset)
+        if isinstance(mod, TwoCompartmentGLIF):
+            code = _lib.NEURON_TWO_COMPARTMENT
+            for k, t in enumerate(tc_coefficients(mod, dt)):
+                put(tcv[k], sl, t)
+        else:
+            code = _lib.NEURON_GLIF3 if isinstance(mod, GLIF3) else _lib.NEURON_LIF
+            put(vec["tau"], sl, mod.tau)
+            put(vec["c_m"], sl, mod.c_m)
+            if getattr(mod, "_use_refractory", False):
+                put(vec["tau_ref"], sl, mod.tau_ref)
+            if mod.hard_reset:
+                code |= _lib.MIXED_HARD_RESET
+            if isinstance(mod, GLIF3):
+                put(vec["v_rest"], sl, mod.v_rest)
+                put(vec["k_v"], sl, exp_euler_coeff(mod.tau, dt))
+                k_cpu = _cpu32(mod.k)
+                lin = -k_cpu
+                kk = torch.expm1(dt * lin) / lin
+                for key, t in (("asc_k", k_cpu), ("asc_kk", kk), ("asc_amp", mod.asc_amps)):
+                    asc[key][sl] = torch.broadcast_to(_cpu32(t), (c, m))
+        kinds[sl] = code
+        models |= 1 << (code & 3)
+    p = _lib.RsnnPlan()
+    p.n = n
+    p.lanes = lanes
+    p.neuron_model = _lib.NEURON_MIXED
+    p.mixed_models = models
+    p.hard_reset = 0
+    p.use_refractory = int(bool(state._use_refractory))
+    p.dt = dt
+
+    def dev_vec(t):
+        d, stride = param_vec(t, n, (n,), device)
+        keep.append(d)
+        return d.data_ptr(), stride
+
+    for field in ("tau", "v_th", "v_reset", "c_m"):
+        ptr, stride = dev_vec(vec[field])
+        setattr(p, field, ptr)
+        setattr(p, field + "_stride", stride)
+    if p.use_refractory:
+        p.tau_ref, p.tau_ref_stride = dev_vec(vec["tau_ref"])
+    if models & (1 << _lib.NEURON_GLIF3):
+        p.n_asc = state.n_asc
+        p.v_rest, p.v_rest_stride = dev_vec(vec["v_rest"])
+        p.k_v, p.k_v_stride = dev_vec(vec["k_v"])
+        for key in ("asc_k", "asc_kk", "asc_amp"):
+            d = _to_device(asc[key][:, : state.n_asc].contiguous(), device)
+            keep.append(d)
+            setattr(p, key, d.data_ptr())
+    if models & (1 << _lib.NEURON_TWO_COMPARTMENT):
+        for k in range(_lib.TC_NPARAM):
+            p.tc[k], p.tc_stride[k] = dev_vec(tcv[k])
+    kd = _to_device(kinds, device)
+    keep.append(kd)
+    p.neuron_kinds = kd.data_ptr()
+    return p

@@ -158,6 +158,10 @@ class GraphRunner:
         """Replay (capturing first when needed) T = xs.shape[0] steps of
         ``eng`` over xs (T, lanes, n); returns (raster, records) cloned."""
         sp = eng.spec
+        from ._engine import MixedState
+        if isinstance(sp.neuron, MixedState):
+            raise NotImplementedError("cudagraph=True replays single-model populations; a "
+                                      "multi-group MixedNeuronPopulation runs eagerly")
         T, lanes = xs.shape[0], xs.shape[1]
         key = (tuple(xs.shape), xs.dtype, str(xs.device), frozenset(record),
                eng._static_key(lanes, xs.device))

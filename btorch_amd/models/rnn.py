@@ -21,7 +21,7 @@ import torch
 import torch.nn as nn
 
 from . import base
-from ._engine import Engine, match_cell
+from ._engine import Engine, MixedState, match_cell
 from .functional import filter_hidden_states, hidden_state_names, prepare_sparse_modules
 from .linear import BaseSparseConn
 
@@ -235,6 +235,13 @@ class RecurrentNN(RecurrentNNAbstract):
             return super()._multi_step_loop(*args, loop_args=loop_args, **kwargs)
         from ._adjoint import fused_adjoint_covers, needs_grad, run_with_grad
 
+        if not self._engine_records(eng):
+            # a requested state the engine does not record (e.g. GLIF3's
+            # after-spike currents): the per-step loop returns it
+            if self.cudagraph:
+                raise NotImplementedError("cudagraph=True: the fused engine does not record "
+                                          "every requested state")
+            return super()._multi_step_loop(*args, loop_args=loop_args, **kwargs)
         graph = self._graph(eng, x)
         if needs_grad(self, x):
             lanes = x[0].numel() // max(1, x.shape[-1]) if x.ndim > 1 else 1
@@ -287,6 +294,14 @@ class RecurrentNN(RecurrentNNAbstract):
 
     @classmethod
     def _record_of(cls, k: str, sp) -> str | None:
+        if isinstance(sp.neuron, MixedState):
+            # "neuron.<group>.<memory>": the population record, sliced to
+            # the group's range ("<record>@<group index>")
+            for g, name in enumerate(sp.neuron.pop.layout.names):
+                pre = f"neuron.{name}."
+                if k.startswith(pre):
+                    rec = cls._NEURON_RECORDS.get(k[len(pre):])
+                    return None if rec is None else f"{rec}@{g}"
         if k.startswith(sp.neuron_key):
             return cls._NEURON_RECORDS.get(k[len(sp.neuron_key):])
         if k in sp.psc_keys:
@@ -304,12 +319,22 @@ class RecurrentNN(RecurrentNNAbstract):
             rec = cls._record_of(k, sp)
             if rec is None:
                 raise NotImplementedError(f"recording '{k}' is not supported by the fused engine")
-            want.add(rec)
+            want.add(rec.partition("@")[0])
         return want
+
+    def _engine_records(self, eng) -> bool:
+        """Whether the fused engine records every requested state."""
+        sp = eng.spec
+        return all(self._record_of(k, sp) is not None for k in self._record_names())
 
     def _assemble(self, eng, names, recs, raster, hist0, cursor0, T, lanes, n):
         sp = eng.spec
-        shape = tuple(sp.neuron.v.shape)
+        mixed = isinstance(sp.neuron, MixedState)
+        if mixed:
+            lead = tuple(sp.neuron.pop.groups()[0].v.shape[:-1])
+            shape = lead + (n,)
+        else:
+            shape = tuple(sp.neuron.v.shape)
         z = recs["z"].reshape(T, *shape)
         states = {}
         for k in names:
@@ -317,6 +342,10 @@ class RecurrentNN(RecurrentNNAbstract):
             if rec == "history":
                 seq = eng.history_sequence(raster, hist0, T, lanes, n, cursor0)
                 states[k] = seq.reshape(T, *sp.history.history.shape)
+            elif "@" in rec:
+                base, _, g = rec.partition("@")
+                sl = sp.neuron.pop.layout.span(int(g))
+                states[k] = recs[base][..., sl].reshape(T, *shape[:-1], sl.stop - sl.start)
             else:
                 states[k] = recs[rec].reshape(T, *shape)
         if getattr(sp.neuron, "pre_spike_v", False) and "v_pre" in recs:
@@ -382,7 +411,7 @@ class ApicalRecurrentNN(RecurrentNN):
                    and x.device.type == "cuda" and x_syn is None
                    and (x_apical is None or (torch.is_tensor(x_apical)
                                              and x_apical.shape[0] == x.shape[0])))
-        if not fusable:
+        if not fusable or not self._engine_records(eng):
             if self.cudagraph:
                 raise NotImplementedError("cudagraph=True replays the fused engine; this call "
                                           "is not one it runs")

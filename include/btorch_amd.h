@@ -180,6 +180,14 @@ int btx_conn_build(const int64_t* pre, const int64_t* post, const int64_t* delay
 #define BTX_NEURON_LIF 0      /* btorch/models/neurons/lif.py:144-187 */
 #define BTX_NEURON_GLIF3 1    /* btorch/models/neurons/glif.py:264-342 */
 #define BTX_NEURON_TWO_COMPARTMENT 2  /* btorch/models/neurons/two_compartment.py:208-292 */
+#define BTX_NEURON_MIXED 3    /* MixedNeuronPopulation, btorch/models/neurons/mixed.py:16-193 */
+
+/* Per-neuron model code of a BTX_NEURON_MIXED plan (plan.neuron_kinds):
+ * the low bits name the neuron's model (BTX_NEURON_LIF, _GLIF3 or
+ * _TWO_COMPARTMENT), BTX_MIXED_HARD_RESET selects the hard reset of a LIF
+ * or GLIF3 neuron (the two-compartment neuron always resets hard). */
+#define BTX_MIXED_MODEL_MASK 3
+#define BTX_MIXED_HARD_RESET 4
 
 /* Synapse kinetics of the fused step (plan.syn_kind). */
 #define BTX_SYN_EXP 0          /* ExponentialPSC, btorch/models/synapse.py:138-187 */
@@ -386,6 +394,20 @@ typedef struct btx_rsnn_plan {
      * earlier run with the same workspace, nbins, bin_delay and bin_sub
      * wrote them), so the run skips writing them; 0 = write them first */
     int tables_ready;
+    /* BTX_NEURON_MIXED (MixedNeuronPopulation.single_step_forward,
+     * mixed.py:113-141: each group's own step on its range of the axis):
+     * neuron j runs the model neuron_kinds[j] names, with every parameter
+     * given per neuron (stride 1, or 0 for a value all groups share) and
+     * every state array over the whole axis; mixed_models is the set of
+     * models present (bit 1 << BTX_NEURON_*).  A plan with the
+     * two-compartment model gives i_a, i_bap, theta_th and every tc
+     * coefficient (any value for the other neurons); one with GLIF3 gives
+     * n_asc, iasc, v_rest, k_v and the after-spike parameters (zeros for
+     * the other neurons).  Neurons of a LIF or GLIF3 group without a
+     * refractory period run with tau_ref 0 and a zero counter, which leaves
+     * their update unchanged.  Exponential synapse only. */
+    const uint8_t* neuron_kinds;  /* (n,) */
+    int mixed_models;
 } btx_rsnn_plan;
 
 size_t btx_rsnn_workspace_bytes(int64_t lanes, int64_t n, int slots, int ordered);

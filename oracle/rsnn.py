@@ -286,6 +286,61 @@ def tc_step(v, i_a, i_bap, theta_th, i_soma, i_apical, p: TwoCompParams, dt: flo
     return spike, v, i_a_next, i_bap, theta_th, v_pre
 
 
+@dataclass
+class MixedParams:
+    """MixedNeuronPopulation (mixed.py:16-193): contiguous groups
+    (start, count, LIFParams | GLIF3Params | TwoCompParams) along the
+    neuron axis, in order."""
+
+    groups: list
+
+    def to(self, dtype):
+        out = []
+        for s0, c, g in self.groups:
+            if isinstance(g, TwoCompParams):
+                g = g.to(dtype)
+            elif isinstance(g, GLIF3Params):
+                g = GLIF3Params(*(None if t is None else t.to(dtype) for t in
+                                  (g.tau, g.v_threshold, g.v_reset, g.v_rest, g.c_m, g.tau_ref,
+                                   g.k, g.asc_amps)), g.hard_reset)
+            else:
+                g = LIFParams(*(None if t is None else t.to(dtype) for t in
+                                (g.tau, g.v_threshold, g.v_reset, g.c_m, g.tau_ref)),
+                              g.hard_reset)
+            out.append((s0, c, g))
+        return MixedParams(out)
+
+
+def mixed_step(st: dict, I, xa, p: MixedParams, dt: float):
+    """MixedNeuronPopulation.single_step_forward (mixed.py:113-141): each
+    group steps on its slice of the input (the two-compartment groups with
+    their slice of the apical input), the spikes are concatenated.  ``st``
+    holds v, refractory, i_a, i_bap, theta_th (B, N) and per-group iasc;
+    returns (spike, v_pre) and updates ``st``."""
+    z = torch.empty_like(I)
+    v_pre = torch.empty_like(I)
+    v, ref = st["v"].clone(), st["refractory"].clone()
+    ia, ib, th = st["i_a"].clone(), st["i_bap"].clone(), st["theta_th"].clone()
+    for gi, (s0, c, g) in enumerate(p.groups):
+        sl = slice(s0, s0 + c)
+        if isinstance(g, TwoCompParams):
+            zg, vg, iag, ibg, thg, vpg = tc_step(st["v"][:, sl], st["i_a"][:, sl],
+                                                 st["i_bap"][:, sl], st["theta_th"][:, sl],
+                                                 I[:, sl], xa[:, sl], g, dt)
+            ia[:, sl], ib[:, sl], th[:, sl] = iag, ibg, thg
+        elif isinstance(g, GLIF3Params):
+            zg, vg, iag, rg, vpg = glif3_step(st["v"][:, sl], st["iasc"][gi],
+                                              st["refractory"][:, sl], I[:, sl], g, dt)
+            st["iasc"][gi] = iag
+            ref[:, sl] = rg
+        else:
+            zg, vg, rg, vpg = lif_step(st["v"][:, sl], st["refractory"][:, sl], I[:, sl], g, dt)
+            ref[:, sl] = rg
+        z[:, sl], v[:, sl], v_pre[:, sl] = zg, vg, vpg
+    st.update(v=v, refractory=ref, i_a=ia, i_bap=ib, theta_th=th)
+    return z, v_pre
+
+
 def exp_psc_decay(psc, tau_syn, dt: float):
     """ExponentialPSC.conductance_charge (synapse.py:171-178) through
     exp_euler_step (ode.py:20-50): x + expm1(dt*A)/A * f(x)."""
@@ -442,7 +497,13 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
     T, B, n = x_seq.shape
     glif = isinstance(net.lif, GLIF3Params)
     tc = isinstance(net.lif, TwoCompParams)
-    if tc:
+    mixed = isinstance(net.lif, MixedParams)
+    if mixed:
+        p = net.lif.to(dtype)
+        p_ref = next((g for _, _, g in p.groups if not isinstance(g, TwoCompParams)), None)
+        v_reset0 = torch.cat([torch.as_tensor(g.v_reset, dtype=dtype).expand(c)
+                              for _, c, g in p.groups])
+    elif tc:
         p = net.lif.to(dtype)
     elif glif:
         g = net.lif
@@ -458,7 +519,7 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
     mm = net.conn if isinstance(net.conn, NativeSparse) else NativeSparse(net.conn, dtype)
     st = state or {}
     # memory v starts at v_reset (register_memory("v", v_reset, ...), base.py:1022)
-    v0 = torch.zeros(B, n, dtype=dtype) + p.v_reset
+    v0 = torch.zeros(B, n, dtype=dtype) + (v_reset0 if mixed else p.v_reset)
     v = st.get("v", v0).to(dtype).clone()
     ref = st.get("refractory", torch.zeros(B, n, dtype=dtype)).to(dtype).clone()
     psc = st.get("psc", torch.zeros(B, n, dtype=dtype)).to(dtype).clone()
@@ -469,9 +530,18 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
                for k in syn.states}
     iasc = (st.get("iasc", torch.zeros(B, n, n_asc, dtype=dtype)).to(dtype).clone()
             if glif else None)
-    if tc:
+    if tc or mixed:
         zero = torch.zeros(B, n, dtype=dtype)
         i_a, i_bap, th = (st.get(k, zero).to(dtype).clone() for k in ("i_a", "i_bap", "theta_th"))
+    if mixed:
+        mst = {"v": v, "refractory": ref, "i_a": i_a, "i_bap": i_bap, "theta_th": th,
+               "iasc": {}}
+        for gi, (s0, c, g) in enumerate(p.groups):
+            if isinstance(g, GLIF3Params):
+                m = int(g.k.shape[-1]) if g.k.ndim else 1
+                init = st.get("iasc_groups", {}).get(gi)
+                mst["iasc"][gi] = (init.to(dtype).clone() if init is not None
+                                   else torch.zeros(B, c, m, dtype=dtype))
     ring = None
     if net.delay_mode == "hetero" and net.max_delay_steps > 1:
         ring = Ring(B, n, net.max_delay_steps, net.circular, dtype, st.get("history"),
@@ -483,7 +553,12 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
                            *(syn.states if syn is not None else ()))}
     x_seq = x_seq.to(dtype)
     for t in range(T):
-        if tc:
+        if mixed:
+            xa = x_apical[t].to(dtype) if x_apical is not None else torch.zeros_like(v)
+            z, v_pre = mixed_step(mst, psc + x_seq[t], xa, p, net.dt)
+            v, ref, i_a, i_bap, th = (mst[k] for k in ("v", "refractory", "i_a", "i_bap",
+                                                       "theta_th"))
+        elif tc:
             xa = x_apical[t].to(dtype) if x_apical is not None else torch.zeros_like(v)
             z, v, i_a, i_bap, th, v_pre = tc_step(v, i_a, i_bap, th, psc + x_seq[t], xa, p,
                                                   net.dt)
@@ -514,7 +589,7 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
                 rec[k].append(sst[k])
         for k, val in (("z", z), ("v", v), ("refractory", ref), ("psc", psc), ("v_pre", v_pre)):
             rec[k].append(val)
-        if tc:
+        if tc or mixed:
             rec["i_a"].append(i_a)
             rec["theta_th"].append(th)
     out = {k: torch.stack(vals) for k, vals in rec.items() if vals}
@@ -523,8 +598,10 @@ def simulate(net: Net, x_seq: torch.Tensor, state=None, dtype=torch.float32, tea
         out["final"].update({k: sst[k] for k in syn.states})
     if glif:
         out["final"]["iasc"] = iasc
-    if tc:
+    if tc or mixed:
         out["final"].update(i_a=i_a, i_bap=i_bap, theta_th=th)
+    if mixed:
+        out["final"]["iasc_groups"] = mst["iasc"]
     if ring is not None:
         out["final"]["history"] = ring.h
         out["final"]["cursor"] = ring.cursor

@@ -301,3 +301,62 @@ def test_save_grad_history_records_state_gradients():
     assert float((hist[0] - R[0]).abs().max()) > 0
     model.clear_grad_history()
     assert model.get_grad_history() == {}
+
+
+def test_mixed_population_glif3_two_compartment_lif_in_fused_step():
+    # a three-group MixedNeuronPopulation (GLIF3, two-compartment, LIF with
+    # hard reset; group sizes not multiples of 4 or 64) in ApicalRecurrentNN
+    # with delayed synapses runs in btx_rsnn_run, bit for bit with the
+    # oracle's per-group loop
+    from btorch_amd.models import (GLIF3, ApicalRecurrentNN, MixedNeuronPopulation,
+                                   TwoCompartmentGLIF)
+    sizes = (333, 402, 265)
+    n, B, T, D = sum(sizes), 2, 120, 5
+    c = workloads.powerlaw_net_fast(n=n, k_min=5, k_max=60, delay_lo=1, delay_hi=D - 1, seed=31)
+    import scipy.sparse
+    cexp = scipy.sparse.coo_array(c.expanded(D) * 5.0).astype(np.float32)
+    g = torch.Generator().manual_seed(3)
+    gl_kw = dict(v_threshold=1.0, v_reset=0.0, c_m=1.0, tau=torch.rand(sizes[0], generator=g) * 20
+                 + 10, tau_ref=2.0, k=(0.2, 0.05), asc_amps=(-0.3, 0.1))
+    tc_kw = dict(v_threshold=-50.0, v_reset=-65.0, E_L=-70.0, delta_th=2.0, tau_a=50.0,
+                 tau_s=torch.rand(sizes[1], generator=g) * 20 + 10, tau_th=200.0)
+    lif_kw = dict(tau=15.0, tau_ref=None, hard_reset=True)
+    pop = MixedNeuronPopulation({"glif": (sizes[0], GLIF3(sizes[0], **gl_kw)),
+                                 "tc": (sizes[1], TwoCompartmentGLIF(sizes[1], **tc_kw)),
+                                 "lif": (sizes[2], LIF(sizes[2], **lif_kw))})
+    syn = HeterSynapsePSC(n, 1, None, SparseConn(cexp), base_psc=ExponentialPSC,
+                          max_delay_steps=D, use_circular_buffer=True, tau_syn=5.0)
+    names = ["neuron.glif.v", "neuron.tc.v", "neuron.tc.i_a", "neuron.lif.v", "synapse.psc"]
+    model = ApicalRecurrentNN(pop, syn, step_mode="m", update_state_names=names).to(DEV)
+    functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
+    from btorch_amd.models._engine import MixedState
+    assert isinstance(model.fused_engine().spec.neuron, MixedState)
+    rng = np.random.default_rng(8)
+    x = torch.from_numpy((rng.random((T, B, n)) < 0.05).astype(np.float32) * 1.5)
+    x[..., sizes[0]:sizes[0] + sizes[1]] = x[..., sizes[0]:sizes[0] + sizes[1]] * 40.0 + 18.0
+    xa = torch.randn(T, B, n, generator=torch.Generator().manual_seed(2))
+    with environ.context(dt=1.0), torch.no_grad():
+        z, st = model(x.to(DEV), None, xa.to(DEV))
+    gp = rsnn.GLIF3Params.make(sizes[0], **gl_kw)
+    mixed = rsnn.MixedParams([(0, sizes[0], gp), (sizes[0], sizes[1], rsnn.TwoCompParams.make(
+        **tc_kw)), (sizes[0] + sizes[1], sizes[2], rsnn.LIFParams.make(sizes[2], **lif_kw))])
+    net = rsnn.Net(n, cexp, mixed, torch.tensor(5.0), 1.0, delay_mode="hetero",
+                   max_delay_steps=D, circular=True)
+    ref = rsnn.simulate(net, x, x_apical=xa)
+    for s0, cnt in ((0, sizes[0]), (sizes[0], sizes[1]), (sizes[0] + sizes[1], sizes[2])):
+        assert float(ref["z"][..., s0:s0 + cnt].sum()) > 20, s0
+    assert bits_equal(z, ref["z"])
+    spans = {"glif": slice(0, sizes[0]), "tc": slice(sizes[0], sizes[0] + sizes[1]),
+             "lif": slice(sizes[0] + sizes[1], n)}
+    for key in ("neuron.glif.v", "neuron.tc.v", "neuron.lif.v"):
+        grp = key.split(".")[1]
+        assert bits_equal(st[key], ref["v"][..., spans[grp]]), key
+    assert bits_equal(st["neuron.tc.i_a"], ref["i_a"][..., spans["tc"]])
+    assert bits_equal(st["synapse.psc"], ref["psc"])
+    fin = ref["final"]
+    assert bits_equal(pop.glif.v, fin["v"][:, spans["glif"]])
+    assert bits_equal(pop.glif.Iasc, fin["iasc_groups"][0])
+    assert bits_equal(pop.glif.refractory, fin["refractory"][:, spans["glif"]])
+    assert bits_equal(pop.tc.theta_th, fin["theta_th"][:, spans["tc"]])
+    assert bits_equal(pop.lif.v, fin["v"][:, spans["lif"]])
+    assert bits_equal(syn.history.history.reshape(B, D, n), fin["history"])

@@ -228,9 +228,13 @@ def test_engine_matches_two_compartment_cells():
     assert match_cell(RecurrentNN(MixedNeuronPopulation([(n, LIF(n))]), syn())) is not None
     # a bare two-compartment neuron returns (spike, v), which no synapse takes
     assert match_cell(RecurrentNN(TwoCompartmentGLIF(n), syn())) is None
-    # two groups, or an apical synapse, run the per-step loop
+    # two groups run in the fused step through the population's state
+    # arrays; an apical synapse runs the per-step loop
+    from btorch_amd.models._engine import MixedState
     two = MixedNeuronPopulation([(6, TwoCompartmentGLIF(6)), (6, LIF(6))])
-    assert match_cell(ApicalRecurrentNN(two, syn())) is None
+    spec2 = match_cell(ApicalRecurrentNN(two, syn()))
+    assert spec2 is not None and isinstance(spec2.neuron, MixedState)
+    assert spec2.neuron.has_tc and spec2.neuron.size == 12
     assert match_cell(ApicalRecurrentNN(one, syn(), synapse_apical=syn())) is None
     with pytest.raises(ValueError):
         MixedNeuronPopulation([])
@@ -445,3 +449,34 @@ def test_spike_history_deferred_writeback_semantics():
     h.defer_writeback(wb, cursor=1)
     h.reset()
     assert float(h.history.sum()) == 0 and len(calls) == 3
+
+
+def test_mixed_plan_lays_parameters_out_per_neuron():
+    import ctypes
+    from btorch_amd import _lib
+    from btorch_amd.models import GLIF3, MixedNeuronPopulation, TwoCompartmentGLIF
+    from btorch_amd.models._engine import MixedState
+    from btorch_amd.models._plan import mixed_plan
+    pop = MixedNeuronPopulation({"a": (5, LIF(5, tau=12.0, tau_ref=2.0, hard_reset=True)),
+                                 "b": (3, TwoCompartmentGLIF(3)),
+                                 "c": (4, GLIF3(4, tau=torch.tensor([5.0, 6.0, 7.0, 8.0]),
+                                                k=(0.1, 0.2), asc_amps=(1.0, -1.0)))})
+    st = MixedState(pop)
+    assert st.has_tc and st.n_asc == 2 and st._use_refractory
+    keep: list = []
+    with environ.context(dt=0.5):
+        p = mixed_plan(st, 2, "cpu", keep)
+
+    def read(ptr, count, ctype=ctypes.c_float):
+        return list((ctype * count).from_address(ptr))
+
+    assert p.neuron_model == _lib.NEURON_MIXED
+    assert p.mixed_models == 0b111 and p.use_refractory == 1 and p.n_asc == 2
+    kinds = read(p.neuron_kinds, 12, ctypes.c_uint8)
+    assert kinds == [_lib.NEURON_LIF | _lib.MIXED_HARD_RESET] * 5 + \
+        [_lib.NEURON_TWO_COMPARTMENT] * 3 + [_lib.NEURON_GLIF3] * 4
+    assert read(p.tau, 12) == [12.0] * 5 + [1.0] * 3 + [5.0, 6.0, 7.0, 8.0]
+    assert read(p.tau_ref, 12)[:5] == [2.0] * 5 and read(p.tau_ref, 12)[5:] == [0.0] * 7
+    assert p.tau_stride == 1 and p.tc_stride[0] == 1
+    asc_k = read(p.asc_k, 24)
+    assert asc_k[-8:] == pytest.approx([0.1, 0.2] * 4) and asc_k[:16] == [0.0] * 16

@@ -730,3 +730,64 @@ def test_step_composition_mistakes_fail_the_known_answers():
     assert abs(a * 2.0 - psc_ref[1]) > 0.1
     # off-by-one delay: the d = 2 trace would equal the d = 3 one
     assert KNOWN_PSC2[2] != 0.0
+
+
+# ---------------------------------------------------------------------------
+# mixed populations: a population of groups equals its groups run apart
+# when no synapse crosses groups, and a population of identical LIF groups
+# equals one LIF population
+# ---------------------------------------------------------------------------
+
+def _block_conn(sizes, p, seed):
+    rng = np.random.default_rng(seed)
+    n = sum(sizes)
+    dense = np.zeros((n, n), dtype=np.float32)
+    s0 = 0
+    for c in sizes:
+        blk = (rng.random((c, c)) < p) * rng.lognormal(-1, 0.5, (c, c))
+        dense[s0:s0 + c, s0:s0 + c] = blk * rng.choice([-1.0, 1.0], size=(c, c))
+        s0 += c
+    return scipy.sparse.coo_array(dense)
+
+
+def test_mixed_population_equals_its_groups_apart():
+    sizes = (37, 29, 18)
+    n, B, T = sum(sizes), 2, 60
+    conn = _block_conn(sizes, 0.2, 3)
+    glif = rsnn.GLIF3Params.make(sizes[0], tau=torch.linspace(10, 30, sizes[0]), v_threshold=1.0,
+                                 v_reset=0.0, c_m=1.0, tau_ref=2.0, k=(0.2, 0.05),
+                                 asc_amps=(-0.3, 0.1))
+    tc = rsnn.TwoCompParams.make(v_threshold=1.0, v_reset=0.0, tau_s=15.0, E_L=0.0, tau_a=20.0,
+                                 tau_th=50.0, delta_th=0.2)
+    lif = rsnn.LIFParams.make(sizes[2], tau=12.0, tau_ref=None, hard_reset=True)
+    mixed = rsnn.MixedParams([(0, sizes[0], glif), (sizes[0], sizes[1], tc),
+                              (sizes[0] + sizes[1], sizes[2], lif)])
+    rng = np.random.default_rng(4)
+    x = torch.from_numpy((rng.random((T, B, n)) < 0.1).astype(np.float32) * 2.5)
+    xa = torch.from_numpy(rng.uniform(0, 0.5, (T, B, n)).astype(np.float32))
+    out = rsnn.simulate(rsnn.Net(n, conn, mixed, torch.tensor(5.0), 0.5), x, x_apical=xa)
+    assert out["z"].sum() > 50
+    s0 = 0
+    for (c, prm) in zip(sizes, (glif, tc, lif)):
+        sl = slice(s0, s0 + c)
+        sub = conn.tocsr()[s0:s0 + c, s0:s0 + c].tocoo()
+        ref = rsnn.simulate(rsnn.Net(c, sub, prm, torch.tensor(5.0), 0.5), x[:, :, sl],
+                            x_apical=xa[:, :, sl] if prm is tc else None)
+        for k in ("z", "v", "psc"):
+            assert torch.equal(out[k][:, :, sl], ref[k]), (k, s0)
+        s0 += c
+
+
+def test_mixed_population_of_identical_lif_groups_equals_one_population():
+    n, B, T = 90, 3, 80
+    conn = _rand_conn(n, n, 0.1, 5)
+    prm = rsnn.LIFParams.make(n, tau=20.0, tau_ref=2.0)
+    mixed = rsnn.MixedParams([(0, 40, rsnn.LIFParams.make(40, tau=20.0, tau_ref=2.0)),
+                              (40, 50, rsnn.LIFParams.make(50, tau=20.0, tau_ref=2.0))])
+    rng = np.random.default_rng(6)
+    x = torch.from_numpy((rng.random((T, B, n)) < 0.1).astype(np.float32) * 15.0)
+    a = rsnn.simulate(rsnn.Net(n, conn, mixed, torch.tensor(5.0), 0.1), x)
+    b = rsnn.simulate(rsnn.Net(n, conn, prm, torch.tensor(5.0), 0.1), x)
+    assert a["z"].sum() > 50
+    for k in ("z", "v", "refractory", "psc"):
+        assert torch.equal(a[k], b[k]), k
