@@ -617,7 +617,10 @@ def serve_bench(args, model, dev):
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t_build
 
-    mix = Server(shared, private).serve(reqs, realtime=True)
+    # admission by HBM budget: every shared lane plus a quarter of that for
+    # the private nets' pools (bytes of resident state and delay ring)
+    budget = int(1.25 * args.serve_lanes * shared.bytes_per_lane())
+    mix = Server(shared, private, hbm_budget=budget).serve(reqs, realtime=True)
     plain = run_plain(lambda r: plain_pools[(r.net, r.batch)], reqs)
     lane_steps = sum(r.T * r.batch for r in reqs)
 
@@ -631,6 +634,7 @@ def serve_bench(args, model, dev):
             "trial_steps": f"U(200, {args.serve_tmax})", "batch": "U{1..32}",
             "private_nets": "20% of requests on ER p=0.02 nets of 4096/16384/32768 neurons",
             "shared_pool_lanes": args.serve_lanes, "lane_steps": lane_steps,
+            "hbm_budget_bytes": budget,
             "continuous": m, "plain": pl,
             "throughput_vs_plain": m["lane_steps_per_s"] / pl["lane_steps_per_s"],
             "build_s": round(t_build, 2)}

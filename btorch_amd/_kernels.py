@@ -149,10 +149,12 @@ def compact_dense(x2d: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
         raise TypeError("compaction expects float32 input")
     x2d = x2d.contiguous()
     lanes, n = x2d.shape
+    lib = _lib.load()
     idx = torch.empty((lanes, max(n, 1)), dtype=torch.int32, device=x2d.device)
     cnt = torch.empty((lanes,), dtype=torch.int32, device=x2d.device)
-    _lib.check(_lib.load().btx_compact_dense(x2d.data_ptr(), lanes, n, n, idx.data_ptr(),
-                                             idx.shape[1], cnt.data_ptr(), _lib.stream_ptr()),
+    tiles = torch.empty((lanes, lib.btx_compact_tiles(n)), dtype=torch.int32, device=x2d.device)
+    _lib.check(lib.btx_compact_dense(x2d.data_ptr(), lanes, n, n, idx.data_ptr(), idx.shape[1],
+                                     cnt.data_ptr(), tiles.data_ptr(), _lib.stream_ptr()),
                "btx_compact_dense")
     return idx, cnt
 

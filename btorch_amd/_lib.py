@@ -169,7 +169,8 @@ class AdjLifArgs(ctypes.Structure):
 _SIGNATURES = {
     "btx_version": (_c_int, []),
     "btx_last_error": (ctypes.c_char_p, []),
-    "btx_compact_dense": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "btx_compact_tiles": (_c_i64, [_c_i64]),
+    "btx_compact_dense": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp, _vp]),
     "btx_compact_bits": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
     "btx_spmv_workspace_bytes": (ctypes.c_size_t, [_c_i64, _c_i64, _c_int, _c_int]),
     "btx_word_popcount": (_c_int, [_vp, _c_i64, _c_i64, _vp, _vp]),

@@ -25,7 +25,9 @@
 // validates the ranges).
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/functional.hpp>
 
 #include "btx_common.h"
 
@@ -52,11 +54,11 @@ BuildWs build_ws(int64_t nnz) {
     BuildWs L{};
     const size_t m = (size_t)(nnz > 0 ? nnz : 1);
     size_t sort_bytes = 0, scan_bytes = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint64_t*)nullptr,
-                                       (uint64_t*)nullptr, (const uint32_t*)nullptr,
-                                       (uint32_t*)nullptr, (int)m);
-    hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, (const uint32_t*)nullptr,
-                                     (uint32_t*)nullptr, (int)m);
+    (void)rocprim::radix_sort_pairs(nullptr, sort_bytes, (const uint64_t*)nullptr,
+                                    (uint64_t*)nullptr, (const uint32_t*)nullptr,
+                                    (uint32_t*)nullptr, m, 0u, 64u);
+    (void)rocprim::inclusive_scan(nullptr, scan_bytes, (const uint32_t*)nullptr,
+                                  (uint32_t*)nullptr, m, rocprim::plus<uint32_t>());
     L.temp_bytes = sort_bytes > scan_bytes ? sort_bytes : scan_bytes;
     size_t off = 0;
     L.keys_a = off; off = align256(off + m * 8);
@@ -303,20 +305,21 @@ BTX_API int btx_conn_build(const int64_t* pre, const int64_t* post, const int64_
                        (int64_t)n_delay_bins, ka, va);
     // coalesced order: (dst, virtual src), stable
     const int end1 = 32 + bit_length((uint64_t)(n_post > 0 ? n_post - 1 : 0));
-    if (hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, ka, kb, va, vb, n, 0, end1, s) !=
-        hipSuccess)
+    if (rocprim::radix_sort_pairs(temp, temp_bytes, ka, kb, va, vb, (size_t)n, 0u,
+                                  (unsigned)end1, s) != hipSuccess)
         return fail("btx_conn_build: radix sort failed");
     hipLaunchKernelGGL(k_segment_starts, dim3(g), dim3(256), 0, s, kb, nnz, flags);
     temp_bytes = L.temp_bytes;
-    if (hipcub::DeviceScan::InclusiveSum(temp, temp_bytes, flags, pos, n, s) != hipSuccess)
+    if (rocprim::inclusive_scan(temp, temp_bytes, flags, pos, (size_t)n,
+                                rocprim::plus<uint32_t>(), s) != hipSuccess)
         return fail("btx_conn_build: scan failed");
     hipLaunchKernelGGL(k_coalesce, dim3(g), dim3(256), 0, s, kb, vb, flags, pos, w, nnz, n_src,
                        out_dst, out_src, out_w, ka, va, nnz_out);
     // source-major order: (virtual src, dst); padding sorts last
     const int end2 = 32 + bit_length((uint64_t)n_src);
     temp_bytes = L.temp_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, ka, kb, va, vb, n, 0, end2, s) !=
-        hipSuccess)
+    if (rocprim::radix_sort_pairs(temp, temp_bytes, ka, kb, va, vb, (size_t)n, 0u,
+                                  (unsigned)end2, s) != hipSuccess)
         return fail("btx_conn_build: radix sort failed");
     hipLaunchKernelGGL(k_csr_cols, dim3(g), dim3(256), 0, s, kb, vb, nnz_out, nnz, col, perm,
                        out_src, csc_src);

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <string>
 
 #include "btx_common.h"
 #include "btx_spmv.h"
@@ -79,24 +80,73 @@ __global__ __launch_bounds__(1024) void k_compact_bits(const uint64_t* __restric
     if (threadIdx.x == 0) counts[lane] = total;
 }
 
-__global__ __launch_bounds__(1024) void k_compact_dense(const float* __restrict__ x, int64_t n,
-                                                        int64_t xstride,
-                                                        int32_t* __restrict__ idx,
-                                                        int64_t istride,
-                                                        int32_t* __restrict__ counts) {
-    const int lane = blockIdx.x;
-    const float* xl = x + (int64_t)lane * xstride;
-    const int64_t per = (n + blockDim.x - 1) / blockDim.x;
-    const int64_t e0 = (int64_t)threadIdx.x * per;
-    const int64_t e1 = e0 + per < n ? e0 + per : n;
+// Compaction of a dense fp32 matrix, in tiles of kCTile consecutive columns
+// of one lane: each wavefront owns kCTile / 4 columns and walks them 64 at a
+// time (one coalesced 256-byte load per step), turning the nonzero test into
+// a 64-bit ballot.  Pass 1 counts each tile; pass 2 adds the counts of the
+// lane's earlier tiles, the earlier wavefronts of its own tile and, within a
+// wavefront, the set lanes below (mbcnt), so the indices come out ascending.
+constexpr int kCTile = 4096;
+constexpr int kCWave = kCTile / 4;
+
+__device__ __forceinline__ int wave_nonzeros(const float* __restrict__ xl, int64_t base,
+                                             int64_t n) {
+    const int ln = threadIdx.x & 63;
     int c = 0;
-    for (int64_t e = e0; e < e1; ++e) c += (xl[e] != 0.f);
-    int total;
-    const int excl = block_excl_scan<16>(c, total);
-    int32_t* out = idx + (int64_t)lane * istride + excl;
-    for (int64_t e = e0; e < e1; ++e)
-        if (xl[e] != 0.f) *out++ = (int32_t)e;
-    if (threadIdx.x == 0) counts[lane] = total;
+#pragma unroll 4
+    for (int j = 0; j < kCWave / 64; ++j) {
+        const int64_t e = base + j * 64 + ln;
+        c += __popcll(__ballot(e < n && xl[e] != 0.f));
+    }
+    return c;
+}
+
+__global__ __launch_bounds__(256) void k_compact_count(const float* __restrict__ x, int64_t n,
+                                                       int64_t xstride, int64_t ntiles,
+                                                       int32_t* __restrict__ tile_counts) {
+    __shared__ int s_c[4];
+    const int64_t lane = blockIdx.y, tile = blockIdx.x;
+    const int w = threadIdx.x >> 6;
+    const int c = wave_nonzeros(x + lane * xstride, tile * kCTile + w * kCWave, n);
+    if ((threadIdx.x & 63) == 0) s_c[w] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_counts[lane * ntiles + tile] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+}
+
+__global__ __launch_bounds__(256) void k_compact_write(const float* __restrict__ x, int64_t n,
+                                                       int64_t xstride, int64_t ntiles,
+                                                       const int32_t* __restrict__ tile_counts,
+                                                       int32_t* __restrict__ idx,
+                                                       int64_t istride,
+                                                       int32_t* __restrict__ counts) {
+    __shared__ int s_c[4];
+    __shared__ int s_pre[4];
+    const int64_t lane = blockIdx.y, tile = blockIdx.x;
+    const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const float* xl = x + lane * xstride;
+    // nonzeros of the lane's earlier tiles
+    int pre = 0;
+    for (int64_t t = threadIdx.x; t < tile; t += 256) pre += tile_counts[lane * ntiles + t];
+    for (int o = 32; o > 0; o >>= 1) pre += __shfl_down(pre, o, 64);
+    const int64_t base = tile * kCTile + w * kCWave;
+    const int c = wave_nonzeros(xl, base, n);
+    if (ln == 0) {
+        s_pre[w] = pre;
+        s_c[w] = c;
+    }
+    __syncthreads();
+    int off = s_pre[0] + s_pre[1] + s_pre[2] + s_pre[3];
+    for (int k = 0; k < w; ++k) off += s_c[k];
+    int32_t* out = idx + lane * istride;
+    const uint64_t below = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
+    for (int j = 0; j < kCWave / 64; ++j) {
+        const int64_t e = base + j * 64 + ln;
+        const bool nz = e < n && xl[e] != 0.f;
+        const uint64_t b = __ballot(nz);
+        if (nz) out[off + __popcll(b & below)] = (int32_t)e;
+        off += __popcll(b);
+    }
+    if (tile == ntiles - 1 && threadIdx.x == 255) counts[lane] = off;
 }
 
 // ---------------------------------------------------------------------------
@@ -384,13 +434,24 @@ BTX_API int btx_unpack_bits(const uint64_t* words, int64_t rows, int64_t n, floa
 
 BTX_API const char* btx_last_error(void) { return g_last_error.c_str(); }
 
+BTX_API int64_t btx_compact_tiles(int64_t n) {
+    return n > 0 ? (n + kCTile - 1) / kCTile : 1;
+}
+
 BTX_API int btx_compact_dense(const float* x, int64_t lanes, int64_t n, int64_t x_lane_stride,
                               int32_t* idx, int64_t idx_lane_stride, int32_t* counts,
-                              void* stream) {
+                              int32_t* tile_counts, void* stream) {
     if (lanes <= 0) return 0;
     if (n >= (int64_t)1 << 31) return fail("btx_compact_dense: n exceeds int32 indices");
-    hipLaunchKernelGGL(k_compact_dense, dim3((unsigned)lanes), dim3(1024), 0,
-                       (hipStream_t)stream, x, n, x_lane_stride, idx, idx_lane_stride, counts);
+    if (lanes > 65535) return fail("btx_compact_dense: lanes exceeds 65535");
+    if (tile_counts == nullptr) return fail("btx_compact_dense: tile_counts is required");
+    const int64_t ntiles = btx_compact_tiles(n);
+    const dim3 grid((unsigned)ntiles, (unsigned)lanes);
+    const hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_compact_count, grid, dim3(256), 0, s, x, n, x_lane_stride, ntiles,
+                       tile_counts);
+    hipLaunchKernelGGL(k_compact_write, grid, dim3(256), 0, s, x, n, x_lane_stride, ntiles,
+                       tile_counts, idx, idx_lane_stride, counts);
     return check_launch("btx_compact_dense");
 }
 
@@ -410,22 +471,14 @@ BTX_API size_t btx_spmv_workspace_bytes(int64_t lanes, int64_t n_dst, int slots,
     return ws_layout(lanes, n_dst, slots, ordered).total;
 }
 
-BTX_API int btx_spmv_event(const int64_t* row_ptr, const int32_t* col, const float* w,
-                           int64_t n_src, int64_t n_dst, int64_t nnz, const int64_t* csc_ptr,
-                           const int32_t* csc_src, const float* csc_w, const int32_t* active,
-                           const int32_t* counts, int64_t lanes, int64_t active_lane_stride,
-                           const float* x, int64_t x_lane_stride, float* out, int ordered,
-                           int slots, void* workspace, size_t workspace_bytes, void* stream) {
-    hipStream_t s = (hipStream_t)stream;
-    if (lanes <= 0 || n_dst <= 0) return 0;
-    if (ordered && (slots < 2 || slots > kMaxSlots))
-        return fail("btx_spmv_event: ordered mode needs 2 <= slots <= 16");
-    if (ordered && (x == nullptr || csc_ptr == nullptr || csc_src == nullptr || csc_w == nullptr))
-        return fail("btx_spmv_event: ordered mode needs x and the dst-major arrays");
-    if (n_src >= (int64_t)1 << 31) return fail("btx_spmv_event: n_src exceeds int32");
-    const WsLayout L = ws_layout(lanes, n_dst, slots, ordered);
-    if (workspace_bytes < L.total) return fail("btx_spmv_event: workspace too small");
-    char* ws = (char*)workspace;
+// The event-driven product on a workspace whose term counts (ordered) or
+// accumulators (fast) are zero on entry; they are zero again on return.
+static int spmv_event_on(const int64_t* row_ptr, const int32_t* col, const float* w,
+                         int64_t n_src, int64_t n_dst, int64_t nnz, const int64_t* csc_ptr,
+                         const int32_t* csc_src, const float* csc_w, const int32_t* active,
+                         const int32_t* counts, int64_t lanes, int64_t active_lane_stride,
+                         const float* x, int64_t x_lane_stride, float* out, int ordered,
+                         int slots, char* ws, const WsLayout& L, hipStream_t s) {
     const int64_t m = lanes * n_dst;
     PushArgs a{};
     a.row_ptr = row_ptr;
@@ -464,6 +517,35 @@ BTX_API int btx_spmv_event(const int64_t* row_ptr, const int32_t* col, const flo
         hipLaunchKernelGGL(k_copy_zero, dim3(rblocks), dim3(256), 0, s, a.acc, out, m);
     }
     return check_launch("btx_spmv_event reduce");
+}
+
+static int spmv_event_check(int64_t n_src, const float* x, const int64_t* csc_ptr,
+                            const int32_t* csc_src, const float* csc_w, int ordered, int slots,
+                            const char* who) {
+    if (ordered && (slots < 2 || slots > kMaxSlots))
+        return fail(std::string(who) + ": ordered mode needs 2 <= slots <= 16");
+    if (ordered && (x == nullptr || csc_ptr == nullptr || csc_src == nullptr || csc_w == nullptr))
+        return fail(std::string(who) + ": ordered mode needs x and the dst-major arrays");
+    if (n_src >= (int64_t)1 << 31) return fail(std::string(who) + ": n_src exceeds int32");
+    return 0;
+}
+
+BTX_API int btx_spmv_event(const int64_t* row_ptr, const int32_t* col, const float* w,
+                           int64_t n_src, int64_t n_dst, int64_t nnz, const int64_t* csc_ptr,
+                           const int32_t* csc_src, const float* csc_w, const int32_t* active,
+                           const int32_t* counts, int64_t lanes, int64_t active_lane_stride,
+                           const float* x, int64_t x_lane_stride, float* out, int ordered,
+                           int slots, void* workspace, size_t workspace_bytes, void* stream) {
+    if (lanes <= 0 || n_dst <= 0) return 0;
+    if (int rc = spmv_event_check(n_src, x, csc_ptr, csc_src, csc_w, ordered, slots,
+                                  "btx_spmv_event"))
+        return rc;
+    const WsLayout L = ws_layout(lanes, n_dst, slots, ordered);
+    if (workspace == nullptr || workspace_bytes < L.total)
+        return fail("btx_spmv_event: workspace too small");
+    return spmv_event_on(row_ptr, col, w, n_src, n_dst, nnz, csc_ptr, csc_src, csc_w, active,
+                         counts, lanes, active_lane_stride, x, x_lane_stride, out, ordered,
+                         slots, (char*)workspace, L, (hipStream_t)stream);
 }
 
 BTX_API int btx_spmv_transpose(const int64_t* row_ptr, const int32_t* col, const float* w,

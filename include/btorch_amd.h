@@ -46,11 +46,16 @@ const char* btx_last_error(void);
 /* Ascending indices of the nonzero entries of each lane of a dense fp32
  * matrix x[lanes, n] (row stride x_lane_stride).  idx[lane*idx_lane_stride + k]
  * receives the k-th nonzero column, counts[lane] the number of them.
- * Replaces the "any source of this task spiked?" scan of the Triton backend
+ * tile_counts is scratch of lanes * btx_compact_tiles(n) int32 (any content
+ * on entry).  Each lane is cut into tiles of 4096 columns: one launch counts
+ * every tile with wavefront ballots over coalesced loads, a second writes
+ * each tile's indices at the offset of the earlier tiles.  Replaces the "any
+ * source of this task spiked?" scan of the Triton backend
  * (btorch/models/_sparse_triton.py:20-57) and equals numpy.nonzero per lane. */
+int64_t btx_compact_tiles(int64_t n);
 int btx_compact_dense(const float* x, int64_t lanes, int64_t n, int64_t x_lane_stride,
                       int32_t* idx, int64_t idx_lane_stride, int32_t* counts,
-                      void* stream);
+                      int32_t* tile_counts, void* stream);
 
 /* Same, from a spike bitmask words[lanes, ceil(n/64)] (bit j of word w is
  * column 64*w + j).  Used by the time-step engine on the raster it writes. */

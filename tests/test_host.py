@@ -26,7 +26,7 @@ ROOT = Path(__file__).resolve().parents[1]
 
 def test_library_exports_every_declared_symbol():
     header = (ROOT / "include" / "btorch_amd.h").read_text()
-    declared = set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(btx_\w+)\(", header, re.M))
+    declared = set(re.findall(r"^\s*(?:int|int64_t|size_t|const char\*)\s+(btx_\w+)\(", header, re.M))
     assert declared, "no declarations parsed"
     lib = _lib.load()
     for name in sorted(declared):
