@@ -185,6 +185,12 @@ _SIGNATURES = {
         [_vp, _vp, _vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i64,
          _vp, _c_i64, _vp, _c_int, _c_int, _vp, ctypes.c_size_t, _vp],
     ),
+    "btx_spmv_event_noalloc_max_bytes": (ctypes.c_size_t, []),
+    "btx_spmv_event_noalloc": (
+        _c_int,
+        [_vp, _vp, _vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i64,
+         _vp, _c_i64, _vp, _c_int, _c_int, _vp],
+    ),
     "btx_spmv_transpose": (_c_int, [_vp, _vp, _vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
     "btx_spmv_weight_grad": (
         _c_int,

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "btx_common.h"
@@ -546,6 +547,82 @@ BTX_API int btx_spmv_event(const int64_t* row_ptr, const int32_t* col, const flo
     return spmv_event_on(row_ptr, col, w, n_src, n_dst, nnz, csc_ptr, csc_src, csc_w, active,
                          counts, lanes, active_lane_stride, x, x_lane_stride, out, ordered,
                          slots, (char*)workspace, L, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// The product without a caller workspace: a file-scope device array serves
+// as its scratch.  Each GPU holds its own copy of the array, at its own
+// address, until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device.
+// ---------------------------------------------------------------------------
+
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The scratch array of the current device (looked up once per device).
+static int scratch_of_current_device(char** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(std::string("btx_spmv_event_noalloc: hipGetDevice: ") +
+                                     hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail("btx_spmv_event_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return fail(std::string("btx_spmv_event_noalloc: hipGetSymbolAddress: ") +
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = (char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+__global__ __launch_bounds__(256) void k_clear_words(uint32_t* __restrict__ p, int64_t words) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) p[i] = 0u;
+}
+
+BTX_API size_t btx_spmv_event_noalloc_max_bytes(void) { return kScratchBytes; }
+
+BTX_API int btx_spmv_event_noalloc(const int64_t* row_ptr, const int32_t* col, const float* w,
+                                   int64_t n_src, int64_t n_dst, int64_t nnz,
+                                   const int64_t* csc_ptr, const int32_t* csc_src,
+                                   const float* csc_w, const int32_t* active,
+                                   const int32_t* counts, int64_t lanes,
+                                   int64_t active_lane_stride, const float* x,
+                                   int64_t x_lane_stride, float* out, int ordered, int slots,
+                                   void* stream) {
+    if (lanes <= 0 || n_dst <= 0) return 0;
+    if (int rc = spmv_event_check(n_src, x, csc_ptr, csc_src, csc_w, ordered, slots,
+                                  "btx_spmv_event_noalloc"))
+        return rc;
+    const WsLayout L = ws_layout(lanes, n_dst, slots, ordered);
+    if (L.total > kScratchBytes)
+        return fail("btx_spmv_event_noalloc: lanes x destinations need " +
+                    std::to_string(L.total) + " bytes of scratch, more than the " +
+                    std::to_string(kScratchBytes) + " this entry point holds per device");
+    char* ws = nullptr;
+    if (int rc = scratch_of_current_device(&ws)) return rc;
+    // the term counts / accumulators start at zero: clear them on this
+    // stream (the array keeps whatever the last call on this device left)
+    const size_t beg = ordered ? L.cnt : L.acc;
+    const size_t end = ordered ? L.slots : L.total;
+    const int64_t words = (int64_t)((end - beg) / 4);
+    const hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_clear_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s,
+                       (uint32_t*)(ws + beg), words);
+    if (int rc = check_launch("btx_spmv_event_noalloc clear")) return rc;
+    return spmv_event_on(row_ptr, col, w, n_src, n_dst, nnz, csc_ptr, csc_src, csc_w, active,
+                         counts, lanes, active_lane_stride, x, x_lane_stride, out, ordered,
+                         slots, ws, L, s);
 }
 
 BTX_API int btx_spmv_transpose(const int64_t* row_ptr, const int32_t* col, const float* w,

@@ -107,6 +107,26 @@ int btx_spmv_event(const int64_t* row_ptr, const int32_t* col, const float* w,
                    float* out, int ordered, int slots,
                    void* workspace, size_t workspace_bytes, void* stream);
 
+/* btx_spmv_event without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference: its
+ * Triton product always takes a workspace.  The scratch is a file-scope
+ * device array of btx_spmv_event_noalloc_max_bytes() bytes (32 MiB) that
+ * each GPU holds until the process ends; a call whose
+ * btx_spmv_workspace_bytes(lanes, n_dst, slots, ordered) exceeds it fails
+ * with a message and launches nothing.  The entry point allocates nothing
+ * and clears the part of the scratch that must start at zero on `stream`.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (different streams, or host threads) race; serialise them. */
+size_t btx_spmv_event_noalloc_max_bytes(void);
+int btx_spmv_event_noalloc(const int64_t* row_ptr, const int32_t* col, const float* w,
+                           int64_t n_src, int64_t n_dst, int64_t nnz,
+                           const int64_t* csc_ptr, const int32_t* csc_src, const float* csc_w,
+                           const int32_t* active, const int32_t* counts, int64_t lanes,
+                           int64_t active_lane_stride,
+                           const float* x, int64_t x_lane_stride,
+                           float* out, int ordered, int slots, void* stream);
+
 /* Transposed product for the backward of SparseConn:
  * gx[lane, s] = sum_e w[e] * g[lane, col[e]] over the CSR row s
  * (every row; g is dense).  Replaces the grad_x index_add of
