@@ -448,7 +448,9 @@ def main():
         "data": "synthetic (seeded power-law connectome, Poisson background)",
         "config": {"workload": "connectome forward sim, N=140k LIF, power-law out-degree "
                                "(2.1, 100..5000), delays 1-20 (D=21), 8 lanes, dt 0.1 ms, "
-                               "5 Hz Poisson background",
+                               "5 Hz Poisson background; timed: one model(x) call of "
+                               f"T={K} steps (x resident in HBM, dense spike record out)",
+                   "steps_per_call": K,
                    "kick": args.kick,
                    "n": n, "synapses": nnz, "lanes": lanes, "delay_bins": D,
                    "weights": f"lognormal(mu={args.w_mu}, sigma=0.5), inhibitory x4",
@@ -705,12 +707,45 @@ def train_bench(args, dev):
     out = [step() for _ in range(args.train_steps)]
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / args.train_steps
+    cpu = None
+    if not args.no_cpu:
+        cpu = train_cpu_baseline(model, x_sample=batch()[0], T_s=10)
     return {"workload": "N=16384 LIF, ER p=0.05 (13.4M synapses), 200 Poisson inputs, "
                         "10 classes, batch 64, T=200, dt=1 ms, checkpoint every 50 steps",
             "synapses": int(c.nnz), "s_per_step": el, "samples_per_s": B / el,
             "neuron_steps_per_s": B * T * n / el, "loss": out[-1][0],
             "mean_rate_per_step": out[-1][1],
-            "grads": "synaptic weights (13.4M), tau_m (16384), readout"}
+            "grads": "synaptic weights (13.4M), tau_m (16384), readout",
+            "cpu_baseline": cpu}
+
+
+def train_cpu_baseline(model, x_sample, T_s: int):
+    """The reference-style autograd (torch-CPU oracle) of the same net on a
+    bounded sample: forward and backward over the first T_s steps of one
+    batch, scaled to the T = 200 iteration."""
+    from oracle import rsnn
+
+    threads = len(os.sched_getaffinity(0))
+    torch.set_num_threads(threads)
+    conn = model.synapse.linear
+    n = conn.out_features
+    dst, src = conn.indices[0].cpu(), conn.indices[1].cpu()
+    mag = conn.magnitude.detach().cpu().clone().requires_grad_(True)
+    tau = model.neuron.tau.detach().cpu().clone().requires_grad_(True)
+    x = model.neuron_inp_module(x_sample[:T_s].reshape(-1, x_sample.shape[-1])).detach()
+    x = x.reshape(T_s, -1, n).cpu()
+    sur = model.neuron.surrogate_function
+    t0 = time.perf_counter()
+    z, v, _, psc = rsnn.simulate_grad(n, dst, src, mag, x, tau, 1.0, 5.0, tau_ref=2.0,
+                                      surrogate=("superspike", sur.alpha, 1.0, True))
+    (z.mean(0).sum() + v.sum() * 1e-3).backward()
+    el = time.perf_counter() - t0
+    B = x.shape[1]
+    per_iter = el * 200 / T_s
+    return {"value": B / per_iter, "unit": "samples/s", "cores": threads, "kind": "port",
+            "s_per_iteration": per_iter,
+            "sample": f"forward + backward of the first {T_s} of the 200 steps of one batch "
+                      f"(torch-CPU oracle autograd, {threads} threads), scaled to T = 200"}
 
 
 def cpu_baseline(args, conn, snap, x_cpu, gpu_z, tables, W):

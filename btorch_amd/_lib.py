@@ -132,6 +132,7 @@ class RsnnPlan(ctypes.Structure):
         ("tables_ready", _c_int),
         ("neuron_kinds", _vp),
         ("mixed_models", _c_int),
+        ("ring_lane_bits", _vp),
     ]
 
 

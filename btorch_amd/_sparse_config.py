@@ -32,7 +32,7 @@ class HipSparseConfig(SparseBackendConfig):
         destination with more terms in one step is reduced by pulling its
         incoming edges (exact, slower for that destination).
     bin_cap: terms per (lane, 1024-destination segment) bin of the recurrent
-        engine's ordered mode, 0 = the maximum (1024); a bin that receives
+        engine's ordered mode, 0 = the maximum (8192); a bin that receives
         more is reduced by pulling every destination of the segment.
     """
 
@@ -43,8 +43,8 @@ class HipSparseConfig(SparseBackendConfig):
     def validate(self) -> None:
         if self.slots < 2 or self.slots > 16:
             raise ValueError("slots must be in [2, 16].")
-        if self.bin_cap < 0 or self.bin_cap > 1024:
-            raise ValueError("bin_cap must be in [0, 1024].")
+        if self.bin_cap < 0 or self.bin_cap > 8192:
+            raise ValueError("bin_cap must be in [0, 8192].")
 
 
 class SparseConfigRegistry:

@@ -60,6 +60,10 @@ struct Ctl {
     int age_of_sub[64];  // ring age read by virtual-row offset, -1 = not read
     int bin_delay[64];   // ring age of delay bin j
     int bin_sub[64];     // virtual-row offset of delay bin j
+    // step engine, ordered mode: spikes pushed into each ring slot (all
+    // lanes), and how the push of each step parity was delivered
+    int slot_total[64];
+    int pull_mode[2];    // 1 = the push was a dense pull into the accumulator
 };
 
 // Slots per destination of the ordered mode are held in registers when the

@@ -54,7 +54,9 @@ namespace btx {
 constexpr int kChunk = BTX_RING_CHUNK;  // neurons per F workgroup = ring segment = bin
 constexpr int kChunkShift = 10;
 constexpr int kThreads = 256;           // 4 neurons per thread in the F role
-constexpr int kBinCap = 1024;           // terms per bin held in LDS
+constexpr int kBinCap = 1024;           // terms per bin sorted in LDS
+constexpr int kBinCapMax = 8192;        // terms per bin in the workspace (larger bins are
+                                        // sorted through a global order array)
 constexpr int kSortMax = 32;            // most terms a neuron sorts in LDS
 constexpr int kMaxChunks = 1024;        // n <= 1,048,576 (push reservation table)
 constexpr int kStage = 1024;            // terms a push workgroup stages in LDS
@@ -85,10 +87,13 @@ struct ReduceIn {
     int cap;
     int sort_max;
     int* bin_cnt;                // [lanes][nchunks] of the buffer being consumed
-    const int32_t* ev_r;         // [lanes][nchunks][kBinCap]
+    const int32_t* ev_r;         // [lanes][nchunks][kBinCapMax]
     const float* ev_w;
     const uint16_t* ev_d;
+    uint16_t* ev_o;              // order of a bin's terms by neuron (bins past kBinCap)
     float* acc;                  // fast mode: [lanes][n] of the buffer being consumed
+    const int* pull_mode;        // ordered mode: 1 = the push was a dense pull ...
+    const float* pull_acc;       // ... whose sums are here, [lanes][n]
     PullArgs pull;
 };
 
@@ -138,6 +143,9 @@ struct StepArgs {
     int32_t* ring_idx;     // this step's ring slot, [lanes][n], or NULL
     int32_t* ring_cnt;     // this step's ring slot, [lanes][nchunks]
     uint64_t* ring_words;  // this step's ring slot, [lanes][nwords]
+    uint64_t* lane_bits;   // ordered mode: this step's slot of the lane-bit ring, [n][lw]
+    int* slot_total;       // ordered mode: spikes pushed into this step's slot
+    int lw;                // 64-lane words per neuron of the lane-bit ring
     float* rec_z;          // step t or NULL
     float* rec_v;
     float* rec_ref;
@@ -197,7 +205,7 @@ struct PushStep {
     const int32_t* ring_cnt;     // [R][lanes][nchunks]
     int cap;
     int* bin_cnt;                // [lanes][nchunks] of this step's buffer
-    int32_t* ev_r;               // [lanes][nchunks][kBinCap]
+    int32_t* ev_r;               // [lanes][nchunks][kBinCapMax]
     float* ev_w;
     uint16_t* ev_d;
     float* acc;                  // fast mode: [lanes][n] of this step's buffer
@@ -205,6 +213,18 @@ struct PushStep {
     const int64_t* in_t0;
     int64_t t;
     int affine;                  // bin j is (delay0 + j, sub0 + j)
+    // ordered mode, dense pull (chosen per step when the push would bin more
+    // than pull_above terms per (lane, segment)): sums of every destination's
+    // active incoming edges in the coalesced order into pull_acc
+    uint64_t* lane_bits;         // [R][n][lw]: bit b of word (slot, j, b/64) = lane b pushed j
+    int lw;
+    Ctl* ctl_rw;                 // the control block, written: slot totals, mode
+    float terms_per_row;         // nnz / (n * csr_d)
+    float pull_above;            // terms per bin above which the step pulls
+    float* pull_acc;             // [lanes][n] of this step's buffer
+    const int64_t* csc_ptr;
+    const int32_t* csc_src;
+    const float* csc_w;
     int delay0;
     int sub0;
 };
@@ -616,12 +636,75 @@ __device__ __forceinline__ RedHead reduce_head(const ReduceIn& a, int64_t nchunk
                                                int64_t seg) {
     RedHead h{0, 0, 0, 0.f};
     if (!a.ordered) return h;
-    const int64_t q0 = (lane * nchunks + seg) * (int64_t)kBinCap + threadIdx.x;
+    if (a.pull_mode != nullptr && ld_const(a.pull_mode) != 0) return h;
+    const int64_t q0 = (lane * nchunks + seg) * (int64_t)kBinCapMax + threadIdx.x;
     h.d = a.ev_d[q0];
     h.r = a.ev_r[q0];
     h.w = a.ev_w[q0];
     h.c = ld_const(a.bin_cnt + lane * nchunks + seg);
     return h;
+}
+
+// A bin of more terms than LDS holds (kBinCap < c <= cap): the same counting
+// sort by neuron, with the order kept in the bin's global order array, then
+// each neuron sums its terms in ascending virtual row by repeated selection
+// of the next larger row (its few terms stay in L2).  Neurons with more
+// than sort_max terms pull instead.  L.cnt is zero on entry (set by the
+// caller, before its barrier).
+__device__ void reduce_big_bin(const ReduceIn& a, RedLds& L, int64_t n, int64_t lane, int64_t q0,
+                               int c, int64_t n0, float s4[4]) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < c; i += kThreads) atomicAdd(&L.cnt[a.ev_d[q0 + i] & (kChunk - 1)], 1);
+    __syncthreads();
+    int v[4];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = L.cnt[tid * 4 + k];
+        sum += v[k];
+    }
+    int total;
+    int base = block_excl_scan<kThreads / 64>(sum, total);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        L.cnt[tid * 4 + k] = base;
+        base += v[k];
+    }
+    __syncthreads();
+    uint16_t* ord = a.ev_o + q0;
+    for (int i = tid; i < c; i += kThreads) {
+        const int p = atomicAdd(&L.cnt[a.ev_d[q0 + i] & (kChunk - 1)], 1);
+        ord[p] = (uint16_t)i;
+    }
+    __threadfence_block();
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int cd = v[k];
+        if (cd == 0 || n0 + k >= n) continue;
+        if (cd > a.sort_max) {
+            s4[k] = pull_sum(a.pull, lane, n0 + k);
+            continue;
+        }
+        const int end = L.cnt[tid * 4 + k], start = end - cd;
+        float s = 0.f;
+        int last = -1;
+        for (int m = 0; m < cd; ++m) {
+            int best = 0x7fffffff;
+            float bw = 0.f;
+            for (int x = start; x < end; ++x) {
+                const int t = ord[x];
+                const int r = a.ev_r[q0 + t];
+                if (r > last && r < best) {
+                    best = r;
+                    bw = a.ev_w[q0 + t];
+                }
+            }
+            s = s + bw;
+            last = best;
+        }
+        s4[k] = s;
+    }
 }
 
 // s4[k] = summed input of neuron seg*1024 + tid*4 + k of `lane` (+0.0f when
@@ -645,8 +728,18 @@ __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int
         }
         return;
     }
+    if (a.pull_mode != nullptr && ld_const(a.pull_mode) != 0) {
+        // the push was delivered as dense sums: the bins stayed empty
+        if (n0 < n) {
+            const float* q = a.pull_acc + lane * n + n0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (n0 + k < n) s4[k] = q[k];
+        }
+        return;
+    }
     int* cntp = a.bin_cnt + lane * nchunks + seg;
-    const int64_t q0 = (lane * nchunks + seg) * (int64_t)kBinCap;
+    const int64_t q0 = (lane * nchunks + seg) * (int64_t)kBinCapMax;
     const int d_pre = h.d, r_pre = h.r;
     const float w_pre = h.w;
     const int c = h.c;
@@ -660,6 +753,10 @@ __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (n0 + k < n) s4[k] = pull_sum(a.pull, lane, n0 + k);
+        return;
+    }
+    if (c > kBinCap) {
+        reduce_big_bin(a, L, n, lane, q0, c, n0, s4);
         return;
     }
     for (int i = tid; i < c; i += kThreads) {
@@ -729,9 +826,9 @@ __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int
 
 // Ascending list of one segment: thread tid holds the pushed bits `bits` of
 // neurons seg_base + tid*4 + b.  Every thread of the workgroup calls it.
-__device__ __forceinline__ void write_segment(uint32_t bits, int64_t seg_base,
-                                              int32_t* __restrict__ idx,
-                                              int32_t* __restrict__ cnt) {
+__device__ __forceinline__ int write_segment(uint32_t bits, int64_t seg_base,
+                                             int32_t* __restrict__ idx,
+                                             int32_t* __restrict__ cnt) {
     int total;
     int pos = block_excl_scan<kThreads / 64>(__popc(bits), total);
     while (bits) {
@@ -740,6 +837,20 @@ __device__ __forceinline__ void write_segment(uint32_t bits, int64_t seg_base,
         bits &= bits - 1;
     }
     if (threadIdx.x == 0) *cnt = total;
+    return total;
+}
+
+// The lane-bit ring of the dense pull: lane `lane` pushed neurons
+// n0 + k (bit k of `bits`) into the slot whose rows start at `slot_bits`.
+__device__ __forceinline__ void mark_lane_bits(uint64_t* __restrict__ slot_bits, int lw,
+                                               int64_t n0, int64_t lane, uint32_t bits) {
+    const int64_t g = lane >> 6;
+    const unsigned long long m = 1ull << (lane & 63);
+    while (bits) {
+        const int b = __ffs(bits) - 1;
+        atomicOr((unsigned long long*)(slot_bits + (n0 + b) * lw + g), m);
+        bits &= bits - 1;
+    }
 }
 
 // State of neurons n0 .. n0+3 of a lane (`room` of them exist): one 16-byte
@@ -976,8 +1087,10 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
             if (room < 4) pushed &= room > 0 ? (1u << room) - 1u : 0u;
         }
         if (lead) a.ring_words[lane * a.nwords + wi] = pw;
-        write_segment(pushed, seg_base, a.ring_idx + lane * a.n,
-                      a.ring_cnt + lane * a.nchunks + seg);
+        if (a.lane_bits != nullptr) mark_lane_bits(a.lane_bits, a.lw, n0, lane, pushed);
+        const int total = write_segment(pushed, seg_base, a.ring_idx + lane * a.n,
+                                        a.ring_cnt + lane * a.nchunks + seg);
+        if (a.slot_total != nullptr && threadIdx.x == 0 && total) atomicAdd(a.slot_total, total);
     }
 }
 
@@ -990,8 +1103,102 @@ __device__ __forceinline__ int ring_slot(int64_t clock, int delay, int R) {
     return (int)(s < 0 ? s + R : s);
 }
 
+// Ordered mode: whether the push of this step is delivered as a dense pull.
+// Every P workgroup decides alike, from the spikes pushed into the ring
+// slots the step reads (counted in earlier launches): the pull is taken
+// when the push would bin more than pull_above terms per (lane, segment)
+// on average.
+__device__ __forceinline__ bool choose_pull(const PushStep& a) {
+    if (a.lane_bits == nullptr) return false;
+    float rows = 0.f;
+    for (int j = 0; j < a.nbins; ++j) {
+        const int dl = a.affine ? a.delay0 + j : a.ctl->bin_delay[j];
+        rows += (float)a.ctl_rw->slot_total[ring_slot(a.clock, dl, a.R)];
+    }
+    return rows * a.terms_per_row > a.pull_above * (float)(a.lanes * a.nchunks);
+}
+
+// The dense pull of one step: workgroup `wid` of `nwg` sums, for a range of
+// destinations and every lane, the weights of the active incoming edges in
+// the coalesced (dst, virtual source) order from +0.0f -- the terms and the
+// order of the ordered push, so the same bits.  Virtual row r = src*csr_d +
+// sub is active for lane b when bit b of the lane-bit ring word of src in the
+// slot that sub's age points at is set.  Groups of Gp threads (one per lane
+// of a 64-lane group) walk one destination's edges together.
+__device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nwg) {
+    const int tid = threadIdx.x;
+    for (int k = tid; k < a.csr_d; k += kThreads) {
+        const int age = a.ctl->age_of_sub[k];
+        L.slot[k] = age < 0 ? -1 : ring_slot(a.clock, age, a.R);
+    }
+    __syncthreads();
+    const int64_t per = (a.n + nwg - 1) / nwg;
+    const int64_t d0 = wid * per;
+    const int64_t d1 = d0 + per < a.n ? d0 + per : a.n;
+    for (int g = 0; g < a.lw; ++g) {
+        const int lanes_g = (int)(a.lanes - 64 * (int64_t)g < 64 ? a.lanes - 64 * (int64_t)g : 64);
+        int Gp = 1;
+        while (Gp < lanes_g) Gp <<= 1;
+        const int gl = tid % Gp;
+        const int64_t lane = 64 * (int64_t)g + gl;
+        const uint64_t* bits = a.lane_bits + g;
+        for (int64_t d = d0 + tid / Gp; d < d1; d += kThreads / Gp) {
+            float acc = 0.f;
+            int64_t e = a.csc_ptr[d];
+            const int64_t e1 = a.csc_ptr[d + 1];
+            constexpr int U = 8;
+            for (; e < e1; e += U) {
+                int r[U];
+                float w[U];
+                uint64_t wb[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool in = e + u < e1;
+                    r[u] = in ? a.csc_src[e + u] : 0;
+                    w[u] = in ? a.csc_w[e + u] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int src = r[u] / a.csr_d;
+                    const int slot = L.slot[r[u] - src * a.csr_d];
+                    wb[u] = (e + u < e1 && slot >= 0)
+                                ? bits[((int64_t)slot * a.n + src) * a.lw] : 0ull;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if ((wb[u] >> gl) & 1ull) acc = acc + w[u];
+            }
+            if (gl < lanes_g) {
+                const bool live = a.lane_t_end == nullptr ||
+                                  a.in_t0[lane] + a.t < a.lane_t_end[lane];
+                a.pull_acc[lane * a.n + d] = live ? acc : 0.f;
+            }
+        }
+    }
+}
+
 template <int G, bool ORDERED>
 __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) {
+    if (ORDERED && a.lane_bits != nullptr) {
+        // the ring slot the next step writes starts empty: its lane bits
+        // (spread over the workgroups) and its spike count; no workgroup of
+        // this launch reads that slot (its age would be ring_d)
+        const int64_t np = (a.nchunks + a.cpb - 1) / a.cpb;
+        const int64_t nwg = np * a.lanes, wid = lane * np + px;
+        const int nxt = ring_slot(a.clock + 1, 0, a.R);
+        uint64_t* nb = a.lane_bits + (int64_t)nxt * a.n * a.lw;
+        for (int64_t i = wid * kThreads + threadIdx.x; i < a.n * a.lw; i += nwg * kThreads)
+            nb[i] = 0ull;
+        const bool pull = choose_pull(a);
+        if (wid == 0 && threadIdx.x == 0) {
+            a.ctl_rw->slot_total[nxt] = 0;
+            a.ctl_rw->pull_mode[a.t & 1] = pull ? 1 : 0;
+        }
+        if (pull) {
+            pull_role(a, L, wid, nwg);
+            return;
+        }
+    }
     if (a.lane_t_end != nullptr && a.in_t0[lane] + a.t >= a.lane_t_end[lane]) return;
     const int64_t c0 = px * a.cpb;
     const int ncb = (int)(a.nchunks - c0 < a.cpb ? a.nchunks - c0 : a.cpb);
@@ -1064,7 +1271,7 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
                     const int tl = dst >> kChunkShift;
                     const int pos = L.base[tl] + atomicAdd(&L.tile[tl], 1);
                     if (pos < a.cap) {
-                        const int64_t q = (lane * a.nchunks + tl) * (int64_t)kBinCap + pos;
+                        const int64_t q = (lane * a.nchunks + tl) * (int64_t)kBinCapMax + pos;
                         a.ev_r[q] = r;
                         a.ev_w[q] = wv;
                         a.ev_d[q] = (uint16_t)(dst & (kChunk - 1));
@@ -1091,7 +1298,7 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
                     const int tl = dst >> kChunkShift;
                     const int pos = L.base[tl] + atomicAdd(&L.tile[tl], 1);
                     if (pos < a.cap) {
-                        const int64_t q = (lane * a.nchunks + tl) * (int64_t)kBinCap + pos;
+                        const int64_t q = (lane * a.nchunks + tl) * (int64_t)kBinCapMax + pos;
                         a.ev_r[q] = L.er[i];
                         a.ev_w[q] = L.ew[i];
                         a.ev_d[q] = (uint16_t)(dst & (kChunk - 1));
@@ -1161,7 +1368,9 @@ __global__ __launch_bounds__(kThreads) void k_ring_from_words(const uint64_t* __
                                                               int64_t n, int64_t nwords,
                                                               int64_t lanes, int64_t nchunks,
                                                               int32_t* __restrict__ ring_idx,
-                                                              int32_t* __restrict__ ring_cnt) {
+                                                              int32_t* __restrict__ ring_cnt,
+                                                              uint64_t* __restrict__ lane_bits,
+                                                              int lw, Ctl* __restrict__ ctl) {
     const int64_t seg = blockIdx.x, lane = blockIdx.y, slot = blockIdx.z;
     const int64_t seg_base = seg * kChunk;
     const int64_t n0 = seg_base + (int64_t)threadIdx.x * 4;
@@ -1173,8 +1382,18 @@ __global__ __launch_bounds__(kThreads) void k_ring_from_words(const uint64_t* __
         const int64_t room = n - n0;  // bits past n never count
         if (room < 4) nib &= (1u << room) - 1u;
     }
-    write_segment(nib, seg_base, ring_idx + (slot * lanes + lane) * n,
-                  ring_cnt + (slot * lanes + lane) * nchunks + seg);
+    if (lane_bits != nullptr) mark_lane_bits(lane_bits + slot * n * lw, lw, n0, lane, nib);
+    const int total = write_segment(nib, seg_base, ring_idx + (slot * lanes + lane) * n,
+                                    ring_cnt + (slot * lanes + lane) * nchunks + seg);
+    if (lane_bits != nullptr && threadIdx.x == 0 && total) atomicAdd(&ctl->slot_total[slot], total);
+}
+
+// Empty lane-bit ring and slot counts (before a rebuild of the ring).
+__global__ __launch_bounds__(256) void k_pull_clear(uint64_t* __restrict__ lane_bits,
+                                                    int64_t words, Ctl* __restrict__ ctl) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) lane_bits[i] = 0ull;
+    if (i < 64) ctl->slot_total[i] = 0;
 }
 
 // Standalone exponential-synapse decay psc += k*((-psc)/tau_s) (ode.py:50).
@@ -1256,7 +1475,7 @@ static inline int64_t mod(int64_t a, int64_t b) {
 // Engine workspace: control tables, then two buffers of term bins (ordered)
 // or of accumulators (fast).
 struct EngineWs {
-    size_t bin_cnt = 0, ev_r = 0, ev_w = 0, ev_d = 0, acc = 0, total = 0;
+    size_t bin_cnt = 0, ev_r = 0, ev_w = 0, ev_d = 0, ev_o = 0, acc = 0, total = 0;
     int64_t bins = 0;  // per buffer
     int64_t m = 0;     // accumulators per buffer
 };
@@ -1270,11 +1489,16 @@ static EngineWs engine_ws(int64_t lanes, int64_t n, int ordered) {
         L.bin_cnt = off;
         off = align256(off + sizeof(int) * 2 * L.bins);
         L.ev_r = off;
-        off = align256(off + sizeof(int32_t) * 2 * L.bins * kBinCap);
+        off = align256(off + sizeof(int32_t) * 2 * L.bins * kBinCapMax);
         L.ev_w = off;
-        off = align256(off + sizeof(float) * 2 * L.bins * kBinCap);
+        off = align256(off + sizeof(float) * 2 * L.bins * kBinCapMax);
         L.ev_d = off;
-        off = align256(off + sizeof(uint16_t) * 2 * L.bins * kBinCap);
+        off = align256(off + sizeof(uint16_t) * 2 * L.bins * kBinCapMax);
+        L.ev_o = off;
+        off = align256(off + sizeof(uint16_t) * 2 * L.bins * kBinCapMax);
+        L.m = lanes * n;  // dense pull sums, two buffers
+        L.acc = off;
+        off = align256(off + sizeof(float) * 2 * L.m);
     } else {
         L.m = lanes * n;
         L.acc = off;
@@ -1625,8 +1849,8 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         return fail("btx_rsnn_run: ordered mode needs the dst-major arrays");
     if (p->ordered && (p->slots < 1 || p->slots > kSortMax))
         return fail("btx_rsnn_run: ordered mode needs 1 <= slots <= 32");
-    if (p->bin_cap < 0 || p->bin_cap > kBinCap)
-        return fail("btx_rsnn_run: bin_cap must be in [0, 1024]");
+    if (p->bin_cap < 0 || p->bin_cap > kBinCapMax)
+        return fail("btx_rsnn_run: bin_cap must be in [0, 8192]");
     if (p->in_prob != nullptr && (p->in_seed == nullptr || p->in_t0 == nullptr))
         return fail("btx_rsnn_run: Poisson drive needs per-lane seeds and clocks");
     if (p->lane_t_end != nullptr && p->in_t0 == nullptr)
@@ -1651,13 +1875,23 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         tb.bin_sub[j] = p->bin_sub[j];
     }
     if (!p->tables_ready) hipLaunchKernelGGL(k_set_tables, dim3(1), dim3(64), 0, s, tb, ctl);
-    if (p->ring_rebuild)
+    // lane-bit ring of the dense pull (ordered mode with the plan's ring of
+    // lane bits); 64-lane words per neuron
+    uint64_t* lane_bits = p->ordered ? p->ring_lane_bits : nullptr;
+    const int lw = (int)((lanes + 63) / 64);
+    if (p->ring_rebuild) {
+        if (lane_bits != nullptr) {
+            const int64_t words = (int64_t)R * n * lw;
+            hipLaunchKernelGGL(k_pull_clear, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
+                               s, lane_bits, words, ctl);
+        }
         hipLaunchKernelGGL(k_ring_from_words,
                            dim3((unsigned)nchunks, (unsigned)lanes, (unsigned)R),
                            dim3(kThreads), 0, s, p->ring_words, n, nwords, lanes, nchunks,
-                           p->ring_idx, p->ring_cnt);
+                           p->ring_idx, p->ring_cnt, lane_bits, lw, ctl);
+    }
 
-    const int cap = p->bin_cap > 0 ? p->bin_cap : kBinCap;
+    const int cap = p->bin_cap > 0 ? p->bin_cap : kBinCapMax;
     // buffer b of the term bins / accumulators
     auto bins_of = [&](int b, ReduceIn& r) {
         r.ordered = p->ordered;
@@ -1665,9 +1899,12 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         r.sort_max = p->slots;
         if (p->ordered) {
             r.bin_cnt = (int*)(ws + L.bin_cnt) + b * L.bins;
-            r.ev_r = (const int32_t*)(ws + L.ev_r) + b * L.bins * kBinCap;
-            r.ev_w = (const float*)(ws + L.ev_w) + b * L.bins * kBinCap;
-            r.ev_d = (const uint16_t*)(ws + L.ev_d) + b * L.bins * kBinCap;
+            r.ev_r = (const int32_t*)(ws + L.ev_r) + b * L.bins * kBinCapMax;
+            r.ev_w = (const float*)(ws + L.ev_w) + b * L.bins * kBinCapMax;
+            r.ev_d = (const uint16_t*)(ws + L.ev_d) + b * L.bins * kBinCapMax;
+            r.ev_o = (uint16_t*)(ws + L.ev_o) + b * L.bins * kBinCapMax;
+            r.pull_mode = p->ring_lane_bits ? &ctl->pull_mode[b] : nullptr;
+            r.pull_acc = (float*)(ws + L.acc) + b * L.m;
         } else {
             r.acc = (float*)(ws + L.acc) + b * L.m;
         }
@@ -1717,6 +1954,14 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ps.cap = cap;
     ps.lane_t_end = p->lane_t_end;
     ps.in_t0 = p->in_t0;
+    ps.lane_bits = lane_bits;
+    ps.lw = lw;
+    ps.ctl_rw = ctl;
+    ps.terms_per_row = (float)((double)p->nnz / (double)(n * p->csr_d));
+    ps.pull_above = 0.5f * (float)cap;
+    ps.csc_ptr = p->csc_ptr;
+    ps.csc_src = p->csc_src;
+    ps.csc_w = p->csc_w;
     ps.affine = 1;
     ps.delay0 = p->bin_delay[0];
     ps.sub0 = p->bin_sub[0];
@@ -1756,6 +2001,9 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         a.ring_idx = p->ring_idx + slot * lanes * n;
         a.ring_cnt = p->ring_cnt + slot * lanes * nchunks;
         a.ring_words = p->ring_words + slot * lanes * nwords;
+        a.lane_bits = lane_bits ? lane_bits + slot * n * lw : nullptr;
+        a.slot_total = lane_bits ? &ctl->slot_total[slot] : nullptr;
+        a.lw = lw;
         a.rec_z = p->rec_z ? p->rec_z + t * rec_step : nullptr;
         a.rec_v = p->rec_v ? p->rec_v + t * rec_step : nullptr;
         a.rec_ref = p->rec_refractory ? p->rec_refractory + t * rec_step : nullptr;
@@ -1779,6 +2027,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         ps.ev_w = (float*)cur.ev_w;
         ps.ev_d = (uint16_t*)cur.ev_d;
         ps.acc = cur.acc;
+        ps.pull_acc = cur.pull_acc ? (float*)cur.pull_acc : nullptr;
         if (fused) {
             kt.begin(0);
             const dim3 grid((unsigned)(nf + npb), (unsigned)lanes);

@@ -251,7 +251,8 @@ class _RingState:
     the (tensor, version, cursor) of the module history it last matched."""
 
     def __init__(self, ring_d, lanes, n, dev):
-        self.idx, self.cnt, self.words = Engine._ring_buffers(ring_d, lanes, n, dev)
+        self.idx, self.cnt, self.words, self.lane_bits = Engine._ring_buffers(ring_d, lanes, n,
+                                                                              dev)
         self.R = ring_d + 1
         self.cursor = ring_d          # device slot of the next run's first step
         self.token = None             # module history the ring holds, or None
@@ -412,13 +413,30 @@ class Engine:
 
     @staticmethod
     def _ring_buffers(ring_d, lanes, n, dev):
-        """Device ring of ring_d + 1 slots (lists, counts per segment, words)."""
+        """Device ring of ring_d + 1 slots: lists, counts per segment, words,
+        and the lane bits of the dense pull ((R, n, ceil(lanes/64)) words,
+        bit b of word (slot, j, b // 64) = lane b pushed neuron j)."""
         R = ring_d + 1
         nwords = (n + 63) // 64
         nchunks = (n + _lib.RING_CHUNK - 1) // _lib.RING_CHUNK
         return (torch.empty((R, lanes, n), dtype=torch.int32, device=dev),
                 torch.zeros((R, lanes, nchunks), dtype=torch.int32, device=dev),
-                torch.zeros((R, lanes, nwords), dtype=torch.int64, device=dev))
+                torch.zeros((R, lanes, nwords), dtype=torch.int64, device=dev),
+                torch.zeros((R, n, (lanes + 63) // 64), dtype=torch.int64, device=dev))
+
+    @staticmethod
+    def clear_lane_bits(lane_bits: torch.Tensor, sl: slice) -> None:
+        """Clear lanes ``sl`` from a ring's lane bits (lanes emptied of a job)."""
+        lanes = range(*sl.indices(lane_bits.shape[-1] * 64))
+        for g in range(lane_bits.shape[-1]):
+            mask = 0
+            for b in lanes:
+                if b // 64 == g:
+                    mask |= 1 << (b % 64)
+            if mask:
+                keep = ~mask & 0xFFFFFFFFFFFFFFFF
+                keep = keep - (1 << 64) if keep >= 1 << 63 else keep  # as int64
+                lane_bits[..., g].bitwise_and_(keep)
 
     @staticmethod
     def _set_outputs(p, T, lanes, n, drive, counts, sample):
@@ -573,6 +591,7 @@ class Engine:
         p.ring_idx = ring.idx.data_ptr()
         p.ring_cnt = ring.cnt.data_ptr()
         p.ring_words = ring.words.data_ptr()
+        p.ring_lane_bits = ring.lane_bits.data_ptr()
         p.ring_cursor = ring.cursor
         self._set_outputs(p, T, lanes, n, drive, counts, sample)
         nwords = (n + 63) // 64
@@ -686,11 +705,13 @@ class ResidentRun:
         self.syn_state = [getattr(sp.exp_psc, k) for k in getattr(sp.exp_psc, "_STATES", ())]
         self.iasc = getattr(neuron, "Iasc", None)
         self.v_reset = neuron.v_reset
-        self.idx, self.cnt, self.words = engine._ring_buffers(sp.ring_d, lanes, n, dev)
-        self.keep += [self.idx, self.cnt, self.words]
+        self.idx, self.cnt, self.words, self.lane_bits = engine._ring_buffers(sp.ring_d, lanes,
+                                                                              n, dev)
+        self.keep += [self.idx, self.cnt, self.words, self.lane_bits]
         p.ring_idx = self.idx.data_ptr()
         p.ring_cnt = self.cnt.data_ptr()
         p.ring_words = self.words.data_ptr()
+        p.ring_lane_bits = self.lane_bits.data_ptr()
         p.ring_cursor = 0
         p.ring_rebuild = 0  # lists stay current between launches (all empty at start)
         engine._set_outputs(p, T, lanes, n, drive, counts, sample)
@@ -721,3 +742,4 @@ class ResidentRun:
             t.view(self.lanes, -1)[sl] = 0.0
         self.cnt[:, sl] = 0
         self.words[:, sl] = 0
+        Engine.clear_lane_bits(self.lane_bits, sl)

@@ -365,7 +365,9 @@ typedef struct btx_rsnn_plan {
      * from the previous run) */
     int ring_rebuild;
     /* ordered mode: terms per (lane, destination segment) bin, 0 = the
-     * maximum (1024); a bin that receives more is reduced by pulling */
+     * maximum (8192); a bin that receives more is reduced by pulling.  Bins
+     * of up to 1024 terms are sorted in LDS, larger ones through an order
+     * array in the workspace */
     int bin_cap;
     /* two-compartment GLIF (neuron_model == BTX_NEURON_TWO_COMPARTMENT,
      * TwoCompartmentGLIF.single_step_forward, two_compartment.py:208-292),
@@ -433,6 +435,15 @@ typedef struct btx_rsnn_plan {
      * their update unchanged.  Exponential synapse only. */
     const uint8_t* neuron_kinds;  /* (n,) */
     int mixed_models;
+    /* ordered mode, optional (NULL = off): the ring of lane bits, uint64
+     * [ring_d + 1][n][ceil(lanes/64)] kept beside ring_words -- bit b of word
+     * (slot, j, b/64) set when lane b pushed neuron j into that slot.  With
+     * it, a step whose push would bin more than bin_cap/2 terms per (lane,
+     * segment) on average is delivered as a dense pull instead: every
+     * destination sums its active incoming edges in the coalesced order
+     * (the same terms and order, so the same bits).  Zero it with the ring;
+     * a caller that empties lanes of the ring clears their bits too. */
+    uint64_t* ring_lane_bits;
 } btx_rsnn_plan;
 
 size_t btx_rsnn_workspace_bytes(int64_t lanes, int64_t n, int slots, int ordered);
