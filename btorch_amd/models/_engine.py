@@ -259,6 +259,37 @@ class _RingState:
         self.fresh = True             # lists not built yet
 
 
+class _HistoryWriteback:
+    """Writes a SpikeHistory memory from the device ring that holds it: module
+    slot j holds delay d(j) (circular: d = (cursor - 1 - j) mod D; cat: d = j),
+    and delay d sits in device slot (dev_cursor - 1 - d) mod R."""
+
+    __slots__ = ("engine", "ring", "lanes", "n", "ring_d", "R", "cursor", "dev_cursor")
+
+    def __init__(self, engine, ring, lanes, n, ring_d, R, cursor, dev_cursor):
+        self.engine, self.ring, self.lanes, self.n = engine, ring, lanes, n
+        self.ring_d, self.R, self.cursor, self.dev_cursor = ring_d, R, cursor, dev_cursor
+
+    def __call__(self, h) -> None:
+        lanes, n, ring_d = self.lanes, self.n, self.ring_d
+        buf = h._buffers.get("history")
+        if buf is None or buf.numel() != lanes * ring_d * n:
+            raise RuntimeError("SpikeHistory changed shape while its delay history was held "
+                               "on the device")
+        if not (buf.is_contiguous() and buf.dtype == torch.float32):
+            buf = torch.empty(buf.shape, dtype=torch.float32, device=buf.device)
+            h._buffers["history"] = buf
+        j = torch.arange(ring_d, dtype=torch.long, device=buf.device)
+        d = (self.cursor - 1 - j) % ring_d if h.use_circular_buffer else j
+        slots = (self.dev_cursor - 1 - d) % self.R
+        words = self.ring.words.index_select(0, slots).transpose(0, 1).contiguous()
+        from .._kernels import unpack_rows
+        unpack_rows(words.reshape(lanes * ring_d, -1), n, out=buf.view(lanes * ring_d, n))
+        if h.use_circular_buffer:
+            h._buffers["_cursor"].fill_(self.cursor)
+        self.ring.token = self.engine._hist_token(h)
+
+
 class Engine:
     """Runs a matched cell for T steps; owns the ring and the workspace.
 
@@ -525,33 +556,7 @@ class Engine:
             return
         cursor = (hist.cursor + T) % hist.max_delay if hist.use_circular_buffer else 0
         ring_d, R = self.spec.ring_d, ring.R
-        dev_cursor = ring.cursor
-        engine = self
-
-        class _Writeback:
-            def __call__(self_, h):
-                buf = h._buffers.get("history")
-                if buf is None or buf.numel() != lanes * ring_d * n:
-                    raise RuntimeError("SpikeHistory changed shape while its delay history was "
-                                       "held on the device")
-                if not (buf.is_contiguous() and buf.dtype == torch.float32):
-                    buf = torch.empty(buf.shape, dtype=torch.float32, device=buf.device)
-                    h._buffers["history"] = buf
-                # module slot j holds delay d(j); delay d is device slot
-                # (dev_cursor - 1 - d) mod R
-                j = torch.arange(ring_d, dtype=torch.long, device=buf.device)
-                d = (cursor - 1 - j) % ring_d if h.use_circular_buffer else j
-                slots = (dev_cursor - 1 - d) % R
-                words = ring.words.index_select(0, slots).transpose(0, 1).contiguous()
-                from .._kernels import unpack_rows
-                unpack_rows(words.reshape(lanes * ring_d, -1), n,
-                            out=buf.view(lanes * ring_d, n))
-                if h.use_circular_buffer:
-                    h._buffers["_cursor"].fill_(cursor)
-                ring.token = engine._hist_token(h)
-
-        fn = _Writeback()
-        fn.ring = ring
+        fn = _HistoryWriteback(self, ring, lanes, n, ring_d, R, cursor, ring.cursor)
         hist.defer_writeback(fn, cursor)
 
     # -- run --------------------------------------------------------------------

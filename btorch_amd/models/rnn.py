@@ -101,10 +101,17 @@ class RecurrentNNAbstract(base.MemoryModule):
 
     def multi_step_forward(self, *args, loop_args=None, **kwargs):
         first = args[0]
+        if self._runs_fused_without_grad(*args, **kwargs):
+            # the fused engine keeps its own packed weights (rebuilt when a
+            # weight changes); the per-run packing is for the step loop
+            return self._multi_step_loop(*args, loop_args=loop_args, **kwargs)
         batch = (first[0].numel() // first.shape[-1]
                  if torch.is_tensor(first) and first.ndim >= 2 else None)
         with prepare_sparse_modules(self, batch_size=batch):
             return self._multi_step_loop(*args, loop_args=loop_args, **kwargs)
+
+    def _runs_fused_without_grad(self, *args, **kwargs) -> bool:
+        return False
 
     def _multi_step_loop(self, *args, loop_args=None, **kwargs):
         T, loop_args, _, large, nchunks = self._chunk_plan(*args, loop_args=loop_args)
@@ -209,7 +216,26 @@ class RecurrentNN(RecurrentNNAbstract):
         return self._engine
 
     def _record_names(self) -> list[str]:
-        return hidden_state_names(self, self.update_state_names, allow_buffer=self.allow_buffer)
+        key = (self._engine_identity(), None if self.update_state_names is None
+               else tuple(self.update_state_names), self.allow_buffer)
+        hit = getattr(self, "_names_cache", None)
+        if hit is None or hit[0] != key:
+            names = hidden_state_names(self, self.update_state_names,
+                                       allow_buffer=self.allow_buffer)
+            self._names_cache = hit = (key, names)
+        return list(hit[1])
+
+    def _runs_fused_without_grad(self, *args, **kwargs) -> bool:
+        """A call the no-grad fused path serves (one input sequence on a HIP
+        device, nothing requiring a gradient, every requested state
+        recordable)."""
+        if torch.is_grad_enabled() or kwargs or len(args) != 1 or self.cudagraph:
+            return False
+        x = args[0]
+        if not torch.is_tensor(x) or x.device.type != "cuda":
+            return False
+        eng = self.fused_engine()
+        return eng is not None and self._engine_records(eng)
 
     def _graph(self, eng, x: torch.Tensor):
         """The graph runner when ``cudagraph`` is set (checks first), else None."""
