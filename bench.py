@@ -44,10 +44,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--config", choices=("connectome", "sheet"), default="connectome",
+    ap.add_argument("--config", choices=("connectome", "sheet", "harness"), default="connectome",
                     help="connectome: the 140k-neuron headline (default); sheet: the largest "
                          "single-GPU configuration, 1M two-compartment neurons on a 2-D sheet "
-                         "with ~1B synapses")
+                         "with ~1B synapses; harness: the reference benchmark's shape "
+                         "(Hemibrain-sized net, T=128, batch 1, spike-rate sweep)")
     ap.add_argument("--sheet-steps", type=int, default=10_000,
                     help="timed steps of --config sheet")
     ap.add_argument("--n", type=int, default=140_000)
@@ -298,6 +299,9 @@ def main():
     if args.config == "sheet":
         print(json.dumps(sheet_bench(args, dev)), flush=True)
         return
+    if args.config == "harness":
+        print(json.dumps(harness_bench(args, dev)), flush=True)
+        return
     if args.copy_probe:
         # a plain streaming kernel of known traffic (512 MiB read, 512 MiB
         # written) to calibrate the memory counters
@@ -470,6 +474,129 @@ def main():
 
 
 SWEEP_PROBS = (0.001, 0.002, 0.005, 0.01, 0.02, 0.05, 0.1)
+
+# The reference benchmark's published rows (RTX 4060 Laptop, Hemibrain,
+# T=128, batch 1, CUDA-graph replay, median of 30): microseconds per step,
+# native torch.sparse.mm and Triton, for the product alone and for
+# RecurrentNN(LIF(hard reset), ExponentialPSC(SparseConn))
+# (benchmarks/sparse_rnn/results/hemibrain_rtx4060.md:26-32, 45-51).
+REFERENCE_HARNESS = {
+    "spmspv": {0.001: (396.93, 46.25), 0.002: (396.94, 38.19), 0.005: (396.90, 55.34),
+               0.01: (396.98, 75.33), 0.02: (396.90, 118.08), 0.05: (396.91, 247.03),
+               0.1: (396.88, 445.70)},
+    "snn": {0.001: (434.86, 72.56), 0.002: (435.30, 75.02), 0.005: (434.67, 88.82),
+            0.01: (434.91, 108.74), 0.02: (434.69, 152.05), 0.05: (434.99, 279.18),
+            0.1: (434.46, 480.68)},
+}
+
+
+def harness_bench(args, dev):
+    """The reference benchmark's shape on this GPU (benchmark_triton_spmspv.py:
+    104-330): a Hemibrain-sized synthetic connectome (21,739 neurons,
+    3,550,403 edges: power-law out-degree with mean 163.3, uniform targets,
+    lognormal(0, 0.5) positive weights normalised to a mean absolute fan-out
+    of 0.01 as the harness's normalize_weights does; the real Hemibrain
+    dataset is not available), T = 128, batch 1, Bernoulli spike traces at
+    0.1 .. 10 %.  "spmspv": 128 spike vectors through SparseConn, the
+    sequence captured in a HIP graph; "snn": RecurrentNN(LIF(tau 20, hard
+    reset), ExponentialPSC(tau 5, SparseConn), cudagraph=True) over the trace
+    x 1.05 at dt = 1 ms, the state reset before each sample.  10 warm-up and
+    30 timed replays, median per step, as the harness."""
+    import statistics
+
+    from btorch_amd import workloads
+    from btorch_amd.models import (
+        LIF, ExponentialPSC, RecurrentNN, SparseConn, environ, functional,
+    )
+    from btorch_amd.models.functional import prepare_sparse_modules, reset_net
+
+    n, E, T = 21_739, 3_550_403, 128
+    rng = np.random.default_rng(args.seed + 5)
+    deg = workloads.powerlaw_degrees(rng, n, 2.1, 40, 4000).astype(np.float64)
+    deg = np.maximum(1, np.round(deg * (E / deg.sum()))).astype(np.int64)
+    pre = np.repeat(np.arange(n, dtype=np.int64), deg)
+    post = rng.integers(0, n - 1, size=pre.size, dtype=np.int64)
+    post[post >= pre] += 1
+    key = np.unique(pre * n + post)
+    pre, post = key // n, key % n
+    w = rng.lognormal(0.0, 0.5, pre.size).astype(np.float32)
+    w *= np.float32(0.01 / max(float(np.abs(w).sum()) / n, 1.0))
+    import scipy.sparse
+    mat = scipy.sparse.coo_array((w, (pre, post)), shape=(n, n))
+
+    def trace(rate):
+        g = torch.Generator(device=dev).manual_seed(args.seed)
+        return (torch.rand((T, 1, n), device=dev, generator=g) < rate).to(torch.float32)
+
+    def time_replays(run, before=None):
+        for _ in range(10):
+            if before:
+                before()
+            run()
+        torch.cuda.synchronize()
+        out = []
+        for _ in range(30):
+            if before:
+                before()
+                torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            run()
+            e1.record()
+            e1.synchronize()
+            out.append(e0.elapsed_time(e1))
+        return statistics.median(out) * 1e3 / T
+
+    rows = []
+    conn = SparseConn(mat, enforce_dale=False, device=dev)
+    lif = LIF(n, tau=20.0, hard_reset=True)
+    model = RecurrentNN(lif, ExponentialPSC(n, 5.0, SparseConn(mat, enforce_dale=False,
+                                                               device=dev)),
+                        unroll=8, cudagraph=True, update_state_names=[]).to(dev)
+    functional.init_net_state(model, batch_size=1, device=dev, dtype=torch.float32)
+    for rate in SWEEP_PROBS:
+        tr = trace(rate)
+        # the product alone: the 128-step sequence captured in one graph
+        with torch.inference_mode(), prepare_sparse_modules(conn, 1):
+            outs = []
+
+            def seq():
+                outs.clear()
+                for t in range(T):
+                    outs.append(conn(tr[t]))
+
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                for _ in range(3):
+                    seq()
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                seq()
+            us_prod = time_replays(g.replay)
+        # the recurrent net, graph-replayed per call, reset before each sample
+        drive = tr * 1.05
+        with torch.no_grad(), environ.context(dt=1.0):
+            us_snn = time_replays(lambda: model(drive),
+                                  before=lambda: reset_net(model, batch_size=1, inplace=True))
+            z, _ = model(drive)
+        ref_p, ref_s = REFERENCE_HARNESS["spmspv"][rate], REFERENCE_HARNESS["snn"][rate]
+        rows.append({"rate": rate, "snn_rate": float(z.mean()),
+                     "spmspv_us_per_step": us_prod, "snn_us_per_step": us_snn,
+                     "reference_rtx4060_spmspv_native_triton_us": ref_p,
+                     "reference_rtx4060_snn_native_triton_us": ref_s,
+                     "snn_events_per_s": float(z.sum()) * float(E / n) / (us_snn * T * 1e-6)})
+        _progress(f"harness rate {rate}: product {us_prod:.2f} us/step, snn {us_snn:.2f} us/step")
+    return {"metric": "us per step at the reference benchmark's shape (Hemibrain-sized net)",
+            "value": rows[0]["snn_us_per_step"], "unit": "us/step", "n_gpus": 1, "steps": T,
+            "higher_is_better": False, "dtype": "f32", "vs_baseline": None,
+            "data": "synthetic Hemibrain-sized connectome (dataset unavailable offline)",
+            "config": {"workload": "reference harness shape: N=21,739, E=3.55M, T=128, batch 1, "
+                                   "rates 0.1-10%, graph replay", "n": n,
+                       "synapses": int(mat.nnz)},
+            "rows": rows}
 
 
 def activity_sweep(args, model, eng, deg, dev):

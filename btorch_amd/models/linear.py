@@ -56,7 +56,7 @@ class _SparseProduct(torch.autograd.Function):
         active, counts = _kernels.compact_dense(x2d)
         ws = conn._workspace_for(x2d.shape[0])
         cfg: HipSparseConfig = conn.sparse_config
-        w_dst = conn._get_effective_weight().detach().to(torch.float32).contiguous()
+        w_dst = conn.effective_weight_f32()
         out = _kernels.spmv_event(conn._layout, w_csr, x2d, active, counts,
                                   ordered=cfg.ordered, slots=cfg.slots, workspace=ws,
                                   w_dst=w_dst)
