@@ -557,7 +557,9 @@ def harness_bench(args, dev):
     for rate in SWEEP_PROBS:
         tr = trace(rate)
         # the product alone: the 128-step sequence captured in one graph
-        with torch.inference_mode(), prepare_sparse_modules(conn, 1):
+        # no_grad, not inference_mode: the generator state that graph capture
+        # registers must stay usable by the module's own capture below
+        with torch.no_grad(), prepare_sparse_modules(conn, 1):
             outs = []
 
             def seq():

@@ -61,9 +61,17 @@ struct Ctl {
     int bin_delay[64];   // ring age of delay bin j
     int bin_sub[64];     // virtual-row offset of delay bin j
     // step engine, ordered mode: spikes pushed into each ring slot (all
-    // lanes), and how the push of each step parity was delivered
+    // lanes, summed one launch after the slot is written), and how the push
+    // of the launch at clock c is delivered, at push_mode[c & 3] (decided
+    // one launch ahead)
     int slot_total[64];
-    int pull_mode[2];    // 1 = the push was a dense pull into the accumulator
+    int push_mode[4];    // 1 = a dense pull into the accumulator, 0 = term bins
+    // the lane bits of the slot written one launch earlier are transposed by
+    // the launch at clock c when lane_bits_mode[c & 3] (decided one launch
+    // ahead, on while activity nears the pull); lane_bits_run counts the
+    // consecutive launches, up to the latest decided, that transposed
+    int lane_bits_mode[4];
+    int lane_bits_run;
 };
 
 // Slots per destination of the ordered mode are held in registers when the

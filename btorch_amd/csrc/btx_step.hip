@@ -92,7 +92,7 @@ struct ReduceIn {
     const uint16_t* ev_d;
     uint16_t* ev_o;              // order of a bin's terms by neuron (bins past kBinCap)
     float* acc;                  // fast mode: [lanes][n] of the buffer being consumed
-    const int* pull_mode;        // ordered mode: 1 = the push was a dense pull ...
+    const int* pull_mode;        // ordered mode: 1 = the push was a dense pull (Ctl::push_mode) ...
     const float* pull_acc;       // ... whose sums are here, [lanes][n]
     PullArgs pull;
 };
@@ -143,9 +143,6 @@ struct StepArgs {
     int32_t* ring_idx;     // this step's ring slot, [lanes][n], or NULL
     int32_t* ring_cnt;     // this step's ring slot, [lanes][nchunks]
     uint64_t* ring_words;  // this step's ring slot, [lanes][nwords]
-    uint64_t* lane_bits;   // ordered mode: this step's slot of the lane-bit ring, [n][lw]
-    int* slot_total;       // ordered mode: spikes pushed into this step's slot
-    int lw;                // 64-lane words per neuron of the lane-bit ring
     float* rec_z;          // step t or NULL
     float* rec_v;
     float* rec_ref;
@@ -218,7 +215,9 @@ struct PushStep {
     // active incoming edges in the coalesced order into pull_acc
     uint64_t* lane_bits;         // [R][n][lw]: bit b of word (slot, j, b/64) = lane b pushed j
     int lw;
-    Ctl* ctl_rw;                 // the control block, written: slot totals, mode
+    const uint64_t* ring_words;  // [R][lanes][nwords]: the lane bits are transposed from these
+    int64_t nwords;
+    Ctl* ctl_rw;                 // the control block, written: slot totals, next push mode
     float terms_per_row;         // nnz / (n * csr_d)
     float pull_above;            // terms per bin above which the step pulls
     float* pull_acc;             // [lanes][n] of this step's buffer
@@ -624,24 +623,26 @@ __device__ float pull_sum(const PullArgs& a, int64_t lane, int64_t dst) {
 
 // The requests that open the reduction of a segment: the bin's term count
 // and its first kThreads terms (slots past the count hold stale terms and
-// are ignored).  Issued ahead of the state loads, so that waiting for them
-// does not wait for the state.
+// are ignored), and how the push was delivered.  Issued together, ahead of
+// the state loads, so that waiting for them does not wait for the state;
+// after a dense pull the bin head is stale and unused.
 struct RedHead {
     int c;
     int d, r;
     float w;
+    int pull;
 };
 
 __device__ __forceinline__ RedHead reduce_head(const ReduceIn& a, int64_t nchunks, int64_t lane,
                                                int64_t seg) {
-    RedHead h{0, 0, 0, 0.f};
+    RedHead h{0, 0, 0, 0.f, 0};
     if (!a.ordered) return h;
-    if (a.pull_mode != nullptr && ld_const(a.pull_mode) != 0) return h;
     const int64_t q0 = (lane * nchunks + seg) * (int64_t)kBinCapMax + threadIdx.x;
     h.d = a.ev_d[q0];
     h.r = a.ev_r[q0];
     h.w = a.ev_w[q0];
     h.c = ld_const(a.bin_cnt + lane * nchunks + seg);
+    h.pull = a.pull_mode != nullptr ? ld_const(a.pull_mode) : 0;
     return h;
 }
 
@@ -728,7 +729,7 @@ __device__ __forceinline__ void reduce_segment(const ReduceIn& a, RedLds& L, int
         }
         return;
     }
-    if (a.pull_mode != nullptr && ld_const(a.pull_mode) != 0) {
+    if (h.pull) {
         // the push was delivered as dense sums: the bins stayed empty
         if (n0 < n) {
             const float* q = a.pull_acc + lane * n + n0;
@@ -840,17 +841,26 @@ __device__ __forceinline__ int write_segment(uint32_t bits, int64_t seg_base,
     return total;
 }
 
-// The lane-bit ring of the dense pull: lane `lane` pushed neurons
-// n0 + k (bit k of `bits`) into the slot whose rows start at `slot_bits`.
-__device__ __forceinline__ void mark_lane_bits(uint64_t* __restrict__ slot_bits, int lw,
-                                               int64_t n0, int64_t lane, uint32_t bits) {
-    const int64_t g = lane >> 6;
-    const unsigned long long m = 1ull << (lane & 63);
-    while (bits) {
-        const int b = __ffs(bits) - 1;
-        atomicOr((unsigned long long*)(slot_bits + (n0 + b) * lw + g), m);
-        bits &= bits - 1;
+// The lane-bit ring of the dense pull, transposed from the ring's spike
+// words without atomics: one wavefront turns word `wi` of the 64 lanes of
+// group g (one load per lane) into the 64 neurons' lane words, bit b of
+// neuron 64*wi + k = bit k of lane 64*g + b, one ballot per neuron.  Every
+// lane of the wavefront calls it with the same (wi, g).
+__device__ __forceinline__ void transpose_lane_bits(const uint64_t* __restrict__ slot_words,
+                                                    uint64_t* __restrict__ slot_bits,
+                                                    int64_t lanes, int64_t nwords, int64_t n,
+                                                    int lw, int64_t wi, int g) {
+    const int l = (int)(threadIdx.x & 63);
+    const int64_t lane = 64 * (int64_t)g + l;
+    const uint64_t word = lane < lanes ? slot_words[lane * nwords + wi] : 0ull;
+    uint64_t mine = 0ull;
+#pragma unroll 8
+    for (int k = 0; k < 64; ++k) {
+        const uint64_t m = __ballot((word >> k) & 1ull);
+        if (l == k) mine = m;
     }
+    const int64_t j = 64 * wi + l;
+    if (j < n) slot_bits[j * lw + g] = mine;
 }
 
 // State of neurons n0 .. n0+3 of a lane (`room` of them exist): one 16-byte
@@ -894,6 +904,13 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
         if (a.prev) {
             float wz[4];
             reduce_segment(a.red, lds.red, a.n, a.nchunks, lane, seg, wz);
+        }
+        // and push nothing: an empty ring entry, so that the spike totals and
+        // the lane bits count only lanes with a trial
+        if (a.ring_idx != nullptr) {
+            const int64_t wi = (seg * kChunk >> 6) + threadIdx.x;
+            if (threadIdx.x < kChunk / 64 && wi < a.nwords) a.ring_words[lane * a.nwords + wi] = 0ull;
+            if (threadIdx.x == 0) a.ring_cnt[lane * a.nchunks + seg] = 0;
         }
         return;
     }
@@ -1087,10 +1104,8 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
             if (room < 4) pushed &= room > 0 ? (1u << room) - 1u : 0u;
         }
         if (lead) a.ring_words[lane * a.nwords + wi] = pw;
-        if (a.lane_bits != nullptr) mark_lane_bits(a.lane_bits, a.lw, n0, lane, pushed);
-        const int total = write_segment(pushed, seg_base, a.ring_idx + lane * a.n,
-                                        a.ring_cnt + lane * a.nchunks + seg);
-        if (a.slot_total != nullptr && threadIdx.x == 0 && total) atomicAdd(a.slot_total, total);
+        write_segment(pushed, seg_base, a.ring_idx + lane * a.n,
+                      a.ring_cnt + lane * a.nchunks + seg);
     }
 }
 
@@ -1103,33 +1118,60 @@ __device__ __forceinline__ int ring_slot(int64_t clock, int delay, int R) {
     return (int)(s < 0 ? s + R : s);
 }
 
-// Ordered mode: whether the push of this step is delivered as a dense pull.
-// Every P workgroup decides alike, from the spikes pushed into the ring
-// slots the step reads (counted in earlier launches): the pull is taken
-// when the push would bin more than pull_above terms per (lane, segment)
-// on average.
-__device__ __forceinline__ bool choose_pull(const PushStep& a) {
-    if (a.lane_bits == nullptr) return false;
+// Ordered mode, one workgroup per launch: the spike total of the ring slot
+// written one launch earlier (complete now; summed from its segment counts,
+// no atomics), then how the next launch delivers its push.  That push reads
+// the slots at ages dl - 1 from this launch's clock for its bins' delays dl:
+// the totals of ages >= 1 are known, and a slot younger than that (written
+// in this launch, or the next) is taken to hold as many spikes as the
+// youngest known one.  The pull is taken when the push would bin more than
+// pull_above terms per (lane, segment) on average and the lane bits it
+// reads (slots of ages 2 .. ring_d - 1, transposed by the ring_d - 2
+// launches before it) are current; the lane bits are kept while the push
+// would bin more than an eighth of that.  Either way the sums are the same
+// bits, so an estimate only ever costs time.
+__device__ void control_role(const PushStep& a) {
+    const int tid = threadIdx.x;
+    const int s1 = ring_slot(a.clock, 1, a.R);
+    const int32_t* cnt = a.ring_cnt + (int64_t)s1 * a.lanes * a.nchunks;
+    const int64_t m = a.lanes * a.nchunks;
+    int sum = 0;
+    for (int64_t i = tid; i < m; i += kThreads) sum += cnt[i];
+    int total;
+    block_excl_scan<kThreads / 64>(sum, total);
+    if (tid != 0) return;
+    a.ctl_rw->slot_total[s1] = total;
     float rows = 0.f;
     for (int j = 0; j < a.nbins; ++j) {
         const int dl = a.affine ? a.delay0 + j : a.ctl->bin_delay[j];
-        rows += (float)a.ctl_rw->slot_total[ring_slot(a.clock, dl, a.R)];
+        rows += dl >= 3 ? (float)a.ctl_rw->slot_total[ring_slot(a.clock, dl - 1, a.R)]
+                        : (float)total;
     }
-    return rows * a.terms_per_row > a.pull_above * (float)(a.lanes * a.nchunks);
+    const float terms = rows * a.terms_per_row;
+    const float above = a.pull_above * (float)(a.lanes * a.nchunks);
+    int run = a.ctl_rw->lane_bits_run;
+    run = a.ctl_rw->lane_bits_mode[a.clock & 3] ? (run < (1 << 20) ? run + 1 : run) : 0;
+    a.ctl_rw->lane_bits_run = run;
+    const bool pull = terms > above && run >= a.R;
+    a.ctl_rw->push_mode[(a.clock + 1) & 3] = pull ? 1 : 0;
+    a.ctl_rw->lane_bits_mode[(a.clock + 1) & 3] = (pull || terms > 0.125f * above) ? 1 : 0;
 }
 
 // The dense pull of one step: workgroup `wid` of `nwg` sums, for a range of
 // destinations and every lane, the weights of the active incoming edges in
 // the coalesced (dst, virtual source) order from +0.0f -- the terms and the
 // order of the ordered push, so the same bits.  Virtual row r = src*csr_d +
-// sub is active for lane b when bit b of the lane-bit ring word of src in the
-// slot that sub's age points at is set.  Groups of Gp threads (one per lane
-// of a 64-lane group) walk one destination's edges together.
+// sub is active for lane b when src was pushed by lane b into the slot that
+// sub's age points at: bit b of the lane-bit word of src for slots of age
+// >= 2 (transposed in an earlier launch), bit src%64 of lane b's spike word
+// for the younger ones.  Groups of Gp threads (one per lane of a 64-lane
+// group) walk one destination's edges together.
 __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nwg) {
     const int tid = threadIdx.x;
     for (int k = tid; k < a.csr_d; k += kThreads) {
         const int age = a.ctl->age_of_sub[k];
         L.slot[k] = age < 0 ? -1 : ring_slot(a.clock, age, a.R);
+        L.sub[k] = age >= 0 && age < 2;  // read from the spike words
     }
     __syncthreads();
     const int64_t per = (a.n + nwg - 1) / nwg;
@@ -1142,6 +1184,9 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
         const int gl = tid % Gp;
         const int64_t lane = 64 * (int64_t)g + gl;
         const uint64_t* bits = a.lane_bits + g;
+        // the lane's spike words (a lane past the last reads the last one's)
+        const uint64_t* lwords = a.ring_words + (lane < a.lanes ? lane : a.lanes - 1) * a.nwords;
+        const int64_t slot_words = a.lanes * a.nwords;
         for (int64_t d = d0 + tid / Gp; d < d1; d += kThreads / Gp) {
             float acc = 0.f;
             int64_t e = a.csc_ptr[d];
@@ -1157,16 +1202,22 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
                     r[u] = in ? a.csc_src[e + u] : 0;
                     w[u] = in ? a.csc_w[e + u] : 0.f;
                 }
+                int sh[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int src = r[u] / a.csr_d;
-                    const int slot = L.slot[r[u] - src * a.csr_d];
+                    const int sub = r[u] - src * a.csr_d;
+                    const int slot = L.slot[sub];
+                    const bool young = L.sub[sub] != 0;
+                    sh[u] = young ? (src & 63) : gl;
                     wb[u] = (e + u < e1 && slot >= 0)
-                                ? bits[((int64_t)slot * a.n + src) * a.lw] : 0ull;
+                                ? (young ? lwords[slot * slot_words + (src >> 6)]
+                                         : bits[((int64_t)slot * a.n + src) * a.lw])
+                                : 0ull;
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if ((wb[u] >> gl) & 1ull) acc = acc + w[u];
+                    if ((wb[u] >> sh[u]) & 1ull) acc = acc + w[u];
             }
             if (gl < lanes_g) {
                 const bool live = a.lane_t_end == nullptr ||
@@ -1180,20 +1231,20 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
 template <int G, bool ORDERED>
 __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) {
     if (ORDERED && a.lane_bits != nullptr) {
-        // the ring slot the next step writes starts empty: its lane bits
-        // (spread over the workgroups) and its spike count; no workgroup of
-        // this launch reads that slot (its age would be ring_d)
+        // the lane bits of the slot written one launch earlier, one
+        // (word, 64-lane group) per wavefront of the P workgroups (no
+        // workgroup of this launch reads them: their age is 1), then the push
+        // as the previous launch decided it
         const int64_t np = (a.nchunks + a.cpb - 1) / a.cpb;
         const int64_t nwg = np * a.lanes, wid = lane * np + px;
-        const int nxt = ring_slot(a.clock + 1, 0, a.R);
-        uint64_t* nb = a.lane_bits + (int64_t)nxt * a.n * a.lw;
-        for (int64_t i = wid * kThreads + threadIdx.x; i < a.n * a.lw; i += nwg * kThreads)
-            nb[i] = 0ull;
-        const bool pull = choose_pull(a);
-        if (wid == 0 && threadIdx.x == 0) {
-            a.ctl_rw->slot_total[nxt] = 0;
-            a.ctl_rw->pull_mode[a.t & 1] = pull ? 1 : 0;
-        }
+        const int pull = ld_const(&a.ctl->push_mode[a.clock & 3]);
+        const int s1 = ring_slot(a.clock, 1, a.R);
+        const int64_t items = ld_const(&a.ctl->lane_bits_mode[a.clock & 3]) ? a.nwords * a.lw : 0;
+        constexpr int kWaves = kThreads / 64;
+        for (int64_t it = wid * kWaves + (threadIdx.x >> 6); it < items; it += nwg * kWaves)
+            transpose_lane_bits(a.ring_words + (int64_t)s1 * a.lanes * a.nwords,
+                                a.lane_bits + (int64_t)s1 * a.n * a.lw, a.lanes, a.nwords, a.n,
+                                a.lw, it / a.lw, (int)(it % a.lw));
         if (pull) {
             pull_role(a, L, wid, nwg);
             return;
@@ -1313,16 +1364,24 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 // Kernels
 // ---------------------------------------------------------------------------
 
-// Workgroups [0, np) of a row play P (the longer chain, dispatched first),
-// [np, gridDim.x) play F.
+// Workgroups [0, nc) of a row are the control column (nc = 0 or 1; lane 0's
+// sums the slot totals and decides the next push, the others end at once),
+// [nc, nc + np) play P (the longer chain, dispatched first), the rest F.
 template <int G, bool ORDERED, int KIND>
-__global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArgs a, PushStep p, int np) {
+__global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArgs a, PushStep p,
+                                                                          int np, int nc) {
     // KIND: kLif, kGlif3, kTwoComp or kMixed, with kSynX for the alpha and
     // dual-exponential synapses
     __shared__ StepLds lds;
     const int64_t lane = blockIdx.y;
-    if ((int)blockIdx.x < np) p_role<G, ORDERED>(p, lds.push, blockIdx.x, lane);
-    else f_role<KIND>(a, lds, (int64_t)blockIdx.x - np, lane);
+    const int x = (int)blockIdx.x - nc;
+    if (x < 0) {
+        if (ORDERED && lane == 0) control_role(p);
+    } else if (x < np) {
+        p_role<G, ORDERED>(p, lds.push, x, lane);
+    } else {
+        f_role<KIND>(a, lds, (int64_t)x - np, lane);
+    }
 }
 
 // FIN: the last push reduced into the synapse (and recorded: psc in `rec`,
@@ -1368,9 +1427,7 @@ __global__ __launch_bounds__(kThreads) void k_ring_from_words(const uint64_t* __
                                                               int64_t n, int64_t nwords,
                                                               int64_t lanes, int64_t nchunks,
                                                               int32_t* __restrict__ ring_idx,
-                                                              int32_t* __restrict__ ring_cnt,
-                                                              uint64_t* __restrict__ lane_bits,
-                                                              int lw, Ctl* __restrict__ ctl) {
+                                                              int32_t* __restrict__ ring_cnt) {
     const int64_t seg = blockIdx.x, lane = blockIdx.y, slot = blockIdx.z;
     const int64_t seg_base = seg * kChunk;
     const int64_t n0 = seg_base + (int64_t)threadIdx.x * 4;
@@ -1382,18 +1439,37 @@ __global__ __launch_bounds__(kThreads) void k_ring_from_words(const uint64_t* __
         const int64_t room = n - n0;  // bits past n never count
         if (room < 4) nib &= (1u << room) - 1u;
     }
-    if (lane_bits != nullptr) mark_lane_bits(lane_bits + slot * n * lw, lw, n0, lane, nib);
-    const int total = write_segment(nib, seg_base, ring_idx + (slot * lanes + lane) * n,
-                                    ring_cnt + (slot * lanes + lane) * nchunks + seg);
-    if (lane_bits != nullptr && threadIdx.x == 0 && total) atomicAdd(&ctl->slot_total[slot], total);
+    write_segment(nib, seg_base, ring_idx + (slot * lanes + lane) * n,
+                  ring_cnt + (slot * lanes + lane) * nchunks + seg);
 }
 
-// Empty lane-bit ring and slot counts (before a rebuild of the ring).
-__global__ __launch_bounds__(256) void k_pull_clear(uint64_t* __restrict__ lane_bits,
-                                                    int64_t words, Ctl* __restrict__ ctl) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < words) lane_bits[i] = 0ull;
-    if (i < 64) ctl->slot_total[i] = 0;
+// The lane bits of every ring slot, transposed from its spike words: one
+// wavefront per (word, 64-lane group, slot).
+__global__ __launch_bounds__(64) void k_ring_lane_bits(const uint64_t* __restrict__ words,
+                                                       uint64_t* __restrict__ lane_bits,
+                                                       int64_t lanes, int64_t nwords, int64_t n,
+                                                       int lw) {
+    const int64_t wi = blockIdx.x, slot = blockIdx.z;
+    transpose_lane_bits(words + slot * lanes * nwords, lane_bits + slot * n * lw, lanes, nwords,
+                        n, lw, wi, (int)blockIdx.y);
+}
+
+// The spike total of every ring slot from its segment counts (one workgroup
+// per slot), every push mode back to the term bins, and the lane bits (just
+// transposed for every slot) current and kept.
+__global__ __launch_bounds__(kThreads) void k_ring_totals(const int32_t* __restrict__ ring_cnt,
+                                                          int64_t m, Ctl* __restrict__ ctl) {
+    const int64_t slot = blockIdx.x;
+    int sum = 0;
+    for (int64_t i = threadIdx.x; i < m; i += kThreads) sum += ring_cnt[slot * m + i];
+    int total;
+    block_excl_scan<kThreads / 64>(sum, total);
+    if (threadIdx.x == 0) ctl->slot_total[slot] = total;
+    if (slot == 0 && threadIdx.x < 4) {
+        ctl->push_mode[threadIdx.x] = 0;
+        ctl->lane_bits_mode[threadIdx.x] = 1;
+    }
+    if (slot == 0 && threadIdx.x == 0) ctl->lane_bits_run = 1 << 20;
 }
 
 // Standalone exponential-synapse decay psc += k*((-psc)/tau_s) (ode.py:50).
@@ -1625,7 +1701,7 @@ static int vec4_ok(const btx_rsnn_plan* p, const float* x) {
 template <int KIND>
 static void launch_neuron_only(const StepArgs& a, dim3 grid, hipStream_t s) {
     PushStep none{};
-    hipLaunchKernelGGL((k_step<16, true, KIND>), grid, dim3(kThreads), 0, s, a, none, 0);
+    hipLaunchKernelGGL((k_step<16, true, KIND>), grid, dim3(kThreads), 0, s, a, none, 0, 0);
 }
 
 BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float* v_pre,
@@ -1751,34 +1827,34 @@ struct KernelTimer {
 static int rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s, KernelTimer& kt);
 
 template <bool ORDERED, int KIND>
-static void launch_step_g(const StepArgs& a, const PushStep& ps, int np, int group, dim3 grid,
-                          hipStream_t s) {
+static void launch_step_g(const StepArgs& a, const PushStep& ps, int np, int nc, int group,
+                          dim3 grid, hipStream_t s) {
     switch (group) {
-        case 8: hipLaunchKernelGGL((k_step<8, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np); break;
-        case 16: hipLaunchKernelGGL((k_step<16, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np); break;
-        case 32: hipLaunchKernelGGL((k_step<32, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np); break;
-        default: hipLaunchKernelGGL((k_step<64, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np); break;
+        case 8: hipLaunchKernelGGL((k_step<8, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np, nc); break;
+        case 16: hipLaunchKernelGGL((k_step<16, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np, nc); break;
+        case 32: hipLaunchKernelGGL((k_step<32, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np, nc); break;
+        default: hipLaunchKernelGGL((k_step<64, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np, nc); break;
     }
 }
 
 template <bool ORDERED>
-static void launch_step_k(const StepArgs& a, const PushStep& ps, int np, int group, dim3 grid,
-                          hipStream_t s) {
+static void launch_step_k(const StepArgs& a, const PushStep& ps, int np, int nc, int group,
+                          dim3 grid, hipStream_t s) {
     if (a.syn != BTX_SYN_EXP) {
-        if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3 | kSynX>(a, ps, np, group, grid, s);
-        else launch_step_g<ORDERED, kLif | kSynX>(a, ps, np, group, grid, s);
+        if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3 | kSynX>(a, ps, np, nc, group, grid, s);
+        else launch_step_g<ORDERED, kLif | kSynX>(a, ps, np, nc, group, grid, s);
         return;
     }
-    if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3>(a, ps, np, group, grid, s);
-    else if (a.kind == kTwoComp) launch_step_g<ORDERED, kTwoComp>(a, ps, np, group, grid, s);
-    else if (a.kind == kMixed) launch_step_g<ORDERED, kMixed>(a, ps, np, group, grid, s);
-    else launch_step_g<ORDERED, kLif>(a, ps, np, group, grid, s);
+    if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3>(a, ps, np, nc, group, grid, s);
+    else if (a.kind == kTwoComp) launch_step_g<ORDERED, kTwoComp>(a, ps, np, nc, group, grid, s);
+    else if (a.kind == kMixed) launch_step_g<ORDERED, kMixed>(a, ps, np, nc, group, grid, s);
+    else launch_step_g<ORDERED, kLif>(a, ps, np, nc, group, grid, s);
 }
 
-static void launch_step(int ordered, const StepArgs& a, const PushStep& ps, int np, int group,
-                        dim3 grid, hipStream_t s) {
-    if (ordered) launch_step_k<true>(a, ps, np, group, grid, s);
-    else launch_step_k<false>(a, ps, np, group, grid, s);
+static void launch_step(int ordered, const StepArgs& a, const PushStep& ps, int np, int nc,
+                        int group, dim3 grid, hipStream_t s) {
+    if (ordered) launch_step_k<true>(a, ps, np, nc, group, grid, s);
+    else launch_step_k<false>(a, ps, np, nc, group, grid, s);
 }
 
 }  // namespace btx
@@ -1880,20 +1956,22 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     uint64_t* lane_bits = p->ordered ? p->ring_lane_bits : nullptr;
     const int lw = (int)((lanes + 63) / 64);
     if (p->ring_rebuild) {
-        if (lane_bits != nullptr) {
-            const int64_t words = (int64_t)R * n * lw;
-            hipLaunchKernelGGL(k_pull_clear, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
-                               s, lane_bits, words, ctl);
-        }
         hipLaunchKernelGGL(k_ring_from_words,
                            dim3((unsigned)nchunks, (unsigned)lanes, (unsigned)R),
                            dim3(kThreads), 0, s, p->ring_words, n, nwords, lanes, nchunks,
-                           p->ring_idx, p->ring_cnt, lane_bits, lw, ctl);
+                           p->ring_idx, p->ring_cnt);
+        if (lane_bits != nullptr) {
+            hipLaunchKernelGGL(k_ring_lane_bits, dim3((unsigned)nwords, (unsigned)lw, (unsigned)R),
+                               dim3(64), 0, s, p->ring_words, lane_bits, lanes, nwords, n, lw);
+            hipLaunchKernelGGL(k_ring_totals, dim3((unsigned)R), dim3(kThreads), 0, s, p->ring_cnt,
+                               lanes * nchunks, ctl);
+        }
     }
 
     const int cap = p->bin_cap > 0 ? p->bin_cap : kBinCapMax;
-    // buffer b of the term bins / accumulators
-    auto bins_of = [&](int b, ReduceIn& r) {
+    // buffer b of the term bins / accumulators, for the push of the launch
+    // at `clock`
+    auto bins_of = [&](int b, int64_t clock, ReduceIn& r) {
         r.ordered = p->ordered;
         r.cap = cap;
         r.sort_max = p->slots;
@@ -1903,7 +1981,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
             r.ev_w = (const float*)(ws + L.ev_w) + b * L.bins * kBinCapMax;
             r.ev_d = (const uint16_t*)(ws + L.ev_d) + b * L.bins * kBinCapMax;
             r.ev_o = (uint16_t*)(ws + L.ev_o) + b * L.bins * kBinCapMax;
-            r.pull_mode = p->ring_lane_bits ? &ctl->pull_mode[b] : nullptr;
+            r.pull_mode = lane_bits ? &ctl->push_mode[clock & 3] : nullptr;
             r.pull_acc = (float*)(ws + L.acc) + b * L.m;
         } else {
             r.acc = (float*)(ws + L.acc) + b * L.m;
@@ -1956,6 +2034,8 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ps.in_t0 = p->in_t0;
     ps.lane_bits = lane_bits;
     ps.lw = lw;
+    ps.ring_words = p->ring_words;
+    ps.nwords = nwords;
     ps.ctl_rw = ctl;
     ps.terms_per_row = (float)((double)p->nnz / (double)(n * p->csr_d));
     ps.pull_above = 0.5f * (float)cap;
@@ -1980,12 +2060,14 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     // one launch per step when no bin reads the step's own spikes
     const bool fused = min_delay >= 1;
     const int nf = (int)nchunks;
+    // the control column: the slot totals and the next launch's push mode
+    const int nc = lane_bits != nullptr ? 1 : 0;
 
     const int64_t rec_step = m;
     for (int64_t t = 0; t < n_steps; ++t) {
         a.prev = t > 0;
         if (a.prev) {
-            bins_of((int)((t - 1) & 1), a.red);
+            bins_of((int)((t - 1) & 1), p->ring_cursor + t - 1, a.red);
             a.red.pull.clock = p->ring_cursor + t - 1;
             a.red.pull.slot0 = (int)mod(a.red.pull.clock, R);
         }
@@ -2001,9 +2083,6 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         a.ring_idx = p->ring_idx + slot * lanes * n;
         a.ring_cnt = p->ring_cnt + slot * lanes * nchunks;
         a.ring_words = p->ring_words + slot * lanes * nwords;
-        a.lane_bits = lane_bits ? lane_bits + slot * n * lw : nullptr;
-        a.slot_total = lane_bits ? &ctl->slot_total[slot] : nullptr;
-        a.lw = lw;
         a.rec_z = p->rec_z ? p->rec_z + t * rec_step : nullptr;
         a.rec_v = p->rec_v ? p->rec_v + t * rec_step : nullptr;
         a.rec_ref = p->rec_refractory ? p->rec_refractory + t * rec_step : nullptr;
@@ -2021,7 +2100,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         ps.clock = p->ring_cursor + t;
         ps.t = t;
         ReduceIn cur{};
-        bins_of((int)(t & 1), cur);
+        bins_of((int)(t & 1), p->ring_cursor + t, cur);
         ps.bin_cnt = cur.bin_cnt;
         ps.ev_r = (int32_t*)cur.ev_r;
         ps.ev_w = (float*)cur.ev_w;
@@ -2030,21 +2109,21 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         ps.pull_acc = cur.pull_acc ? (float*)cur.pull_acc : nullptr;
         if (fused) {
             kt.begin(0);
-            const dim3 grid((unsigned)(nf + npb), (unsigned)lanes);
-            launch_step(p->ordered, a, ps, npb, group, grid, s);
+            const dim3 grid((unsigned)(nc + nf + npb), (unsigned)lanes);
+            launch_step(p->ordered, a, ps, npb, nc, group, grid, s);
             kt.end(0);
         } else {
             kt.begin(0);
             const dim3 fgrid((unsigned)nf, (unsigned)lanes);
-            launch_step(p->ordered, a, ps, 0, group, fgrid, s);
+            launch_step(p->ordered, a, ps, 0, 0, group, fgrid, s);
             kt.end(0);
             kt.begin(2);
-            const dim3 pgrid((unsigned)npb, (unsigned)lanes);
-            launch_step(p->ordered, a, ps, npb, group, pgrid, s);
+            const dim3 pgrid((unsigned)(nc + npb), (unsigned)lanes);
+            launch_step(p->ordered, a, ps, npb, nc, group, pgrid, s);
             kt.end(2);
         }
     }
-    bins_of((int)((n_steps - 1) & 1), a.red);
+    bins_of((int)((n_steps - 1) & 1), p->ring_cursor + n_steps - 1, a.red);
     a.rec_sh1_prev = p->rec_syn_h1 ? p->rec_syn_h1 + (n_steps - 1) * rec_step : nullptr;
     a.rec_sh2_prev = p->rec_syn_h2 ? p->rec_syn_h2 + (n_steps - 1) * rec_step : nullptr;
     a.red.pull.clock = p->ring_cursor + n_steps - 1;

@@ -26,6 +26,7 @@ delivers at the end of step t+d.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass, field
 
 import torch
@@ -255,7 +256,7 @@ class _RingState:
                                                                               dev)
         self.R = ring_d + 1
         self.cursor = ring_d          # device slot of the next run's first step
-        self.token = None             # module history the ring holds, or None
+        self.token = None             # module history the ring holds (Engine._hist_token), or None
         self.fresh = True             # lists not built yet
 
 
@@ -506,9 +507,23 @@ class Engine:
             self._rings = {key: ring}
         return ring
 
-    def _hist_token(self, hist):
+    @staticmethod
+    def _hist_token(hist):
+        """Identifies the module's history memory as it is now: the buffer
+        itself (by weak reference: an id can be reused once it is freed), its
+        version counter and the cursor."""
         buf = hist._buffers.get("history")
-        return None if buf is None else (id(buf), buf._version, hist.cursor)
+        return None if buf is None else (weakref.ref(buf), buf._version, hist.cursor)
+
+    @staticmethod
+    def _holds(token, hist) -> bool:
+        """Whether `token` still names the module's current history memory."""
+        if token is None:
+            return False
+        buf = hist._buffers.get("history")
+        ref, version, cursor = token
+        return buf is not None and ref() is buf and buf._version == version and \
+            hist.cursor == cursor
 
     def _prepare_ring(self, ring: _RingState, lanes, n, dev, keep) -> int:
         """Make the ring hold the module's history; returns ring_rebuild."""
@@ -524,10 +539,14 @@ class Engine:
         if fn is not None and getattr(fn, "ring", None) is ring:
             # the module's history is the state this ring left: continue
             hist.discard_writeback()
+            ring.token = None
             return 0
         if fn is not None:
             hist._sync_memories()  # another engine's deferred state
-        if not ring.fresh and ring.token is not None and ring.token == self._hist_token(hist):
+        held = not ring.fresh and self._holds(ring.token, hist)
+        # the run moves the ring on: it no longer holds this memory
+        ring.token = None
+        if held:
             return 0
         # rebuild from the dense memory: device slot ring_d - 1 - d holds
         # delay d; the run starts at slot ring_d

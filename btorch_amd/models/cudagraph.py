@@ -229,7 +229,10 @@ class GraphRunner:
             return
         if fn is not None:
             hist._sync_memories()
-        if ring.token is not None and ring.token == eng._hist_token(hist):
+        held = eng._holds(ring.token, hist)
+        # the replay moves the ring on: it no longer holds this memory
+        ring.token = None
+        if held:
             return
         lanes, n, ring_d = entry.lanes, sp.neuron.size, sp.ring_d
         h = hist.history
@@ -243,7 +246,7 @@ class GraphRunner:
         words = pack_rows(h.reshape(lanes * ring_d, n)).reshape(lanes, ring_d, -1)
         ring.words[:ring_d] = words.index_select(1, order).transpose(0, 1)
         ring.words[ring_d:] = 0
-        ring.token = eng._hist_token(hist)
+        ring.token = None
 
 
 __all__ = ["GraphRunner"]

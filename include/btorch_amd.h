@@ -441,8 +441,12 @@ typedef struct btx_rsnn_plan {
      * it, a step whose push would bin more than bin_cap/2 terms per (lane,
      * segment) on average is delivered as a dense pull instead: every
      * destination sums its active incoming edges in the coalesced order
-     * (the same terms and order, so the same bits).  Zero it with the ring;
-     * a caller that empties lanes of the ring clears their bits too. */
+     * (the same terms and order, so the same bits).  The engine keeps it:
+     * each step transposes the slot written one step earlier from
+     * ring_words, and ring_rebuild rebuilds every slot; a caller that empties
+     * lanes of the ring clears their bits too.  The decision is made one
+     * step ahead from the ring's spike totals (the newest slot estimated by
+     * the one before). */
     uint64_t* ring_lane_bits;
 } btx_rsnn_plan;
 

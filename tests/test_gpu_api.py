@@ -403,7 +403,7 @@ def test_engine_high_activity_large_bins_and_dense_pull(bin_cap, lanes, expect):
     with environ.context(dt=0.1), torch.no_grad():
         z, st = model(x.to(DEV))
     ws = next(iter(model.fused_engine()._ws.values())).view(torch.int32)
-    pull_flags = ws[256:258].cpu().tolist()   # Ctl.pull_mode of the last two steps
+    pull_flags = ws[256:260].cpu().tolist()   # Ctl.push_mode of the last launches
     net = rsnn.Net(n, cexp, rsnn.LIFParams.make(n, tau_ref=2.0), torch.tensor(5.0), 0.1,
                    delay_mode="hetero", max_delay_steps=D, circular=True)
     ref = rsnn.simulate(net, x)
@@ -411,9 +411,9 @@ def test_engine_high_activity_large_bins_and_dense_pull(bin_cap, lanes, expect):
     print(f"\nbin_cap {bin_cap}, {lanes} lanes: terms per bin max {terms.max():.0f} mean "
           f"{terms[10:].mean():.0f}; pull flags {pull_flags}", flush=True)
     if expect == "big":
-        assert 1024 < terms.max() <= 8192 and pull_flags == [0, 0]
+        assert 1024 < terms.max() <= 8192 and pull_flags == [0, 0, 0, 0]
     else:
-        assert pull_flags == [1, 1]
+        assert pull_flags == [1, 1, 1, 1]
     assert bits_equal(z, ref["z"])
     assert bits_equal(st["neuron.v"], ref["v"]) and bits_equal(st["synapse.psc"], ref["psc"])
     assert bits_equal(model.synapse.history.history.reshape(lanes, D, n), ref["final"]["history"])

@@ -451,6 +451,35 @@ def test_spike_history_deferred_writeback_semantics():
     assert float(h.history.sum()) == 0 and len(calls) == 3
 
 
+def test_ring_history_token_names_only_the_live_memory():
+    # the engine reuses its device ring only while the module's history is
+    # the very memory the ring was written back to: a replaced buffer (even
+    # one that reuses a freed object's id), an in-place change or another
+    # cursor all count as a different history
+    import gc
+
+    from btorch_amd.models import SpikeHistory
+    from btorch_amd.models._engine import Engine
+    h = SpikeHistory(5, max_delay_steps=3, use_circular_buffer=False)
+    h.init_state(batch_size=2, persistent=True)
+    tok = Engine._hist_token(h)
+    assert Engine._holds(tok, h) and not Engine._holds(None, h)
+    h.history.add_(0.0)
+    assert not Engine._holds(tok, h)
+    tok = Engine._hist_token(h)
+    ids = set()
+    for _ in range(20):  # fresh buffers, some of them at a freed buffer's id
+        h.history = torch.zeros(2, 3, 5)
+        ids.add(id(h._buffers["history"]))
+        assert not Engine._holds(tok, h)
+        gc.collect()
+    c = SpikeHistory(5, max_delay_steps=3)
+    c.init_state(batch_size=2, persistent=True)
+    tok = Engine._hist_token(c)
+    c.set_cursor(1)
+    assert not Engine._holds(tok, c)
+
+
 def test_mixed_plan_lays_parameters_out_per_neuron():
     import ctypes
     from btorch_amd import _lib
