@@ -66,10 +66,10 @@ struct Ctl {
     // one launch ahead)
     int slot_total[64];
     int push_mode[4];    // 1 = a dense pull into the accumulator, 0 = term bins
-    // the lane bits of the slot written one launch earlier are transposed by
-    // the launch at clock c when lane_bits_mode[c & 3] (decided one launch
-    // ahead, on while activity nears the pull); lane_bits_run counts the
-    // consecutive launches, up to the latest decided, that transposed
+    // the launch at clock c marks the lane bits of the slot it writes when
+    // lane_bits_mode[c & 3] (decided one launch ahead, on while activity
+    // nears the pull); lane_bits_run counts the consecutive launches, up to
+    // the latest decided, that marked
     int lane_bits_mode[4];
     int lane_bits_run;
 };

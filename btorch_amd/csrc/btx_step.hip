@@ -143,6 +143,9 @@ struct StepArgs {
     int32_t* ring_idx;     // this step's ring slot, [lanes][n], or NULL
     int32_t* ring_cnt;     // this step's ring slot, [lanes][nchunks]
     uint64_t* ring_words;  // this step's ring slot, [lanes][nwords]
+    uint64_t* lane_bits;   // ordered mode: this step's slot of the lane-bit ring, [n][lw],
+    int lw;                // marked when *lane_bits_on (set one launch ahead)
+    const int* lane_bits_on;
     float* rec_z;          // step t or NULL
     float* rec_v;
     float* rec_ref;
@@ -215,8 +218,6 @@ struct PushStep {
     // active incoming edges in the coalesced order into pull_acc
     uint64_t* lane_bits;         // [R][n][lw]: bit b of word (slot, j, b/64) = lane b pushed j
     int lw;
-    const uint64_t* ring_words;  // [R][lanes][nwords]: the lane bits are transposed from these
-    int64_t nwords;
     Ctl* ctl_rw;                 // the control block, written: slot totals, next push mode
     float terms_per_row;         // nnz / (n * csr_d)
     float pull_above;            // terms per bin above which the step pulls
@@ -841,8 +842,22 @@ __device__ __forceinline__ int write_segment(uint32_t bits, int64_t seg_base,
     return total;
 }
 
+// The lane-bit ring of the dense pull: lane `lane` pushed neurons n0 + k
+// (bit k of `bits`) into the slot whose rows start at `slot_bits` (zeroed one
+// launch earlier).
+__device__ __forceinline__ void mark_lane_bits(uint64_t* __restrict__ slot_bits, int lw,
+                                               int64_t n0, int64_t lane, uint32_t bits) {
+    const int64_t g = lane >> 6;
+    const unsigned long long m = 1ull << (lane & 63);
+    while (bits) {
+        const int b = __ffs(bits) - 1;
+        atomicOr((unsigned long long*)(slot_bits + (n0 + b) * lw + g), m);
+        bits &= bits - 1;
+    }
+}
+
 // The lane-bit ring of the dense pull, transposed from the ring's spike
-// words without atomics: one wavefront turns word `wi` of the 64 lanes of
+// words without atomics (when the ring is rebuilt): one wavefront turns word `wi` of the 64 lanes of
 // group g (one load per lane) into the 64 neurons' lane words, bit b of
 // neuron 64*wi + k = bit k of lane 64*g + b, one ballot per neuron.  Every
 // lane of the wavefront calls it with the same (wi, g).
@@ -935,6 +950,7 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
     q.tsyn = a.psc ? ld_const(a.tsyn) : 1.f;
     q.ksyn = a.psc ? ld_const(a.ksyn) : 0.f;
     const DriveArgs dargs = drive_args(a, lane);
+    const bool mark = a.lane_bits != nullptr && ld_const(a.lane_bits_on) != 0;
     const int64_t safe = lane * a.n;
     float v[4] = {0.f, 0.f, 0.f, 0.f}, p[4] = {0.f, 0.f, 0.f, 0.f};
     float r[4] = {0.f, 0.f, 0.f, 0.f}, x[4] = {0.f, 0.f, 0.f, 0.f};
@@ -1104,6 +1120,7 @@ __device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lan
             if (room < 4) pushed &= room > 0 ? (1u << room) - 1u : 0u;
         }
         if (lead) a.ring_words[lane * a.nwords + wi] = pw;
+        if (mark) mark_lane_bits(a.lane_bits, a.lw, n0, lane, pushed);
         write_segment(pushed, seg_base, a.ring_idx + lane * a.n,
                       a.ring_cnt + lane * a.nchunks + seg);
     }
@@ -1125,11 +1142,11 @@ __device__ __forceinline__ int ring_slot(int64_t clock, int delay, int R) {
 // the totals of ages >= 1 are known, and a slot younger than that (written
 // in this launch, or the next) is taken to hold as many spikes as the
 // youngest known one.  The pull is taken when the push would bin more than
-// pull_above terms per (lane, segment) on average and the lane bits it
-// reads (slots of ages 2 .. ring_d - 1, transposed by the ring_d - 2
-// launches before it) are current; the lane bits are kept while the push
-// would bin more than an eighth of that.  Either way the sums are the same
-// bits, so an estimate only ever costs time.
+// pull_above terms per (lane, segment) on average and every slot it reads
+// (ages up to ring_d - 1) had its lane bits marked when it was written; the
+// marking is kept on while the push would bin more than an eighth of that.
+// Either way the sums are the same bits, so an estimate only ever costs
+// time.
 __device__ void control_role(const PushStep& a) {
     const int tid = threadIdx.x;
     const int s1 = ring_slot(a.clock, 1, a.R);
@@ -1161,17 +1178,15 @@ __device__ void control_role(const PushStep& a) {
 // destinations and every lane, the weights of the active incoming edges in
 // the coalesced (dst, virtual source) order from +0.0f -- the terms and the
 // order of the ordered push, so the same bits.  Virtual row r = src*csr_d +
-// sub is active for lane b when src was pushed by lane b into the slot that
-// sub's age points at: bit b of the lane-bit word of src for slots of age
-// >= 2 (transposed in an earlier launch), bit src%64 of lane b's spike word
-// for the younger ones.  Groups of Gp threads (one per lane of a 64-lane
-// group) walk one destination's edges together.
+// sub is active for lane b when bit b of the lane-bit word of src in the
+// slot that sub's age points at is set (every slot the step reads was marked
+// when it was written: the pull is only taken then).  Groups of Gp threads
+// (one per lane of a 64-lane group) walk one destination's edges together.
 __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nwg) {
     const int tid = threadIdx.x;
     for (int k = tid; k < a.csr_d; k += kThreads) {
         const int age = a.ctl->age_of_sub[k];
         L.slot[k] = age < 0 ? -1 : ring_slot(a.clock, age, a.R);
-        L.sub[k] = age >= 0 && age < 2;  // read from the spike words
     }
     __syncthreads();
     const int64_t per = (a.n + nwg - 1) / nwg;
@@ -1184,9 +1199,6 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
         const int gl = tid % Gp;
         const int64_t lane = 64 * (int64_t)g + gl;
         const uint64_t* bits = a.lane_bits + g;
-        // the lane's spike words (a lane past the last reads the last one's)
-        const uint64_t* lwords = a.ring_words + (lane < a.lanes ? lane : a.lanes - 1) * a.nwords;
-        const int64_t slot_words = a.lanes * a.nwords;
         for (int64_t d = d0 + tid / Gp; d < d1; d += kThreads / Gp) {
             float acc = 0.f;
             int64_t e = a.csc_ptr[d];
@@ -1202,22 +1214,16 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
                     r[u] = in ? a.csc_src[e + u] : 0;
                     w[u] = in ? a.csc_w[e + u] : 0.f;
                 }
-                int sh[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int src = r[u] / a.csr_d;
-                    const int sub = r[u] - src * a.csr_d;
-                    const int slot = L.slot[sub];
-                    const bool young = L.sub[sub] != 0;
-                    sh[u] = young ? (src & 63) : gl;
+                    const int slot = L.slot[r[u] - src * a.csr_d];
                     wb[u] = (e + u < e1 && slot >= 0)
-                                ? (young ? lwords[slot * slot_words + (src >> 6)]
-                                         : bits[((int64_t)slot * a.n + src) * a.lw])
-                                : 0ull;
+                                ? bits[((int64_t)slot * a.n + src) * a.lw] : 0ull;
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if ((wb[u] >> sh[u]) & 1ull) acc = acc + w[u];
+                    if ((wb[u] >> gl) & 1ull) acc = acc + w[u];
             }
             if (gl < lanes_g) {
                 const bool live = a.lane_t_end == nullptr ||
@@ -1231,20 +1237,16 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
 template <int G, bool ORDERED>
 __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) {
     if (ORDERED && a.lane_bits != nullptr) {
-        // the lane bits of the slot written one launch earlier, one
-        // (word, 64-lane group) per wavefront of the P workgroups (no
-        // workgroup of this launch reads them: their age is 1), then the push
-        // as the previous launch decided it
+        // the lane bits of the slot the next launch writes start empty
+        // (spread over the P workgroups; no workgroup of this launch reads
+        // that slot: its age would be ring_d), then the push as the previous
+        // launch decided it
         const int64_t np = (a.nchunks + a.cpb - 1) / a.cpb;
         const int64_t nwg = np * a.lanes, wid = lane * np + px;
         const int pull = ld_const(&a.ctl->push_mode[a.clock & 3]);
-        const int s1 = ring_slot(a.clock, 1, a.R);
-        const int64_t items = ld_const(&a.ctl->lane_bits_mode[a.clock & 3]) ? a.nwords * a.lw : 0;
-        constexpr int kWaves = kThreads / 64;
-        for (int64_t it = wid * kWaves + (threadIdx.x >> 6); it < items; it += nwg * kWaves)
-            transpose_lane_bits(a.ring_words + (int64_t)s1 * a.lanes * a.nwords,
-                                a.lane_bits + (int64_t)s1 * a.n * a.lw, a.lanes, a.nwords, a.n,
-                                a.lw, it / a.lw, (int)(it % a.lw));
+        uint64_t* nb = a.lane_bits + (int64_t)ring_slot(a.clock + 1, 0, a.R) * a.n * a.lw;
+        for (int64_t i = wid * kThreads + threadIdx.x; i < a.n * a.lw; i += nwg * kThreads)
+            nb[i] = 0ull;
         if (pull) {
             pull_role(a, L, wid, nwg);
             return;
@@ -2034,8 +2036,6 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ps.in_t0 = p->in_t0;
     ps.lane_bits = lane_bits;
     ps.lw = lw;
-    ps.ring_words = p->ring_words;
-    ps.nwords = nwords;
     ps.ctl_rw = ctl;
     ps.terms_per_row = (float)((double)p->nnz / (double)(n * p->csr_d));
     ps.pull_above = 0.5f * (float)cap;
@@ -2083,6 +2083,9 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         a.ring_idx = p->ring_idx + slot * lanes * n;
         a.ring_cnt = p->ring_cnt + slot * lanes * nchunks;
         a.ring_words = p->ring_words + slot * lanes * nwords;
+        a.lane_bits = lane_bits ? lane_bits + slot * n * lw : nullptr;
+        a.lw = lw;
+        a.lane_bits_on = lane_bits ? &ctl->lane_bits_mode[(p->ring_cursor + t) & 3] : nullptr;
         a.rec_z = p->rec_z ? p->rec_z + t * rec_step : nullptr;
         a.rec_v = p->rec_v ? p->rec_v + t * rec_step : nullptr;
         a.rec_ref = p->rec_refractory ? p->rec_refractory + t * rec_step : nullptr;

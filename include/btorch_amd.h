@@ -442,8 +442,8 @@ typedef struct btx_rsnn_plan {
      * segment) on average is delivered as a dense pull instead: every
      * destination sums its active incoming edges in the coalesced order
      * (the same terms and order, so the same bits).  The engine keeps it:
-     * each step transposes the slot written one step earlier from
-     * ring_words, and ring_rebuild rebuilds every slot; a caller that empties
+     * while activity nears the pull, each step marks the slot it writes, and
+     * ring_rebuild rebuilds every slot from ring_words; a caller that empties
      * lanes of the ring clears their bits too.  The decision is made one
      * step ahead from the ring's spike totals (the newest slot estimated by
      * the one before). */
