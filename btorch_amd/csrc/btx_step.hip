@@ -218,15 +218,30 @@ struct PushStep {
     // active incoming edges in the coalesced order into pull_acc
     uint64_t* lane_bits;         // [R][n][lw]: bit b of word (slot, j, b/64) = lane b pushed j
     int lw;
-    Ctl* ctl_rw;                 // the control block, written: slot totals, next push mode
-    float terms_per_row;         // nnz / (n * csr_d)
-    float pull_above;            // terms per bin above which the step pulls
     float* pull_acc;             // [lanes][n] of this step's buffer
     const int64_t* csc_ptr;
     const int32_t* csc_src;
     const float* csc_w;
     int delay0;
     int sub0;
+};
+
+// Arguments of the control workgroup, a block of their own: were they read
+// from the push's block, the compiler would hold the shared values for the
+// whole kernel, reloading them from register lanes thousands of times on the
+// step's other paths.
+struct ControlArgs {
+    const int32_t* ring_cnt;     // [R][lanes][nchunks]
+    const Ctl* ctl;              // bin delays
+    Ctl* ctl_rw;                 // written: slot totals, next push mode, lane-bit marking
+    int64_t m;                   // lanes * nchunks
+    int64_t clock;               // cursor + t
+    int R;
+    int nbins;
+    int affine;                  // bin j has delay delay0 + j
+    int delay0;
+    float terms_per_row;         // nnz / (n * csr_d)
+    float above;                 // terms per bin above which a step pulls, x lanes x nchunks
 };
 
 // LDS of one workgroup, shared by the two roles.
@@ -643,7 +658,11 @@ __device__ __forceinline__ RedHead reduce_head(const ReduceIn& a, int64_t nchunk
     h.r = a.ev_r[q0];
     h.w = a.ev_w[q0];
     h.c = ld_const(a.bin_cnt + lane * nchunks + seg);
-    h.pull = a.pull_mode != nullptr ? ld_const(a.pull_mode) : 0;
+    // a vector load, waited for with the bin head rather than with the
+    // scalar loads of the launch's parameters
+    h.pull = a.pull_mode != nullptr
+                 ? __hip_atomic_load(a.pull_mode, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                 : 0;
     return h;
 }
 
@@ -1147,13 +1166,12 @@ __device__ __forceinline__ int ring_slot(int64_t clock, int delay, int R) {
 // marking is kept on while the push would bin more than an eighth of that.
 // Either way the sums are the same bits, so an estimate only ever costs
 // time.
-__device__ void control_role(const PushStep& a) {
+__device__ void control_role(const ControlArgs& a) {
     const int tid = threadIdx.x;
     const int s1 = ring_slot(a.clock, 1, a.R);
-    const int32_t* cnt = a.ring_cnt + (int64_t)s1 * a.lanes * a.nchunks;
-    const int64_t m = a.lanes * a.nchunks;
+    const int32_t* cnt = a.ring_cnt + (int64_t)s1 * a.m;
     int sum = 0;
-    for (int64_t i = tid; i < m; i += kThreads) sum += cnt[i];
+    for (int64_t i = tid; i < a.m; i += kThreads) sum += cnt[i];
     int total;
     block_excl_scan<kThreads / 64>(sum, total);
     if (tid != 0) return;
@@ -1165,13 +1183,12 @@ __device__ void control_role(const PushStep& a) {
                         : (float)total;
     }
     const float terms = rows * a.terms_per_row;
-    const float above = a.pull_above * (float)(a.lanes * a.nchunks);
     int run = a.ctl_rw->lane_bits_run;
     run = a.ctl_rw->lane_bits_mode[a.clock & 3] ? (run < (1 << 20) ? run + 1 : run) : 0;
     a.ctl_rw->lane_bits_run = run;
-    const bool pull = terms > above && run >= a.R;
+    const bool pull = terms > a.above && run >= a.R;
     a.ctl_rw->push_mode[(a.clock + 1) & 3] = pull ? 1 : 0;
-    a.ctl_rw->lane_bits_mode[(a.clock + 1) & 3] = (pull || terms > 0.125f * above) ? 1 : 0;
+    a.ctl_rw->lane_bits_mode[(a.clock + 1) & 3] = (pull || terms > 0.125f * a.above) ? 1 : 0;
 }
 
 // The dense pull of one step: workgroup `wid` of `nwg` sums, for a range of
@@ -1236,23 +1253,26 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
 
 template <int G, bool ORDERED>
 __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) {
+    // a lane past its trial end pushes nothing (a dense pull still covers its
+    // share of the destinations)
+    const bool idle = a.lane_t_end != nullptr && a.in_t0[lane] + a.t >= a.lane_t_end[lane];
+    const int64_t np = (a.nchunks + a.cpb - 1) / a.cpb;
+    const int64_t nwg = np * a.lanes, wid = lane * np + px;
+    int pull = 0;
     if (ORDERED && a.lane_bits != nullptr) {
-        // the lane bits of the slot the next launch writes start empty
-        // (spread over the P workgroups; no workgroup of this launch reads
-        // that slot: its age would be ring_d), then the push as the previous
-        // launch decided it
-        const int64_t np = (a.nchunks + a.cpb - 1) / a.cpb;
-        const int64_t nwg = np * a.lanes, wid = lane * np + px;
-        const int pull = ld_const(&a.ctl->push_mode[a.clock & 3]);
+        // how the previous launch decided this push is delivered (looked at
+        // once the segment counts are in, so that their requests do not wait
+        // for it).  While this launch marks lane bits, those of the slot the
+        // next launch writes start empty (spread over the P workgroups; no
+        // workgroup of this launch reads that slot: its age would be
+        // ring_d).  The first marking launch after a pause marks a slot left
+        // unzeroed, which no pull reads: the pull needs R marking launches
+        // in a row, by when that slot is written again.
+        pull = ld_const(&a.ctl->push_mode[a.clock & 3]);
+        const int64_t zn = ld_const(&a.ctl->lane_bits_mode[a.clock & 3]) ? a.n * a.lw : 0;
         uint64_t* nb = a.lane_bits + (int64_t)ring_slot(a.clock + 1, 0, a.R) * a.n * a.lw;
-        for (int64_t i = wid * kThreads + threadIdx.x; i < a.n * a.lw; i += nwg * kThreads)
-            nb[i] = 0ull;
-        if (pull) {
-            pull_role(a, L, wid, nwg);
-            return;
-        }
+        for (int64_t i = wid * kThreads + threadIdx.x; i < zn; i += nwg * kThreads) nb[i] = 0ull;
     }
-    if (a.lane_t_end != nullptr && a.in_t0[lane] + a.t >= a.lane_t_end[lane]) return;
     const int64_t c0 = px * a.cpb;
     const int ncb = (int)(a.nchunks - c0 < a.cpb ? a.nchunks - c0 : a.cpb);
     const int tid = threadIdx.x;
@@ -1269,7 +1289,7 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
     __syncthreads();
     const int nseg = a.nbins * ncb;  // <= kThreads (host check)
     int c = 0;
-    if (tid < nseg) {
+    if (tid < nseg && !idle) {
         const int j = tid / ncb, cc = tid - j * ncb;
         c = a.ring_cnt[((int64_t)L.slot[j] * a.lanes + lane) * a.nchunks + c0 + cc];
         // a segment never lists more neurons than it has
@@ -1282,6 +1302,10 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
     if (tid < nseg) L.off[tid] = off;
     if (tid == 0) L.off[nseg] = total;
     __syncthreads();
+    if (ORDERED && pull) {
+        pull_role(a, L, wid, nwg);
+        return;
+    }
     if (total == 0) return;
 
     constexpr int gpb = kThreads / G;
@@ -1371,14 +1395,15 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 // [nc, nc + np) play P (the longer chain, dispatched first), the rest F.
 template <int G, bool ORDERED, int KIND>
 __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArgs a, PushStep p,
-                                                                          int np, int nc) {
+                                                                          ControlArgs ca, int np,
+                                                                          int nc) {
     // KIND: kLif, kGlif3, kTwoComp or kMixed, with kSynX for the alpha and
     // dual-exponential synapses
     __shared__ StepLds lds;
     const int64_t lane = blockIdx.y;
     const int x = (int)blockIdx.x - nc;
     if (x < 0) {
-        if (ORDERED && lane == 0) control_role(p);
+        if (ORDERED && lane == 0) control_role(ca);
     } else if (x < np) {
         p_role<G, ORDERED>(p, lds.push, x, lane);
     } else {
@@ -1703,7 +1728,8 @@ static int vec4_ok(const btx_rsnn_plan* p, const float* x) {
 template <int KIND>
 static void launch_neuron_only(const StepArgs& a, dim3 grid, hipStream_t s) {
     PushStep none{};
-    hipLaunchKernelGGL((k_step<16, true, KIND>), grid, dim3(kThreads), 0, s, a, none, 0, 0);
+    hipLaunchKernelGGL((k_step<16, true, KIND>), grid, dim3(kThreads), 0, s, a, none, ControlArgs{}, 0,
+                       0);
 }
 
 BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float* v_pre,
@@ -1829,34 +1855,35 @@ struct KernelTimer {
 static int rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s, KernelTimer& kt);
 
 template <bool ORDERED, int KIND>
-static void launch_step_g(const StepArgs& a, const PushStep& ps, int np, int nc, int group,
-                          dim3 grid, hipStream_t s) {
+static void launch_step_g(const StepArgs& a, const PushStep& ps, const ControlArgs& ca, int np,
+                          int nc, int group, dim3 grid, hipStream_t s) {
     switch (group) {
-        case 8: hipLaunchKernelGGL((k_step<8, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np, nc); break;
-        case 16: hipLaunchKernelGGL((k_step<16, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np, nc); break;
-        case 32: hipLaunchKernelGGL((k_step<32, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np, nc); break;
-        default: hipLaunchKernelGGL((k_step<64, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, np, nc); break;
+        case 8: hipLaunchKernelGGL((k_step<8, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc); break;
+        case 16: hipLaunchKernelGGL((k_step<16, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc); break;
+        case 32: hipLaunchKernelGGL((k_step<32, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc); break;
+        default: hipLaunchKernelGGL((k_step<64, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc); break;
     }
 }
 
 template <bool ORDERED>
-static void launch_step_k(const StepArgs& a, const PushStep& ps, int np, int nc, int group,
-                          dim3 grid, hipStream_t s) {
+static void launch_step_k(const StepArgs& a, const PushStep& ps, const ControlArgs& ca, int np,
+                          int nc, int group, dim3 grid, hipStream_t s) {
     if (a.syn != BTX_SYN_EXP) {
-        if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3 | kSynX>(a, ps, np, nc, group, grid, s);
-        else launch_step_g<ORDERED, kLif | kSynX>(a, ps, np, nc, group, grid, s);
+        if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3 | kSynX>(a, ps, ca, np, nc, group, grid, s);
+        else launch_step_g<ORDERED, kLif | kSynX>(a, ps, ca, np, nc, group, grid, s);
         return;
     }
-    if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3>(a, ps, np, nc, group, grid, s);
-    else if (a.kind == kTwoComp) launch_step_g<ORDERED, kTwoComp>(a, ps, np, nc, group, grid, s);
-    else if (a.kind == kMixed) launch_step_g<ORDERED, kMixed>(a, ps, np, nc, group, grid, s);
-    else launch_step_g<ORDERED, kLif>(a, ps, np, nc, group, grid, s);
+    if (a.kind == kGlif3) launch_step_g<ORDERED, kGlif3>(a, ps, ca, np, nc, group, grid, s);
+    else if (a.kind == kTwoComp) launch_step_g<ORDERED, kTwoComp>(a, ps, ca, np, nc, group, grid, s);
+    else if (a.kind == kMixed) launch_step_g<ORDERED, kMixed>(a, ps, ca, np, nc, group, grid, s);
+    else launch_step_g<ORDERED, kLif>(a, ps, ca, np, nc, group, grid, s);
 }
 
-static void launch_step(int ordered, const StepArgs& a, const PushStep& ps, int np, int nc,
-                        int group, dim3 grid, hipStream_t s) {
-    if (ordered) launch_step_k<true>(a, ps, np, nc, group, grid, s);
-    else launch_step_k<false>(a, ps, np, nc, group, grid, s);
+static void launch_step(int ordered, const StepArgs& a, const PushStep& ps,
+                        const ControlArgs& ca, int np, int nc, int group, dim3 grid,
+                        hipStream_t s) {
+    if (ordered) launch_step_k<true>(a, ps, ca, np, nc, group, grid, s);
+    else launch_step_k<false>(a, ps, ca, np, nc, group, grid, s);
 }
 
 }  // namespace btx
@@ -2036,9 +2063,6 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ps.in_t0 = p->in_t0;
     ps.lane_bits = lane_bits;
     ps.lw = lw;
-    ps.ctl_rw = ctl;
-    ps.terms_per_row = (float)((double)p->nnz / (double)(n * p->csr_d));
-    ps.pull_above = 0.5f * (float)cap;
     ps.csc_ptr = p->csc_ptr;
     ps.csc_src = p->csc_src;
     ps.csc_w = p->csc_w;
@@ -2062,6 +2086,17 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     const int nf = (int)nchunks;
     // the control column: the slot totals and the next launch's push mode
     const int nc = lane_bits != nullptr ? 1 : 0;
+    ControlArgs ca{};
+    ca.ring_cnt = p->ring_cnt;
+    ca.ctl = ctl;
+    ca.ctl_rw = ctl;
+    ca.m = lanes * nchunks;
+    ca.R = R;
+    ca.nbins = p->nbins;
+    ca.affine = ps.affine;
+    ca.delay0 = ps.delay0;
+    ca.terms_per_row = (float)((double)p->nnz / (double)(n * p->csr_d));
+    ca.above = 0.5f * (float)cap * (float)(lanes * nchunks);
 
     const int64_t rec_step = m;
     for (int64_t t = 0; t < n_steps; ++t) {
@@ -2101,6 +2136,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         a.rec_sh2_prev = (p->rec_syn_h2 && t > 0) ? p->rec_syn_h2 + (t - 1) * rec_step : nullptr;
         a.rec_any = set_rec_any(a);
         ps.clock = p->ring_cursor + t;
+        ca.clock = ps.clock;
         ps.t = t;
         ReduceIn cur{};
         bins_of((int)(t & 1), p->ring_cursor + t, cur);
@@ -2113,16 +2149,16 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         if (fused) {
             kt.begin(0);
             const dim3 grid((unsigned)(nc + nf + npb), (unsigned)lanes);
-            launch_step(p->ordered, a, ps, npb, nc, group, grid, s);
+            launch_step(p->ordered, a, ps, ca, npb, nc, group, grid, s);
             kt.end(0);
         } else {
             kt.begin(0);
             const dim3 fgrid((unsigned)nf, (unsigned)lanes);
-            launch_step(p->ordered, a, ps, 0, 0, group, fgrid, s);
+            launch_step(p->ordered, a, ps, ca, 0, 0, group, fgrid, s);
             kt.end(0);
             kt.begin(2);
             const dim3 pgrid((unsigned)(nc + npb), (unsigned)lanes);
-            launch_step(p->ordered, a, ps, npb, nc, group, pgrid, s);
+            launch_step(p->ordered, a, ps, ca, npb, nc, group, pgrid, s);
             kt.end(2);
         }
     }
