@@ -34,11 +34,16 @@ class HipSparseConfig(SparseBackendConfig):
     bin_cap: terms per (lane, 1024-destination segment) bin of the recurrent
         engine's ordered mode, 0 = the maximum (8192); a bin that receives
         more is reduced by pulling every destination of the segment.
+    dense_pull: let the recurrent engine deliver a step whose push would
+        bin more than min(1024, bin_cap / 2) terms per bin on average as a
+        dense pull over every destination's incoming edges (same bits);
+        False always pushes into the bins.
     """
 
     ordered: bool = True
     slots: int = 16
     bin_cap: int = 0
+    dense_pull: bool = True
 
     def validate(self) -> None:
         if self.slots < 2 or self.slots > 16:

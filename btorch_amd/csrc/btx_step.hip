@@ -60,6 +60,7 @@ constexpr int kBinCapMax = 8192;        // terms per bin in the workspace (large
 constexpr int kSortMax = 32;            // most terms a neuron sorts in LDS
 constexpr int kMaxChunks = 1024;        // n <= 1,048,576 (push reservation table)
 constexpr int kStage = 1024;            // terms a push workgroup stages in LDS
+constexpr int kPullWgs = 512;           // workgroups a dense pull spreads over, at least
 
 static_assert(kChunk == (1 << kChunkShift) && kChunk == 4 * kThreads, "segment layout");
 
@@ -214,7 +215,7 @@ struct PushStep {
     int64_t t;
     int affine;                  // bin j is (delay0 + j, sub0 + j)
     // ordered mode, dense pull (chosen per step when the push would bin more
-    // than pull_above terms per (lane, segment)): sums of every destination's
+    // than a threshold of terms per (lane, segment)): sums of every destination's
     // active incoming edges in the coalesced order into pull_acc
     uint64_t* lane_bits;         // [R][n][lw]: bit b of word (slot, j, b/64) = lane b pushed j
     int lw;
@@ -224,6 +225,7 @@ struct PushStep {
     const float* csc_w;
     int delay0;
     int sub0;
+    int nh;                      // pull-only workgroups per lane row (see k_step)
 };
 
 // Arguments of the control workgroup, a block of their own: were they read
@@ -1161,7 +1163,7 @@ __device__ __forceinline__ int ring_slot(int64_t clock, int delay, int R) {
 // the totals of ages >= 1 are known, and a slot younger than that (written
 // in this launch, or the next) is taken to hold as many spikes as the
 // youngest known one.  The pull is taken when the push would bin more than
-// pull_above terms per (lane, segment) on average and every slot it reads
+// min(kBinCap, cap / 2) terms per (lane, segment) on average and every slot it reads
 // (ages up to ring_d - 1) had its lane bits marked when it was written; the
 // marking is kept on while the push would bin more than an eighth of that.
 // Either way the sums are the same bits, so an estimate only ever costs
@@ -1253,10 +1255,13 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
 
 template <int G, bool ORDERED>
 __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) {
-    // a lane past its trial end pushes nothing (a dense pull still covers its
-    // share of the destinations)
-    const bool idle = a.lane_t_end != nullptr && a.in_t0[lane] + a.t >= a.lane_t_end[lane];
+    // workgroups px >= np only join a dense pull; a lane past its trial end
+    // pushes nothing (a dense pull still covers its share of the
+    // destinations)
     const int64_t np = (a.nchunks + a.cpb - 1) / a.cpb;
+    const bool helper = px >= np;
+    const bool idle = helper || (a.lane_t_end != nullptr &&
+                                 a.in_t0[lane] + a.t >= a.lane_t_end[lane]);
     const int64_t nwg = np * a.lanes, wid = lane * np + px;
     int pull = 0;
     if (ORDERED && a.lane_bits != nullptr) {
@@ -1269,11 +1274,12 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
         // unzeroed, which no pull reads: the pull needs R marking launches
         // in a row, by when that slot is written again.
         pull = ld_const(&a.ctl->push_mode[a.clock & 3]);
-        const int64_t zn = ld_const(&a.ctl->lane_bits_mode[a.clock & 3]) ? a.n * a.lw : 0;
+        const int64_t zn =
+            !helper && ld_const(&a.ctl->lane_bits_mode[a.clock & 3]) ? a.n * a.lw : 0;
         uint64_t* nb = a.lane_bits + (int64_t)ring_slot(a.clock + 1, 0, a.R) * a.n * a.lw;
         for (int64_t i = wid * kThreads + threadIdx.x; i < zn; i += nwg * kThreads) nb[i] = 0ull;
     }
-    const int64_t c0 = px * a.cpb;
+    const int64_t c0 = helper ? 0 : px * a.cpb;
     const int ncb = (int)(a.nchunks - c0 < a.cpb ? a.nchunks - c0 : a.cpb);
     const int tid = threadIdx.x;
     if (tid < a.nbins) {
@@ -1303,7 +1309,8 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
     if (tid == 0) L.off[nseg] = total;
     __syncthreads();
     if (ORDERED && pull) {
-        pull_role(a, L, wid, nwg);
+        const int64_t npp = np + a.nh;
+        pull_role(a, L, lane * npp + px, npp * a.lanes);
         return;
     }
     if (total == 0) return;
@@ -1392,11 +1399,15 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 
 // Workgroups [0, nc) of a row are the control column (nc = 0 or 1; lane 0's
 // sums the slot totals and decides the next push, the others end at once),
-// [nc, nc + np) play P (the longer chain, dispatched first), the rest F.
+// [nc, nc + np) play P (the longer chain, dispatched first), the next nh
+// (= p.nh; a scalar argument of its own, like np, so that the role choice
+// reads no argument block) only join a dense pull (a launch of few lanes and segments has too few P
+// workgroups to pull every destination; they end at once after a push), the
+// rest F.
 template <int G, bool ORDERED, int KIND>
 __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArgs a, PushStep p,
                                                                           ControlArgs ca, int np,
-                                                                          int nc) {
+                                                                          int nc, int nh) {
     // KIND: kLif, kGlif3, kTwoComp or kMixed, with kSynX for the alpha and
     // dual-exponential synapses
     __shared__ StepLds lds;
@@ -1404,10 +1415,10 @@ __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArg
     const int x = (int)blockIdx.x - nc;
     if (x < 0) {
         if (ORDERED && lane == 0) control_role(ca);
-    } else if (x < np) {
+    } else if (x < np + nh) {
         p_role<G, ORDERED>(p, lds.push, x, lane);
     } else {
-        f_role<KIND>(a, lds, (int64_t)x - np, lane);
+        f_role<KIND>(a, lds, (int64_t)x - np - nh, lane);
     }
 }
 
@@ -1729,7 +1740,7 @@ template <int KIND>
 static void launch_neuron_only(const StepArgs& a, dim3 grid, hipStream_t s) {
     PushStep none{};
     hipLaunchKernelGGL((k_step<16, true, KIND>), grid, dim3(kThreads), 0, s, a, none, ControlArgs{}, 0,
-                       0);
+                       0, 0);
 }
 
 BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float* v_pre,
@@ -1858,10 +1869,10 @@ template <bool ORDERED, int KIND>
 static void launch_step_g(const StepArgs& a, const PushStep& ps, const ControlArgs& ca, int np,
                           int nc, int group, dim3 grid, hipStream_t s) {
     switch (group) {
-        case 8: hipLaunchKernelGGL((k_step<8, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc); break;
-        case 16: hipLaunchKernelGGL((k_step<16, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc); break;
-        case 32: hipLaunchKernelGGL((k_step<32, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc); break;
-        default: hipLaunchKernelGGL((k_step<64, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc); break;
+        case 8: hipLaunchKernelGGL((k_step<8, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc, ps.nh); break;
+        case 16: hipLaunchKernelGGL((k_step<16, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc, ps.nh); break;
+        case 32: hipLaunchKernelGGL((k_step<32, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc, ps.nh); break;
+        default: hipLaunchKernelGGL((k_step<64, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc, ps.nh); break;
     }
 }
 
@@ -2086,6 +2097,12 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     const int nf = (int)nchunks;
     // the control column: the slot totals and the next launch's push mode
     const int nc = lane_bits != nullptr ? 1 : 0;
+    // pull-only workgroups: a dense pull spreads over at least kPullWgs
+    ps.nh = 0;
+    if (lane_bits != nullptr) {
+        const int64_t want = (kPullWgs + lanes - 1) / lanes;
+        ps.nh = want > npb ? (int)(want - npb) : 0;
+    }
     ControlArgs ca{};
     ca.ring_cnt = p->ring_cnt;
     ca.ctl = ctl;
@@ -2096,7 +2113,11 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ca.affine = ps.affine;
     ca.delay0 = ps.delay0;
     ca.terms_per_row = (float)((double)p->nnz / (double)(n * p->csr_d));
-    ca.above = 0.5f * (float)cap * (float)(lanes * nchunks);
+    // a dense pull once the push would bin more terms per (lane, segment)
+    // than LDS sorts (kBinCap; the larger bins take the slower global
+    // order), or half the bin capacity when that is smaller
+    const int above = cap / 2 < kBinCap ? cap / 2 : kBinCap;
+    ca.above = (float)above * (float)(lanes * nchunks);
 
     const int64_t rec_step = m;
     for (int64_t t = 0; t < n_steps; ++t) {
@@ -2148,16 +2169,18 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         ps.pull_acc = cur.pull_acc ? (float*)cur.pull_acc : nullptr;
         if (fused) {
             kt.begin(0);
-            const dim3 grid((unsigned)(nc + nf + npb), (unsigned)lanes);
+            const dim3 grid((unsigned)(nc + npb + ps.nh + nf), (unsigned)lanes);
             launch_step(p->ordered, a, ps, ca, npb, nc, group, grid, s);
             kt.end(0);
         } else {
             kt.begin(0);
             const dim3 fgrid((unsigned)nf, (unsigned)lanes);
-            launch_step(p->ordered, a, ps, ca, 0, 0, group, fgrid, s);
+            PushStep fonly = ps;  // the F launch has no pull-only workgroups
+            fonly.nh = 0;
+            launch_step(p->ordered, a, fonly, ca, 0, 0, group, fgrid, s);
             kt.end(0);
             kt.begin(2);
-            const dim3 pgrid((unsigned)(nc + npb), (unsigned)lanes);
+            const dim3 pgrid((unsigned)(nc + npb + ps.nh), (unsigned)lanes);
             launch_step(p->ordered, a, ps, ca, npb, nc, group, pgrid, s);
             kt.end(2);
         }

@@ -328,6 +328,7 @@ class Engine:
         cfg = conn.sparse_config
         parts = [lanes, str(dev), float(environ.get("dt")), bool(torch.is_grad_enabled()),
                  int(cfg.ordered), int(cfg.slots), int(getattr(cfg, "bin_cap", 0)),
+                 self.dense_pull(),
                  id(conn._csr_row_ptr), id(conn._csc_ptr), conn.weight_key(),
                  id(conn._prepared_sparse_weight)]
         mods = (sp.neuron.modules() if isinstance(sp.neuron, MixedState) else [sp.neuron])
@@ -341,6 +342,11 @@ class Engine:
             for name in self._SCALAR_ATTRS:
                 parts.append(getattr(m, name, None))
         return tuple(parts)
+
+    def dense_pull(self) -> bool:
+        """Whether runs hand the kernel the ring's lane bits (and with them
+        the dense pull of busy steps)."""
+        return bool(getattr(self.spec.conn.sparse_config, "dense_pull", True))
 
     def _template(self, lanes: int):
         """(template plan, tensors it points into): parameters,
@@ -615,7 +621,7 @@ class Engine:
         p.ring_idx = ring.idx.data_ptr()
         p.ring_cnt = ring.cnt.data_ptr()
         p.ring_words = ring.words.data_ptr()
-        p.ring_lane_bits = ring.lane_bits.data_ptr()
+        p.ring_lane_bits = ring.lane_bits.data_ptr() if self.dense_pull() else None
         p.ring_cursor = ring.cursor
         self._set_outputs(p, T, lanes, n, drive, counts, sample)
         nwords = (n + 63) // 64
@@ -735,7 +741,7 @@ class ResidentRun:
         p.ring_idx = self.idx.data_ptr()
         p.ring_cnt = self.cnt.data_ptr()
         p.ring_words = self.words.data_ptr()
-        p.ring_lane_bits = self.lane_bits.data_ptr()
+        p.ring_lane_bits = self.lane_bits.data_ptr() if engine.dense_pull() else None
         p.ring_cursor = 0
         p.ring_rebuild = 0  # lists stay current between launches (all empty at start)
         engine._set_outputs(p, T, lanes, n, drive, counts, sample)

@@ -86,7 +86,7 @@ class _GraphEntry:
         p.ring_idx = self.ring.idx.data_ptr()
         p.ring_cnt = self.ring.cnt.data_ptr()
         p.ring_words = self.ring.words.data_ptr()
-        p.ring_lane_bits = self.ring.lane_bits.data_ptr()
+        p.ring_lane_bits = self.ring.lane_bits.data_ptr() if eng.dense_pull() else None
         p.ring_cursor = ring_d
         p.ring_rebuild = 1
         p.tables_ready = 0

@@ -380,20 +380,21 @@ def _terms_per_bin(z, cexp, n, D):
     return pad.reshape(T, B, segs, 1024).sum(-1)
 
 
-@pytest.mark.parametrize("bin_cap,lanes,expect", [(0, 4, "big"), (64, 4, "pull"),
-                                                  (64, 70, "pull")])
+@pytest.mark.parametrize("bin_cap,lanes,expect", [(0, 4, "big"), (0, 4, "pull"),
+                                                  (64, 4, "pull"), (64, 70, "pull")])
 def test_engine_high_activity_large_bins_and_dense_pull(bin_cap, lanes, expect):
     # activity at which the push bins more terms per (lane, segment) than
-    # the 1024 sorted in LDS: with the default capacity the bins are sorted
-    # through the global order array; with a small capacity the step is
-    # delivered as a dense pull (two 64-lane groups at 70 lanes); both
-    # bit-exact with the oracle
+    # the 1024 sorted in LDS: without the dense pull the bins are sorted
+    # through the global order array; with it the step is delivered as a
+    # dense pull (two 64-lane groups at 70 lanes; pull-only workgroups join
+    # at 4 lanes of 3 segments); all bit-exact with the oracle
     import scipy.sparse
     n, T, D = 3000, 50, 5
     c = workloads.powerlaw_net_fast(n=n, k_min=100, k_max=600, delay_lo=1, delay_hi=D - 1,
                                     seed=43)
     cexp = scipy.sparse.coo_array(c.expanded(D) * 0.05).astype(np.float32)
-    syn = HeterSynapsePSC(n, 1, None, SparseConn(cexp, sparse_config={"bin_cap": bin_cap}),
+    cfg = {"bin_cap": bin_cap, "dense_pull": expect == "pull"}
+    syn = HeterSynapsePSC(n, 1, None, SparseConn(cexp, sparse_config=cfg),
                           base_psc=ExponentialPSC, max_delay_steps=D,
                           use_circular_buffer=True, tau_syn=5.0)
     names = ["neuron.v", "synapse.psc"]
