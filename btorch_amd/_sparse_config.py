@@ -35,9 +35,9 @@ class HipSparseConfig(SparseBackendConfig):
         engine's ordered mode, 0 = the maximum (8192); a bin that receives
         more is reduced by pulling every destination of the segment.
     dense_pull: let the recurrent engine deliver a step whose push would
-        bin more than min(1024, bin_cap / 2) terms per bin on average as a
-        dense pull over every destination's incoming edges (same bits);
-        False always pushes into the bins.
+        bin more than bin_cap / 2 terms per bin on average (more than 1024
+        when lanes x segments < 256) as a dense pull over every destination's
+        incoming edges (same bits); False always pushes into the bins.
     """
 
     ordered: bool = True

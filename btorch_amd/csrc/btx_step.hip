@@ -1163,7 +1163,7 @@ __device__ __forceinline__ int ring_slot(int64_t clock, int delay, int R) {
 // the totals of ages >= 1 are known, and a slot younger than that (written
 // in this launch, or the next) is taken to hold as many spikes as the
 // youngest known one.  The pull is taken when the push would bin more than
-// min(kBinCap, cap / 2) terms per (lane, segment) on average and every slot it reads
+// a threshold of terms per (lane, segment) on average (set by the host) and every slot it reads
 // (ages up to ring_d - 1) had its lane bits marked when it was written; the
 // marking is kept on while the push would bin more than an eighth of that.
 // Either way the sums are the same bits, so an estimate only ever costs
@@ -2113,10 +2113,15 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     ca.affine = ps.affine;
     ca.delay0 = ps.delay0;
     ca.terms_per_row = (float)((double)p->nnz / (double)(n * p->csr_d));
-    // a dense pull once the push would bin more terms per (lane, segment)
-    // than LDS sorts (kBinCap; the larger bins take the slower global
-    // order), or half the bin capacity when that is smaller
-    const int above = cap / 2 < kBinCap ? cap / 2 : kBinCap;
+    // a dense pull once the push would bin more than half the bin capacity
+    // per (lane, segment) on average; in a launch of fewer bins than CUs
+    // (each bin one workgroup's serial sort) already once the bins outgrow
+    // what LDS sorts (kBinCap: the larger ones take the slower global order).
+    // Measured at the 140k connectome, 8 lanes (1096 bins): the pull reads
+    // all 50M edges, ~1.3 ms, against 0.1-0.4 ms for pushes of 1-4M terms;
+    // at 21.7k neurons, one lane (22 bins): ~65 us against 137 us for a
+    // push of 71k terms.
+    const int above = (lanes * nchunks < 256 && kBinCap < cap / 2) ? kBinCap : cap / 2;
     ca.above = (float)above * (float)(lanes * nchunks);
 
     const int64_t rec_step = m;

@@ -438,8 +438,9 @@ typedef struct btx_rsnn_plan {
     /* ordered mode, optional (NULL = off): the ring of lane bits, uint64
      * [ring_d + 1][n][ceil(lanes/64)] kept beside ring_words -- bit b of word
      * (slot, j, b/64) set when lane b pushed neuron j into that slot.  With
-     * it, a step whose push would bin more than min(1024, bin_cap/2) terms per (lane,
-     * segment) on average is delivered as a dense pull instead: every
+     * it, a step whose push would bin more than bin_cap/2 terms per (lane,
+     * segment) on average (more than 1024, the terms a bin sorts in LDS, when
+     * lanes x segments < 256) is delivered as a dense pull instead: every
      * destination sums its active incoming edges in the coalesced order
      * (the same terms and order, so the same bits).  The engine keeps it:
      * while activity nears the pull, each step marks the slot it writes, and
