@@ -392,25 +392,35 @@ def main():
     # weight, destination).  P: per active row the index and the int64
     # row_ptr pair (20 B), per term col + weight read and the term record
     # written (18 B).
-    spikes_per_step = spikes_t / K
-    bytes_launch = (28.0 * n * lanes + 4.0 * n + 16.0 * nwords * lanes + 4.0 * spikes_per_step
-                    + 20.0 * rows_t / K + 28.0 * ev_t / K)
-    avg_ms = region_ms / K
-    achieved = bytes_launch / (avg_ms * 1e-3)
+    def step_bytes(spikes, rows, events, steps):
+        return (28.0 * n * lanes + 4.0 * n + 16.0 * nwords * lanes + 4.0 * spikes / steps
+                + 20.0 * rows / steps + 28.0 * events / steps)
+
+    # the kernel: algorithmic bytes of a k_step launch over the mean duration
+    # of the launches (a HIP event pair around each, on the launch stream) of
+    # a KS-step run of the same launches (input tensor, dense spike record)
+    bytes_k = step_bytes(float(unpack_count(r_k)), rows_k, ev_k, KS)
+    step_us = per["step"][0] * 1e3 if per.get("step") else float("nan")
+    achieved = bytes_k / (step_us * 1e-6)
+    # the call: the timed model(x) call's region over its steps
+    bytes_launch = step_bytes(spikes_t, rows_t, ev_t, K)
+    call_us = region_ms / K * 1e3
     traffic = _traffic(args)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK,
                 "traffic": None if traffic is None else traffic["bytes_per_launch"],
                 "traffic_source": None if traffic is None else traffic["source"],
                 "kernel": "k_step (fused step + push + reduction)",
-                "bytes_per_launch": bytes_launch, "avg_us": avg_ms * 1e3,
-                "avg_us_source": "HIP events around the timed model(x) call / steps "
-                                 "(includes the host's enqueue of the call and the final "
-                                 "reduction launch)",
+                "bytes_per_launch": bytes_k, "avg_us": step_us,
+                "avg_us_source": f"HIP event pair around each k_step launch of a {KS}-step "
+                                 f"run of the timed call's launches, on the launch stream",
                 "per_launch_event_us": {k: round(v[0] * 1e3, 3) for k, v in per.items()},
                 "per_launch_event_p50_p90_p99_us": pct,
-                "per_launch_event_source": f"HIP event pair around each launch of a "
-                                           f"{KS}-step run of the same launches"}
+                "call": {"us_per_step": call_us, "bytes_per_step": bytes_launch,
+                         "frac": bytes_launch / (call_us * 1e-6) / HBM_PEAK,
+                         "what": "the timed model(x) call: HIP events around the call / "
+                                 "steps (includes the host's work before the first launch "
+                                 "and the final reduction launch)"}}
     calls = {"host_enqueue_ms": round(host_ms, 3), "first_call_ms": round(first_call_ms, 2),
              "what": "host_enqueue_ms: wall time of the timed model(x) call until it "
                      "returned (work enqueued); first_call_ms: the warm-up call, which "

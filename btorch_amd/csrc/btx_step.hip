@@ -1193,6 +1193,27 @@ __device__ void control_role(const ControlArgs& a) {
     a.ctl_rw->lane_bits_mode[(a.clock + 1) & 3] = (pull || terms > 0.125f * a.above) ? 1 : 0;
 }
 
+// Division of 0 <= x < 2^31 by a run-time 1 <= d <= 64 without a divide:
+// q = (umulhi(x, m) + x) >> l with l = ceil(log2 d), m = 2^32 (2^l - d) / d + 1
+// (the round-up multiplier of Granlund and Montgomery, "Division by
+// invariant integers using multiplication", 1994; checked against x / d for
+// every d on [0, 2^20) and a stride of the rest of [0, 2^31)).
+struct FastDiv {
+    uint32_t m;
+    int l;
+};
+
+__device__ __forceinline__ FastDiv fast_div(int d) {
+    int l = 0;
+    while ((1 << l) < d) ++l;
+    const uint64_t m = ((uint64_t)1 << 32) * (((uint64_t)1 << l) - (uint64_t)d) / (uint64_t)d + 1;
+    return FastDiv{(uint32_t)m, l};
+}
+
+__device__ __forceinline__ uint32_t div_u31(const FastDiv& f, uint32_t x) {
+    return (__umulhi(x, f.m) + x) >> f.l;
+}
+
 // The dense pull of one step: workgroup `wid` of `nwg` sums, for a range of
 // destinations and every lane, the weights of the active incoming edges in
 // the coalesced (dst, virtual source) order from +0.0f -- the terms and the
@@ -1203,6 +1224,7 @@ __device__ void control_role(const ControlArgs& a) {
 // (one per lane of a 64-lane group) walk one destination's edges together.
 __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nwg) {
     const int tid = threadIdx.x;
+    const FastDiv fd = fast_div(a.csr_d);
     for (int k = tid; k < a.csr_d; k += kThreads) {
         const int age = a.ctl->age_of_sub[k];
         L.slot[k] = age < 0 ? -1 : ring_slot(a.clock, age, a.R);
@@ -1222,27 +1244,44 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
             float acc = 0.f;
             int64_t e = a.csc_ptr[d];
             const int64_t e1 = a.csc_ptr[d + 1];
+            // software pipeline: the edges of the next U are requested before
+            // the lane bits of the current U are waited for
             constexpr int U = 8;
-            for (; e < e1; e += U) {
-                int r[U];
-                float w[U];
+            int r0[U];
+            float w0[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool in = e + u < e1;
+                r0[u] = in ? a.csc_src[e + u] : 0;
+                w0[u] = in ? a.csc_w[e + u] : 0.f;
+            }
+            while (e < e1) {
                 uint64_t wb[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const bool in = e + u < e1;
-                    r[u] = in ? a.csc_src[e + u] : 0;
-                    w[u] = in ? a.csc_w[e + u] : 0.f;
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int src = r[u] / a.csr_d;
-                    const int slot = L.slot[r[u] - src * a.csr_d];
+                    const uint32_t src = div_u31(fd, (uint32_t)r0[u]);
+                    const int slot = L.slot[r0[u] - (int)src * a.csr_d];
                     wb[u] = (e + u < e1 && slot >= 0)
                                 ? bits[((int64_t)slot * a.n + src) * a.lw] : 0ull;
                 }
+                const int64_t en = e + U;
+                int r1[U];
+                float w1[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool in = en + u < e1;
+                    r1[u] = in ? a.csc_src[en + u] : 0;
+                    w1[u] = in ? a.csc_w[en + u] : 0.f;
+                }
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if ((wb[u] >> gl) & 1ull) acc = acc + w[u];
+                    if ((wb[u] >> gl) & 1ull) acc = acc + w0[u];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    r0[u] = r1[u];
+                    w0[u] = w1[u];
+                }
+                e = en;
             }
             if (gl < lanes_g) {
                 const bool live = a.lane_t_end == nullptr ||
