@@ -1244,44 +1244,29 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
             float acc = 0.f;
             int64_t e = a.csc_ptr[d];
             const int64_t e1 = a.csc_ptr[d + 1];
-            // software pipeline: the edges of the next U are requested before
-            // the lane bits of the current U are waited for
-            constexpr int U = 8;
-            int r0[U];
-            float w0[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const bool in = e + u < e1;
-                r0[u] = in ? a.csc_src[e + u] : 0;
-                w0[u] = in ? a.csc_w[e + u] : 0.f;
-            }
-            while (e < e1) {
+            // 12 edges in flight per round of dependent loads (edges, then
+            // lane bits): the most that keeps the step within its registers
+            constexpr int U = 12;
+            for (; e < e1; e += U) {
+                int r[U];
+                float w[U];
                 uint64_t wb[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const uint32_t src = div_u31(fd, (uint32_t)r0[u]);
-                    const int slot = L.slot[r0[u] - (int)src * a.csr_d];
+                    const bool in = e + u < e1;
+                    r[u] = in ? a.csc_src[e + u] : 0;
+                    w[u] = in ? a.csc_w[e + u] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int src = (int)div_u31(fd, (uint32_t)r[u]);
+                    const int slot = L.slot[r[u] - src * a.csr_d];
                     wb[u] = (e + u < e1 && slot >= 0)
                                 ? bits[((int64_t)slot * a.n + src) * a.lw] : 0ull;
                 }
-                const int64_t en = e + U;
-                int r1[U];
-                float w1[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const bool in = en + u < e1;
-                    r1[u] = in ? a.csc_src[en + u] : 0;
-                    w1[u] = in ? a.csc_w[en + u] : 0.f;
-                }
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if ((wb[u] >> gl) & 1ull) acc = acc + w0[u];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    r0[u] = r1[u];
-                    w0[u] = w1[u];
-                }
-                e = en;
+                    if ((wb[u] >> gl) & 1ull) acc = acc + w[u];
             }
             if (gl < lanes_g) {
                 const bool live = a.lane_t_end == nullptr ||
