@@ -828,24 +828,44 @@ def train_bench(args, dev):
         x = (torch.rand((T, B, n_in), device=dev, generator=g) < prob).float()
         return x, labels
 
+    split = {"forward": 0.0, "backward": 0.0}
+
     def step():
         x, labels = batch()
         functional.init_net_state(model, batch_size=B, device=dev, dtype=torch.float32)
         opt.zero_grad(set_to_none=True)
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
         with environ.context(dt=1.0):
             z, _ = model(x)
         logits = readout(z.mean(0))  # readout of spike counts / T
         loss = torch.nn.functional.cross_entropy(logits, labels)
+        e1.record()
         loss.backward()
+        e2.record()
         opt.step()
+        e2.synchronize()
+        split["forward"] += e0.elapsed_time(e1)
+        split["backward"] += e1.elapsed_time(e2)
         return float(loss.detach()), float(z.detach().mean())
 
     step()  # warm-up
     torch.cuda.synchronize()
+    split["forward"] = split["backward"] = 0.0
     t0 = time.perf_counter()
     out = [step() for _ in range(args.train_steps)]
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / args.train_steps
+    fwd_ms = split["forward"] / args.train_steps
+    bwd_ms = split["backward"] / args.train_steps
+    # algorithmic bytes of the backward pass per step: the recompute of the
+    # checkpointed forward (at this activity a dense pull: the CSC edge
+    # stream, 8 B per edge; state and the three records, 44 B per
+    # neuron-lane) and the reverse step (k_adj_transpose: col + w, 8 B per
+    # edge; k_adj_wgrad: col read, gw read and written, 12 B per edge;
+    # k_adj_lif: 11 arrays, 44 B per neuron-lane)
+    bwd_bytes = T * (28.0 * c.nnz + 88.0 * n * B)
+    bwd_achieved = bwd_bytes / (bwd_ms * 1e-3)
     cpu = None
     if not args.no_cpu:
         cpu = train_cpu_baseline(model, x_sample=batch()[0], T_s=10)
@@ -855,6 +875,15 @@ def train_bench(args, dev):
             "neuron_steps_per_s": B * T * n / el, "loss": out[-1][0],
             "mean_rate_per_step": out[-1][1],
             "grads": "synaptic weights (13.4M), tau_m (16384), readout",
+            "forward_ms": fwd_ms, "backward_ms": bwd_ms,
+            "roofline": {"bound": "hbm", "kernel": "backward pass (chunk recomputes + reverse "
+                                                  "sweep: k_step, k_adj_transpose, k_adj_wgrad, "
+                                                  "k_adj_lif)",
+                         "achieved": bwd_achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": bwd_achieved / HBM_PEAK, "bytes_per_iteration": bwd_bytes,
+                         "avg_ms": bwd_ms,
+                         "avg_ms_source": "HIP events around loss.backward() of the timed "
+                                          "iterations"},
             "cpu_baseline": cpu}
 
 

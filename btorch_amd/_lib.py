@@ -208,6 +208,8 @@ _SIGNATURES = {
     "btx_exp_decay": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp]),
     "btx_syn_decay": (_c_int, [ctypes.POINTER(RsnnPlan), _vp]),
     "btx_adj_lif_step": (_c_int, [ctypes.POINTER(AdjLifArgs), _vp]),
+    "btx_adj_lif_chunk": (_c_int, [ctypes.POINTER(AdjLifArgs), _c_i64, _vp, _vp, _vp, _vp, _vp,
+                                   _vp]),
     "btx_adj_transpose": (_c_int, [_vp, _vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp]),
     "btx_adj_wgrad": (_c_int, [_vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp, _vp]),
     "btx_adj_transpose_delayed": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int, _c_i64, _vp, _c_i64,

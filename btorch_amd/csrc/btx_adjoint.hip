@@ -289,6 +289,36 @@ BTX_API int btx_adj_wgrad(const int64_t* row_ptr, const int32_t* col, int64_t n_
     return check_launch("btx_adj_wgrad");
 }
 
+BTX_API int btx_adj_lif_chunk(const btx_adj_lif_args* p, int64_t K, const int64_t* row_ptr,
+                              const int32_t* col, const float* w, const uint64_t* masks,
+                              float* gw, void* stream) {
+    if (p == nullptr) return fail("btx_adj_lif_chunk: null args");
+    if (K <= 0 || p->n <= 0 || p->lanes <= 0) return 0;
+    if (p->lanes > 64) return fail("btx_adj_lif_chunk: at most 64 lanes per mask word");
+    if (p->gsyn == nullptr || p->gpsc == nullptr || masks == nullptr || gw == nullptr)
+        return fail("btx_adj_lif_chunk: gsyn, gpsc, masks and gw are required");
+    const int64_t m = p->n * p->lanes;
+    btx_adj_lif_args s = *p;
+    for (int64_t t = K - 1; t >= 0; --t) {
+        // dL/dpsc after step t through the transposed product, the weight
+        // gradient of step t's spikes, then the elementwise reverse step
+        if (int rc = btx_adj_transpose(row_ptr, col, w, p->n, p->lanes, p->gpsc,
+                                       (float*)p->gsyn, stream))
+            return rc;
+        if (int rc = btx_adj_wgrad(row_ptr, col, p->n, p->lanes, masks + t * p->n, p->gpsc, gw,
+                                   stream))
+            return rc;
+        s.v_prev = p->v_prev + t * m;
+        s.v_pre = p->v_pre + t * m;
+        s.z = p->z + t * m;
+        s.ref_prev = p->ref_prev ? p->ref_prev + t * m : nullptr;
+        s.gz = p->gz + t * m;
+        s.gx = p->gx + t * m;
+        if (int rc = btx_adj_lif_step(&s, stream)) return rc;
+    }
+    return 0;
+}
+
 BTX_API int btx_adj_transpose_delayed(const int64_t* row_ptr, const int32_t* col, const float* w,
                                       int64_t n, int csr_d, int64_t lanes, const float* G,
                                       int64_t K, int64_t tau, const int* ages, float* g_src,

@@ -164,6 +164,16 @@ def adjoint_chunk(spec, traj: dict, grads: dict, w_csr: torch.Tensor, dt: float,
             raise NotImplementedError("the relaxed weight gradient covers undelayed synapses")
         return _adjoint_delayed(spec, a, lib, stream, layout, w, masks, zz, v_prev, v_pre,
                                 ref_prev, gz, gv, gpsc, gsyn, gx, gw, gtau, past, shifts)
+    if not valued_weights:
+        # the whole reverse sweep in one call
+        a.v_prev, a.v_pre, a.z = v_prev.data_ptr(), v_pre.data_ptr(), zz.data_ptr()
+        a.ref_prev = ref_prev.data_ptr() if ref_prev is not None else None
+        a.gz, a.gx = gz.data_ptr(), gx.data_ptr()
+        _lib.check(lib.btx_adj_lif_chunk(a, K, layout.row_ptr.data_ptr(),
+                                         layout.col.data_ptr(), w.data_ptr(), masks.data_ptr(),
+                                         gw.data_ptr(), stream), "btx_adj_lif_chunk")
+        return {"gx": gx.permute(0, 2, 1).contiguous(), "gw": gw, "gtau": gtau,
+                "gv": gv.t().contiguous(), "gpsc": gpsc.t().contiguous()}
     for t in range(K - 1, -1, -1):
         _lib.check(lib.btx_adj_transpose(layout.row_ptr.data_ptr(), layout.col.data_ptr(),
                                          w.data_ptr(), n, B, gpsc.data_ptr(),

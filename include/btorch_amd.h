@@ -538,6 +538,17 @@ typedef struct btx_adj_lif_args {
 
 int btx_adj_lif_step(const btx_adj_lif_args* args, void* stream);
 
+/* The reverse sweep of a K-step chunk of an undelayed LIF + exponential
+ * synapse in one call (the backward of the reference's checkpointed chunk,
+ * rnn.py:177-197, through the eager step's autograd chain, lif.py:144-187,
+ * synapse.py:171-178): for t = K-1 .. 0, btx_adj_transpose of args->gpsc into
+ * args->gsyn, btx_adj_wgrad of masks[t] (n words) into gw, then
+ * btx_adj_lif_step.  v_prev, v_pre, z, ref_prev, gz and gx of args are the
+ * bases of (K, n, lanes) arrays; gv, gpsc and gtau carry across the steps. */
+int btx_adj_lif_chunk(const btx_adj_lif_args* args, int64_t K, const int64_t* row_ptr,
+                      const int32_t* col, const float* w, const uint64_t* masks, float* gw,
+                      void* stream);
+
 /* g_src[j, :] = sum over CSR row j of w[e] * g_dst[col[e], :] (every row;
  * (n, lanes) layout).  The transposed product of the backward pass. */
 int btx_adj_transpose(const int64_t* row_ptr, const int32_t* col, const float* w,
