@@ -931,7 +931,7 @@ __device__ __forceinline__ bool any4(const float* a) {
 }
 
 template <int KIND_SX>
-__device__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lane) {
+__device__ __forceinline__ void f_role(const StepArgs& a, StepLds& lds, int64_t seg, int64_t lane) {
     constexpr int KIND = KIND_SX & ~kSynX;
     constexpr bool SX = (KIND_SX & kSynX) != 0;
     if (a.lane_t_end != nullptr && a.in_t0[lane] + a.t >= a.lane_t_end[lane]) {
@@ -1421,6 +1421,11 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 // Kernels
 // ---------------------------------------------------------------------------
 
+// The roles are inlined (the F role always, the P role by its call for the
+// larger neuron kinds): a role left out of line (the compiler's choice for
+// those kinds) takes its argument block by reference, and the whole block
+// (~1 KB) is then copied into scratch memory for every lane, which slows the
+// launch several times.
 // Workgroups [0, nc) of a row are the control column (nc = 0 or 1; lane 0's
 // sums the slot totals and decides the next push, the others end at once),
 // [nc, nc + np) play P (the longer chain, dispatched first), the next nh
@@ -1440,7 +1445,13 @@ __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArg
     if (x < 0) {
         if (ORDERED && lane == 0) control_role(ca);
     } else if (x < np + nh) {
-        p_role<G, ORDERED>(p, lds.push, x, lane);
+        if constexpr (KIND == kLif) {
+            p_role<G, ORDERED>(p, lds.push, x, lane);
+        } else {
+            // the larger kinds: the compiler would leave the P role out of line
+            // and copy its argument block to scratch (see f_role above)
+            [[clang::always_inline]] p_role<G, ORDERED>(p, lds.push, x, lane);
+        }
     } else {
         f_role<KIND>(a, lds, (int64_t)x - np - nh, lane);
     }
