@@ -1426,13 +1426,16 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 // those kinds) takes its argument block by reference, and the whole block
 // (~1 KB) is then copied into scratch memory for every lane, which slows the
 // launch several times.
-// Workgroups [0, nc) of a row are the control column (nc = 0 or 1; lane 0's
-// sums the slot totals and decides the next push, the others end at once),
-// [nc, nc + np) play P (the longer chain, dispatched first), the next nh
-// (= p.nh; a scalar argument of its own, like np, so that the role choice
-// reads no argument block) only join a dense pull (a launch of few lanes and segments has too few P
-// workgroups to pull every destination; they end at once after a push), the
-// rest F.
+// A row of workgroups per lane: [0, nc) is the control column (nc = 0 or 1;
+// lane 0's sums the slot totals and decides the next push, the others end at
+// once), [nc, nc + np) play P, the next nh (= p.nh; a scalar argument of its
+// own, like np, so that the role choice reads no argument block) only join a
+// dense pull (a launch of few lanes and segments has too few P workgroups to
+// pull every destination; they end at once after a push), the rest F.  The
+// grid is (lanes, row positions) with the positions reversed, so the F
+// workgroups of every lane are dispatched first, then the P ones: a launch
+// has more workgroups than the GPU holds at once, and those that wait for a
+// slot then run the shorter P chain.
 template <int G, bool ORDERED, int KIND>
 __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArgs a, PushStep p,
                                                                           ControlArgs ca, int np,
@@ -1440,8 +1443,11 @@ __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArg
     // KIND: kLif, kGlif3, kTwoComp or kMixed, with kSynX for the alpha and
     // dual-exponential synapses
     __shared__ StepLds lds;
-    const int64_t lane = blockIdx.y;
-    const int x = (int)blockIdx.x - nc;
+    // grid (lanes, columns), dispatched lane fastest; column c plays the row
+    // position gridDim.y - 1 - c, so the F workgroups of every lane are
+    // dispatched first, then the P workgroups, the control one last
+    const int64_t lane = blockIdx.x;
+    const int x = (int)(gridDim.y - 1 - blockIdx.y) - nc;
     if (x < 0) {
         if (ORDERED && lane == 0) control_role(ca);
     } else if (x < np + nh) {
@@ -1790,7 +1796,7 @@ BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float
     a.xa = (a.kind == kTwoComp || a.kind == kMixed) ? p->x_apical : nullptr;
     a.vec4 = vec4_ok(p, x);
     a.rec_any = set_rec_any(a);
-    const dim3 grid((unsigned)a.nchunks, (unsigned)p->lanes);
+    const dim3 grid((unsigned)p->lanes, (unsigned)a.nchunks);
     const hipStream_t s = (hipStream_t)stream;
     if (a.kind == kGlif3) launch_neuron_only<kGlif3>(a, grid, s);
     else if (a.kind == kTwoComp) launch_neuron_only<kTwoComp>(a, grid, s);
@@ -2209,18 +2215,18 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         ps.pull_acc = cur.pull_acc ? (float*)cur.pull_acc : nullptr;
         if (fused) {
             kt.begin(0);
-            const dim3 grid((unsigned)(nc + npb + ps.nh + nf), (unsigned)lanes);
+            const dim3 grid((unsigned)lanes, (unsigned)(nc + npb + ps.nh + nf));
             launch_step(p->ordered, a, ps, ca, npb, nc, group, grid, s);
             kt.end(0);
         } else {
             kt.begin(0);
-            const dim3 fgrid((unsigned)nf, (unsigned)lanes);
+            const dim3 fgrid((unsigned)lanes, (unsigned)nf);
             PushStep fonly = ps;  // the F launch has no pull-only workgroups
             fonly.nh = 0;
             launch_step(p->ordered, a, fonly, ca, 0, 0, group, fgrid, s);
             kt.end(0);
             kt.begin(2);
-            const dim3 pgrid((unsigned)(nc + npb + ps.nh), (unsigned)lanes);
+            const dim3 pgrid((unsigned)lanes, (unsigned)(nc + npb + ps.nh));
             launch_step(p->ordered, a, ps, ca, npb, nc, group, pgrid, s);
             kt.end(2);
         }
