@@ -383,17 +383,19 @@ def main():
     pct = {k[:-5]: [round(float(np.percentile(v, q)) * 1e3, 2) for q in (50, 90, 99)]
            for k, v in timings.items() if k.endswith("_each") and v}
     nwords = (n + 63) // 64
-    # algorithmic bytes of one step launch over the timed region: F reads
-    # v, psc and refractory and the input x and writes v, psc and the spike
-    # record z (28 B per neuron-lane; the refractory counter is written back
-    # only where it changes, a few per cent of neuron-lanes, not counted),
+    # algorithmic bytes of one step launch: F reads v, psc and refractory
+    # and the input x and writes v and psc (24 B per neuron-lane; the
+    # refractory counter is written back only where it changes, a few per
+    # cent of neuron-lanes, not counted),
     # per-neuron tau, spike words to the raster and the ring, ring indices of
     # the spikes; the previous push's term records read (10 B per term: row,
     # weight, destination).  P: per active row the index and the int64
     # row_ptr pair (20 B), per term col + weight read and the term record
     # written (18 B).
     def step_bytes(spikes, rows, events, steps):
-        return (28.0 * n * lanes + 4.0 * n + 16.0 * nwords * lanes + 4.0 * spikes / steps
+        # (the spike record is unpacked from the raster after the run: 4 B per
+        # neuron-lane more for the call)
+        return (24.0 * n * lanes + 4.0 * n + 16.0 * nwords * lanes + 4.0 * spikes / steps
                 + 20.0 * rows / steps + 28.0 * events / steps)
 
     # the kernel: algorithmic bytes of a k_step launch over the mean duration
@@ -403,7 +405,7 @@ def main():
     step_us = per["step"][0] * 1e3 if per.get("step") else float("nan")
     achieved = bytes_k / (step_us * 1e-6)
     # the call: the timed model(x) call's region over its steps
-    bytes_launch = step_bytes(spikes_t, rows_t, ev_t, K)
+    bytes_launch = step_bytes(spikes_t, rows_t, ev_t, K) + 4.0 * n * lanes
     call_us = region_ms / K * 1e3
     traffic = _traffic(args)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,

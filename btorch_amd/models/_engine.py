@@ -633,7 +633,12 @@ class Engine:
                 raise ValueError("teacher must be contiguous int64 (T, lanes, ceil(n/64))")
             p.teacher = teacher.data_ptr()
         recs = {}
-        for key in ("z", "v", "refractory", "psc", "v_pre", "v_prev", "i_a", "i_bap",
+        # the spike record is not written by the step: it is the raster's
+        # bits, unpacked after the run by one streaming kernel, so the steps
+        # move 4 B less per neuron-lane (and without other records run the
+        # record-free variant)
+        want_z = "z" in record
+        for key in ("v", "refractory", "psc", "v_pre", "v_prev", "i_a", "i_bap",
                     "theta_th", "syn_h1", "syn_h2"):
             if key in record:
                 recs[key] = torch.empty((T, lanes, n), dtype=torch.float32, device=dev)
@@ -643,7 +648,7 @@ class Engine:
                         mixed and sp.neuron.has_tc):
                     raise ValueError(f"'{key}' is a two-compartment state")
                 setattr(p, "rec_" + key, recs[key].data_ptr())
-        p.rec_z = recs["z"].data_ptr() if "z" in recs else None
+        p.rec_z = None
         p.rec_v = recs["v"].data_ptr() if "v" in recs else None
         p.rec_refractory = recs["refractory"].data_ptr() if "refractory" in recs else None
         p.rec_psc = recs["psc"].data_ptr() if "psc" in recs else None
@@ -673,6 +678,11 @@ class Engine:
                 timings[name] = (float(ms[k]), int(calls[k]))
                 timings[name + "_each"] = [per[k * cap + i] for i in range(min(calls[k], cap))]
         self._tables_ok.add(tables)
+        if want_z:
+            from .._kernels import unpack_rows
+            z = torch.empty((T, lanes, n), dtype=torch.float32, device=dev)
+            unpack_rows(raster.view(T * lanes, nwords), n, out=z.view(T * lanes, n))
+            recs["z"] = z
         if mixed:
             sp.neuron.scatter()
         ring.cursor = (ring.cursor + T) % ring.R

@@ -96,7 +96,9 @@ class _GraphEntry:
         self.recs = {}
         for key in record:
             self.recs[key] = torch.empty((T, lanes, n), dtype=torch.float32, device=dev)
-        for key, field in (("z", "rec_z"), ("v", "rec_v"), ("refractory", "rec_refractory"),
+        # the spike record is the raster's bits, unpacked in the graph after
+        # the run (as Engine.run does)
+        for key, field in (("v", "rec_v"), ("refractory", "rec_refractory"),
                            ("psc", "rec_psc"), ("v_pre", "rec_v_pre"), ("v_prev", "rec_v_prev"),
                            ("i_a", "rec_i_a"), ("i_bap", "rec_i_bap"),
                            ("theta_th", "rec_theta_th"), ("syn_h1", "rec_syn_h1"),
@@ -113,6 +115,11 @@ class _GraphEntry:
     def _device_work(self) -> None:
         _lib.check(_lib.load().btx_rsnn_run(self.plan, self.T, _lib.stream_ptr()),
                    "btx_rsnn_run")
+        if "z" in self.recs:
+            from .._kernels import unpack_rows
+            n = self.raster.shape[0] * self.raster.shape[1]
+            unpack_rows(self.raster.view(n, -1), self.recs["z"].shape[-1],
+                        out=self.recs["z"].view(n, -1))
         # slot ring_d of tmp_words stays zero: the next run writes it first
         w = self.ring.words
         self.tmp_words.index_copy_(0, self.rot_dst, w.index_select(0, self.rot_src))
