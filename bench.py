@@ -169,7 +169,7 @@ def _traffic(args):
     cfg = d.get("config", {})
     if (cfg.get("n"), cfg.get("lanes"), cfg.get("kmin"), cfg.get("w_mu"), cfg.get("mode"),
             cfg.get("input")) != (args.n, args.lanes, args.kmin, args.w_mu, args.mode,
-                                  "x tensor, dense spike record (model(x))"):
+                                  "x tensor, record-free step (model(x))"):
         return None
     return {"bytes_per_launch": d.get("bytes_per_launch"),
             "source": "profiles/k_step_140k_8lanes_traffic.txt: rocprofv3 FETCH_SIZE and "
@@ -342,11 +342,14 @@ def main():
         torch.cuda.synchronize()
         first_call_ms = 1e3 * (time.perf_counter() - t_first)
         # snapshot of the state entering the timed region (for the CPU
-        # sample); reading the history writes it back here, before the clock
-        snap = {"v": model.neuron.v.detach().cpu().clone(),
-                "refractory": model.neuron.refractory.detach().cpu().clone(),
-                "psc": model.synapse.base_psc.psc.detach().cpu().clone(),
-                "history": model.synapse.history.history.detach().cpu().clone(),
+        # sample); reading the history writes it back here, before the clock.
+        # The copies stay on the device until the region has ended: host
+        # copies here would leave the GPU idle and the runtime's staging
+        # state changed just before the clock
+        snap = {"v": model.neuron.v.detach().clone(),
+                "refractory": model.neuron.refractory.detach().clone(),
+                "psc": model.synapse.base_psc.psc.detach().clone(),
+                "history": model.synapse.history.history.detach().clone(),
                 "cursor": model.synapse.history.cursor}
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -359,6 +362,7 @@ def main():
         elapsed = time.perf_counter() - t0
         region_ms = ev0.elapsed_time(ev1)
         r_t = words(z_t)
+        snap = {k: (v.cpu() if torch.is_tensor(v) else v) for k, v in snap.items()}
         z_cpu_sample = z_t[:args.cpu_steps].cpu() if not args.no_cpu else None
         del z_t
         # the same launches (input x, dense spike record) with a HIP event

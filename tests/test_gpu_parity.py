@@ -56,6 +56,28 @@ def test_compaction_dense_and_bits_bit_exact(lanes, n, density):
         assert np.array_equal(idx2[b, :c].cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("rows,n,offset,pad", [(3, 1, 0, 0), (5, 4093, 0, 0), (7, 4099, 1, 0),
+                                               (3, 8196, 0, 0), (4, 70, 0, 2), (70001, 67, 0, 0),
+                                               (66000, 68, 0, 0)])
+def test_pack_unpack_rows_round_trip(rows, n, offset, pad):
+    """Bits of each row against numpy's packing; unpack through the 16-byte
+    store path (16-byte aligned rows, with a partial last group when rows are
+    padded), the per-float path (rows one float off alignment, or a stride
+    not a multiple of 4) and the row chunks above 65535 rows."""
+    rng = np.random.default_rng(rows + n)
+    x = (rng.random((rows, n)) < 0.3).astype(np.float32)
+    words = _kernels.pack_rows(torch.from_numpy(x).to(DEV))
+    ref = np.packbits(x.astype(bool), axis=1, bitorder="little")
+    ref = np.pad(ref, ((0, 0), (0, words.shape[1] * 8 - ref.shape[1])))
+    assert np.array_equal(words.cpu().numpy().view(np.uint8).reshape(rows, -1), ref)
+    buf = torch.full((rows * (n + pad) + offset,), 7.0, device=DEV)
+    out = buf[offset:].view(rows, n + pad)[:, :n]
+    _kernels.unpack_rows(words, n, out=out)
+    assert torch.equal(out.cpu(), torch.from_numpy(x))
+    rest = buf[offset:].view(rows, n + pad)[:, n:]
+    assert bool((rest == 7.0).all()) and (offset == 0 or float(buf[0]) == 7.0)
+
+
 # ---------------------------------------------------------------------------
 # event-driven product
 # ---------------------------------------------------------------------------
