@@ -337,10 +337,22 @@ def main():
     # The timed region is one call of the module, model(x) -> (z, states),
     # with x resident in HBM: the drop-in API a btorch user calls.
     with environ.context(dt=0.1), torch.no_grad():
+        # the W warm-up steps run as two calls (W - 1 steps, then 1): the
+        # first builds the plan, workspace and delay ring, the second takes
+        # the path every later call takes, so the timed call is not the
+        # first run of that host code
         t_first = time.perf_counter()
-        r_w = words(model(x_w)[0]) if W else None
+        parts = [x_w[:W - 1], x_w[W - 1:]] if W >= 2 else [x_w] if W else []
+        r_parts = []
+        for i, xp in enumerate(parts):
+            r_parts.append(words(model(xp)[0]))
+            if i == 0:
+                torch.cuda.synchronize()
+                first_call_ms = 1e3 * (time.perf_counter() - t_first)
+        r_w = torch.cat(r_parts, 0) if r_parts else None
+        if not parts:
+            first_call_ms = 0.0
         torch.cuda.synchronize()
-        first_call_ms = 1e3 * (time.perf_counter() - t_first)
         # snapshot of the state entering the timed region (for the CPU
         # sample); reading the history writes it back here, before the clock.
         # The copies stay on the device until the region has ended: host
@@ -429,8 +441,10 @@ def main():
                                  "and the final reduction launch)"}}
     calls = {"host_enqueue_ms": round(host_ms, 3), "first_call_ms": round(first_call_ms, 2),
              "what": "host_enqueue_ms: wall time of the timed model(x) call until it "
-                     "returned (work enqueued); first_call_ms: the warm-up call, which "
-                     "builds the plan, workspace and delay ring"}
+                     "returned (work enqueued); first_call_ms: the first warm-up call "
+                     "(W - 1 steps), which builds the plan, workspace and delay ring; "
+                     "the last warm-up step runs as a call of its own",
+             "warmup_calls": [int(p.shape[0]) for p in parts]}
     _progress(f"kernel: {ms_step:.4f} ms per step")
     sweep = activity_sweep(args, model, eng, deg, dev) if args.sweep else None
     _progress("activity sweep done")

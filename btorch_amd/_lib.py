@@ -259,17 +259,25 @@ def check(rc: int, what: str) -> None:
 
 def require_hip(*tensors: torch.Tensor) -> None:
     """Raise unless every tensor lives on a HIP device."""
+    seen = False
     for t in tensors:
-        if t is not None and t.device.type != "cuda":
-            raise RuntimeError(
-                "btorch_amd runs its kernels on a HIP device only; got a tensor on "
-                f"{t.device}. Move the module and inputs to 'cuda'."
-            )
-    if not torch.cuda.is_available():
+        if t is not None:
+            if t.device.type != "cuda":
+                raise RuntimeError(
+                    "btorch_amd runs its kernels on a HIP device only; got a tensor on "
+                    f"{t.device}. Move the module and inputs to 'cuda'."
+                )
+            seen = True
+    # a tensor on a HIP device already shows that one is visible
+    if not seen and not torch.cuda.is_available():
         raise RuntimeError("btorch_amd: no HIP device is visible")
 
 
 def stream_ptr(device: torch.device | None = None) -> int:
+    """The current HIP stream of `device` (default: the current device)."""
+    if device is None:
+        # torch's raw-stream query, without building a Stream object
+        return torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice())
     return torch.cuda.current_stream(device).cuda_stream
 
 

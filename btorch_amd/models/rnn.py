@@ -26,6 +26,26 @@ from .functional import filter_hidden_states, hidden_state_names, prepare_sparse
 from .linear import BaseSparseConn
 
 
+
+_MISSING = object()
+
+
+def _attr(m, name: str):
+    """m.name, or None when m is None or has no such attribute (instance
+    entries and nn.Module's parameter, buffer and submodule tables first;
+    class attributes such as properties through getattr)."""
+    if m is None:
+        return None
+    d = m.__dict__
+    v = d.get(name, _MISSING)
+    if v is not _MISSING:
+        return v
+    for table in ("_modules", "_parameters", "_buffers"):
+        t = d.get(table)
+        if t is not None and name in t:
+            return t[name]
+    return getattr(m, name, None) if hasattr(type(m), name) else None
+
 class RecurrentNNAbstract(base.MemoryModule):
     """Loop driver; subclasses define ``single_step_forward -> (z, states)``."""
 
@@ -196,16 +216,17 @@ class RecurrentNN(RecurrentNNAbstract):
     def _engine_identity(self) -> tuple:
         """Modules and layout tensors a matched CellSpec froze: a change of
         any of them (a new connection, a reloaded layout, a new history
-        module) matches the cell again."""
-        syn = self.synapse
-        inner = getattr(syn, "psc_module", None) or getattr(syn, "base_psc", None)
-        conn = getattr(syn, "linear", None) or getattr(inner, "linear", None)
-        hist = getattr(syn, "history", None)
-        nrn = self.neuron
-        groups = tuple(id(g) for g in nrn.groups()) if hasattr(nrn, "groups") else ()
-        return (id(nrn), groups, id(syn), id(inner), id(self.syn_inp_module), id(conn),
-                id(getattr(conn, "_csr_row_ptr", None)), id(hist),
-                id(getattr(self, "synapse_apical", None)))
+        module) matches the cell again.  Read on every call, so attributes
+        are looked up without the exception a missing nn.Module attribute
+        raises."""
+        syn = _attr(self, "synapse")
+        inner = _attr(syn, "psc_module") or _attr(syn, "base_psc")
+        conn = _attr(syn, "linear") or _attr(inner, "linear")
+        hist = _attr(syn, "history")
+        nrn = _attr(self, "neuron")
+        groups = tuple(id(g) for g in nrn.groups()) if hasattr(type(nrn), "groups") else ()
+        return (id(nrn), groups, id(syn), id(inner), id(_attr(self, "syn_inp_module")), id(conn),
+                id(_attr(conn, "_csr_row_ptr")), id(hist), id(_attr(self, "synapse_apical")))
 
     def fused_engine(self) -> Engine | None:
         key = self._engine_identity()
