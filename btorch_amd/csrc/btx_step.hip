@@ -1431,23 +1431,25 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
 // once), [nc, nc + np) play P, the next nh (= p.nh; a scalar argument of its
 // own, like np, so that the role choice reads no argument block) only join a
 // dense pull (a launch of few lanes and segments has too few P workgroups to
-// pull every destination; they end at once after a push), the rest F.  The
-// grid is (lanes, row positions) with the positions reversed, so the F
-// workgroups of every lane are dispatched first, then the P ones: a launch
-// has more workgroups than the GPU holds at once, and those that wait for a
-// slot then run the shorter P chain.
-template <int G, bool ORDERED, int KIND>
+// pull every destination; they end at once after a push), the rest F.
+template <int G, bool ORDERED, int KIND, bool TR>
 __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArgs a, PushStep p,
                                                                           ControlArgs ca, int np,
                                                                           int nc, int nh) {
     // KIND: kLif, kGlif3, kTwoComp or kMixed, with kSynX for the alpha and
     // dual-exponential synapses
     __shared__ StepLds lds;
-    // grid (lanes, columns), dispatched lane fastest; column c plays the row
-    // position gridDim.y - 1 - c, so the F workgroups of every lane are
-    // dispatched first, then the P workgroups, the control one last
-    const int64_t lane = blockIdx.x;
-    const int x = (int)(gridDim.y - 1 - blockIdx.y) - nc;
+    // Row positions of a lane: the control workgroup (lane 0 of position 0
+    // when nc = 1), the P workgroups, then the F workgroups.  TR (a grid the
+    // GPU holds at once): the grid is (lanes, positions) with the positions
+    // reversed, so, dispatched lane fastest, every lane's F workgroups go
+    // out first.  Otherwise (positions, lanes): one lane's row after
+    // another, so the P and F work of the lanes dispatched first overlaps
+    // (measured on 256-lane pools: lane-fastest dispatch cost 25% of the
+    // pool's rate).  A compile-time choice: picking blockIdx.x or .y at run
+    // time made the compiler spill scalar registers to scratch.
+    const int64_t lane = TR ? blockIdx.x : blockIdx.y;
+    const int x = (TR ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.x) - nc;
     if (x < 0) {
         if (ORDERED && lane == 0) control_role(ca);
     } else if (x < np + nh) {
@@ -1780,8 +1782,8 @@ static int vec4_ok(const btx_rsnn_plan* p, const float* x) {
 template <int KIND>
 static void launch_neuron_only(const StepArgs& a, dim3 grid, hipStream_t s) {
     PushStep none{};
-    hipLaunchKernelGGL((k_step<16, true, KIND>), grid, dim3(kThreads), 0, s, a, none, ControlArgs{}, 0,
-                       0, 0);
+    hipLaunchKernelGGL((k_step<16, true, KIND, false>), grid, dim3(kThreads), 0, s, a, none,
+                       ControlArgs{}, 0, 0, 0);
 }
 
 BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float* v_pre,
@@ -1796,7 +1798,7 @@ BTX_API int btx_lif_step(const btx_rsnn_plan* p, const float* x, float* z, float
     a.xa = (a.kind == kTwoComp || a.kind == kMixed) ? p->x_apical : nullptr;
     a.vec4 = vec4_ok(p, x);
     a.rec_any = set_rec_any(a);
-    const dim3 grid((unsigned)p->lanes, (unsigned)a.nchunks);
+    const dim3 grid((unsigned)a.nchunks, (unsigned)p->lanes);  // (positions, lanes)
     const hipStream_t s = (hipStream_t)stream;
     if (a.kind == kGlif3) launch_neuron_only<kGlif3>(a, grid, s);
     else if (a.kind == kTwoComp) launch_neuron_only<kTwoComp>(a, grid, s);
@@ -1906,14 +1908,49 @@ struct KernelTimer {
 
 static int rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s, KernelTimer& kt);
 
+// Workgroups of k_step<G, ORDERED, KIND, true> the device holds at once
+// (its occupancy per CU times the CU count), queried on first use.
+template <int G, bool ORDERED, int KIND>
+static int64_t resident_wgs() {
+    static const int64_t n = [] {
+        int blocks = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &blocks, reinterpret_cast<const void*>(&k_step<G, ORDERED, KIND, true>), kThreads,
+                0) != hipSuccess)
+            blocks = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 0;
+        return (int64_t)blocks * cus;
+    }();
+    return n;
+}
+
+// `grid` is (lanes, row positions).  The LIF step launches it as it is
+// (positions reversed) when every workgroup is resident at once; otherwise,
+// and for the other kinds, it is launched transposed.
+template <int G, bool ORDERED, int KIND>
+static void launch_step_one(const StepArgs& a, const PushStep& ps, const ControlArgs& ca, int np,
+                            int nc, dim3 grid, hipStream_t s) {
+    if constexpr (KIND == kLif) {
+        if ((int64_t)grid.x * grid.y <= resident_wgs<G, ORDERED, KIND>()) {
+            hipLaunchKernelGGL((k_step<G, ORDERED, KIND, true>), grid, dim3(kThreads), 0, s, a,
+                               ps, ca, np, nc, ps.nh);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_step<G, ORDERED, KIND, false>), dim3(grid.y, grid.x), dim3(kThreads), 0,
+                       s, a, ps, ca, np, nc, ps.nh);
+}
+
 template <bool ORDERED, int KIND>
 static void launch_step_g(const StepArgs& a, const PushStep& ps, const ControlArgs& ca, int np,
                           int nc, int group, dim3 grid, hipStream_t s) {
     switch (group) {
-        case 8: hipLaunchKernelGGL((k_step<8, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc, ps.nh); break;
-        case 16: hipLaunchKernelGGL((k_step<16, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc, ps.nh); break;
-        case 32: hipLaunchKernelGGL((k_step<32, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc, ps.nh); break;
-        default: hipLaunchKernelGGL((k_step<64, ORDERED, KIND>), grid, dim3(kThreads), 0, s, a, ps, ca, np, nc, ps.nh); break;
+        case 8: launch_step_one<8, ORDERED, KIND>(a, ps, ca, np, nc, grid, s); break;
+        case 16: launch_step_one<16, ORDERED, KIND>(a, ps, ca, np, nc, grid, s); break;
+        case 32: launch_step_one<32, ORDERED, KIND>(a, ps, ca, np, nc, grid, s); break;
+        default: launch_step_one<64, ORDERED, KIND>(a, ps, ca, np, nc, grid, s); break;
     }
 }
 

@@ -239,6 +239,26 @@ def unpack_bits(words: torch.Tensor, n: int) -> torch.Tensor:
     return bits.reshape(*words.shape[:-1], words.shape[-1] * 64)[..., :n].to(torch.float32)
 
 
+_MISSING = object()
+
+
+def _attr(m, name: str):
+    """m.name, or None when m is None or has no such attribute (instance
+    entries and nn.Module's parameter, buffer and submodule tables first;
+    class attributes such as properties through getattr)."""
+    if m is None:
+        return None
+    d = m.__dict__
+    v = d.get(name, _MISSING)
+    if v is not _MISSING:
+        return v
+    for table in ("_modules", "_parameters", "_buffers"):
+        t = d.get(table)
+        if t is not None and name in t:
+            return t[name]
+    return getattr(m, name, None) if hasattr(type(m), name) else None
+
+
 def _tkey(t):
     """Identity and in-place version of a tensor (None passes through)."""
     if t is None or not torch.is_tensor(t):
@@ -333,14 +353,15 @@ class Engine:
                  id(conn._prepared_sparse_weight)]
         mods = (sp.neuron.modules() if isinstance(sp.neuron, MixedState) else [sp.neuron])
         for m in (*mods, sp.exp_psc):
-            mem = getattr(m, "_memories_rv", {})
-            for name, t in m._parameters.items():
+            d = m.__dict__
+            mem = d.get("_memories_rv") or ()
+            for name, t in d["_parameters"].items():
                 parts.append((name, _tkey(t)))
-            for name, t in m._buffers.items():
+            for name, t in d["_buffers"].items():
                 if name not in mem:
                     parts.append((name, _tkey(t)))
             for name in self._SCALAR_ATTRS:
-                parts.append(getattr(m, name, None))
+                parts.append(_attr(m, name))
         return tuple(parts)
 
     def dense_pull(self) -> bool:

@@ -38,7 +38,7 @@ def lane_state(module, name: str, shape: tuple[int, ...]) -> torch.Tensor:
     meeting a batched input) is broadcast into a fresh copy; one that cannot
     broadcast, or does not hold ``prod(shape)`` elements afterwards, raises
     before any kernel sees it."""
-    t = getattr(module, name, None)
+    t = orig = getattr(module, name, None)
     if t is None:
         raise RuntimeError(f"{type(module).__name__}.{name} is not initialised "
                            "(call init_net_state)")
@@ -55,7 +55,8 @@ def lane_state(module, name: str, shape: tuple[int, ...]) -> torch.Tensor:
     if t.numel() != n:
         raise RuntimeError(f"{type(module).__name__}.{name} holds {t.numel()} values, the "
                            f"launch covers {n}")
-    setattr(module, name, t)
+    if t is not orig:
+        setattr(module, name, t)
     return t
 
 

@@ -21,30 +21,10 @@ import torch
 import torch.nn as nn
 
 from . import base
-from ._engine import Engine, MixedState, match_cell
+from ._engine import Engine, MixedState, _attr, match_cell
 from .functional import filter_hidden_states, hidden_state_names, prepare_sparse_modules
 from .linear import BaseSparseConn
 
-
-
-_MISSING = object()
-
-
-def _attr(m, name: str):
-    """m.name, or None when m is None or has no such attribute (instance
-    entries and nn.Module's parameter, buffer and submodule tables first;
-    class attributes such as properties through getattr)."""
-    if m is None:
-        return None
-    d = m.__dict__
-    v = d.get(name, _MISSING)
-    if v is not _MISSING:
-        return v
-    for table in ("_modules", "_parameters", "_buffers"):
-        t = d.get(table)
-        if t is not None and name in t:
-            return t[name]
-    return getattr(m, name, None) if hasattr(type(m), name) else None
 
 class RecurrentNNAbstract(base.MemoryModule):
     """Loop driver; subclasses define ``single_step_forward -> (z, states)``."""
