@@ -1448,8 +1448,27 @@ __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArg
     // (measured on 256-lane pools: lane-fastest dispatch cost 25% of the
     // pool's rate).  A compile-time choice: picking blockIdx.x or .y at run
     // time made the compiler spill scalar registers to scratch.
-    const int64_t lane = TR ? blockIdx.x : blockIdx.y;
-    const int x = (TR ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.x) - nc;
+    int64_t lane;
+    int x;
+    if constexpr (TR) {
+        lane = blockIdx.x;
+        x = (int)(gridDim.y - 1 - blockIdx.y) - nc;
+    } else {
+        // workgroup i of the dispatch runs on XCD i mod 8: with lanes a
+        // multiple of 8, XCD k takes the rows of lanes k, k + 8, ... in
+        // order, so a lane's state, bins and ring stay in one XCD's L2
+        const uint32_t cols = gridDim.x;
+        if ((gridDim.y & 7u) == 0) {
+            const uint32_t i = blockIdx.x + blockIdx.y * cols;
+            const uint32_t slot = i >> 3;
+            const uint32_t q = slot / cols;
+            lane = (int64_t)(q * 8u + (i & 7u));
+            x = (int)(slot - q * cols) - nc;
+        } else {
+            lane = blockIdx.y;
+            x = (int)blockIdx.x - nc;
+        }
+    }
     if (x < 0) {
         if (ORDERED && lane == 0) control_role(ca);
     } else if (x < np + nh) {
