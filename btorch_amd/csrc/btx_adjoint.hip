@@ -108,30 +108,63 @@ __global__ __launch_bounds__(256) void k_adj_lif(AdjArgs a) {
 
 // g_src[j, b] = sum over edges e of row j of w[e] * g_dst[col[e], b]; one
 // wavefront per row, lane = batch lane (rows of more than 64 lanes loop).
+// The row's edges are read 64 at a time, lane k holding edge c + k, and
+// handed to every lane by readlane, so the g_dst loads of a chunk do not wait
+// on an index load each.  Sums: even edge offsets of the row into acc0, odd
+// ones into acc1, each in edge order, then acc0 + acc1.
 __global__ __launch_bounds__(256) void k_adj_transpose(const int64_t* __restrict__ rp,
                                                        const int32_t* __restrict__ col,
                                                        const float* __restrict__ w,
                                                        int64_t n_rows, int64_t lanes,
                                                        const float* __restrict__ g_dst,
                                                        float* __restrict__ g_src) {
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // uniform across the wavefront (four rows per workgroup)
+    const int64_t row =
+        __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
     const int ln = threadIdx.x & 63;
     if (row >= n_rows) return;
     const int64_t e0 = rp[row], e1 = rp[row + 1];
     for (int64_t b0 = 0; b0 < lanes; b0 += 64) {
         const int64_t b = b0 + ln;
         const bool on = b < lanes;
+        // lanes past `lanes` read column 0 of the row (never stored)
+        const float* gb = g_dst + (on ? b : 0);
         float acc0 = 0.f, acc1 = 0.f;
-        int64_t e = e0;
-        for (; e + 2 <= e1; e += 2) {
-            const int64_t d0 = col[e], d1 = col[e + 1];
-            const float w0 = w[e], w1 = w[e + 1];
-            if (on) {
-                acc0 += w0 * g_dst[d0 * lanes + b];
-                acc1 += w1 * g_dst[d1 * lanes + b];
+        for (int64_t c = e0; c < e1; c += 64) {
+            const int m = (int)(e1 - c < 64 ? e1 - c : 64);
+            const int my_d = ln < m ? col[c + ln] : 0;
+            const int my_w = ln < m ? __float_as_int(w[c + ln]) : 0;
+            int k = 0;
+            for (; k + 8 <= m; k += 8) {
+                // eight loads in flight, then the sums in edge order
+                float g8[8], w8[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int64_t d = __builtin_amdgcn_readlane(my_d, k + j);
+                    w8[j] = __int_as_float(__builtin_amdgcn_readlane(my_w, k + j));
+                    g8[j] = gb[d * lanes];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; j += 2) {
+                    acc0 += w8[j] * g8[j];
+                    acc1 += w8[j + 1] * g8[j + 1];
+                }
+            }
+            for (; k + 2 <= m; k += 2) {
+                const int64_t d0 = __builtin_amdgcn_readlane(my_d, k);
+                const int64_t d1 = __builtin_amdgcn_readlane(my_d, k + 1);
+                const float w0 = __int_as_float(__builtin_amdgcn_readlane(my_w, k));
+                const float w1 = __int_as_float(__builtin_amdgcn_readlane(my_w, k + 1));
+                const float g0 = gb[d0 * lanes], g1 = gb[d1 * lanes];
+                acc0 += w0 * g0;
+                acc1 += w1 * g1;
+            }
+            if (k < m) {
+                const int64_t d0 = __builtin_amdgcn_readlane(my_d, k);
+                const float w0 = __int_as_float(__builtin_amdgcn_readlane(my_w, k));
+                acc0 += w0 * gb[d0 * lanes];
             }
         }
-        if (e < e1 && on) acc0 += w[e] * g_dst[(int64_t)col[e] * lanes + b];
         if (on) g_src[row * lanes + b] = acc0 + acc1;
     }
 }
