@@ -171,29 +171,82 @@ __global__ __launch_bounds__(256) void k_adj_transpose(const int64_t* __restrict
 
 // gw[e] += sum over lanes b spiking at neuron j of g_dst[col[e], b], for
 // every edge of every neuron with a nonzero spike mask (lanes <= 64, mask
-// bit b = lane b).  One wavefront per neuron; lanes stride the edges.
+// bit b = lane b), the lanes taken in ascending order.  One wavefront per
+// neuron.  A neuron that spiked in a few lanes: lane k of the wavefront
+// takes edge c + k and gathers its destinations' values lane by lane.  In
+// more lanes: the g_dst rows of 32 edges are staged in the wavefront's LDS
+// tile with 256-byte loads (lane = batch lane), and lane k then sums row k
+// of the tile over the set bits.
+constexpr int kWgradTile = 32;
+constexpr int kWgradFew = 2;  // set bits up to which a lane gathers directly
+
 __global__ __launch_bounds__(256) void k_adj_wgrad(const int64_t* __restrict__ rp,
                                                    const int32_t* __restrict__ col,
                                                    int64_t n_rows, int64_t lanes,
                                                    const uint64_t* __restrict__ masks,
                                                    const float* __restrict__ g_dst,
                                                    float* __restrict__ gw) {
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ float tile[4][kWgradTile * 65];
+    const int wv = threadIdx.x >> 6;
+    // uniform across the wavefront (four rows per workgroup)
+    const int64_t row = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (unsigned)wv));
     const int ln = threadIdx.x & 63;
     if (row >= n_rows) return;
     const uint64_t mk = masks[row];
     if (mk == 0) return;
     const int64_t e0 = rp[row], e1 = rp[row + 1];
-    for (int64_t e = e0 + ln; e < e1; e += 64) {
-        const int64_t d = col[e];
-        float acc = 0.f;
-        uint64_t q = mk;
-        while (q) {
-            const int b = __ffsll((unsigned long long)q) - 1;
-            acc += g_dst[d * lanes + b];
-            q &= q - 1;
+    if (__popcll((unsigned long long)mk) <= kWgradFew) {
+        for (int64_t e = e0 + ln; e < e1; e += 64) {
+            const int64_t d = col[e];
+            float acc = 0.f;
+            uint64_t q = mk;
+            while (q) {
+                const int b = __ffsll((unsigned long long)q) - 1;
+                acc += g_dst[d * lanes + b];
+                q &= q - 1;
+            }
+            gw[e] += acc;
         }
-        gw[e] += acc;
+        return;
+    }
+    float* t = tile[wv];
+    const bool on = ln < lanes;
+    for (int64_t c = e0; c < e1; c += kWgradTile) {
+        const int m = (int)(e1 - c < kWgradTile ? e1 - c : kWgradTile);
+        const int my_d = ln < m ? col[c + ln] : 0;
+        int k = 0;
+        for (; k + 8 <= m; k += 8) {
+            float g8[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int64_t d = __builtin_amdgcn_readlane(my_d, k + j);
+                g8[j] = on ? g_dst[d * lanes + ln] : 0.f;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t[(k + j) * 65 + ln] = g8[j];
+        }
+        for (; k < m; ++k) {
+            const int64_t d = __builtin_amdgcn_readlane(my_d, k);
+            t[k * 65 + ln] = on ? g_dst[d * lanes + ln] : 0.f;
+        }
+        // the tile is the wavefront's own: its LDS writes land before its
+        // reads, the fences keep the compiler from moving them across
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (ln < m) {
+            float acc = 0.f;
+            uint64_t q = mk;
+            while (q) {
+                const int b = __ffsll((unsigned long long)q) - 1;
+                acc += t[ln * 65 + b];
+                q &= q - 1;
+            }
+            gw[c + ln] += acc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
