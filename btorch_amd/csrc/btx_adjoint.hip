@@ -250,6 +250,93 @@ __global__ __launch_bounds__(256) void k_adj_wgrad(const int64_t* __restrict__ r
     }
 }
 
+// k_adj_transpose and k_adj_wgrad of one reverse step in one pass (lanes <=
+// 64): each destination row of g_dst is loaded once per edge, feeds the
+// transposed sum of its lane and, when the neuron spiked in some lane, the
+// wavefront's LDS tile for the weight gradient.  Same sums in the same order
+// as the two kernels.
+__global__ __launch_bounds__(256) void k_adj_transpose_wgrad(
+    const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+    const float* __restrict__ w, int64_t n_rows, int64_t lanes,
+    const uint64_t* __restrict__ masks, const float* __restrict__ g_dst,
+    float* __restrict__ g_src, float* __restrict__ gw) {
+    __shared__ float tile[4][kWgradTile * 65];
+    const int wv = threadIdx.x >> 6;
+    const int64_t row = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (unsigned)wv));
+    const int ln = threadIdx.x & 63;
+    if (row >= n_rows) return;
+    const uint64_t mk = masks[row];
+    const int64_t e0 = rp[row], e1 = rp[row + 1];
+    const bool on = ln < lanes;
+    // lanes past `lanes` read column 0 (never stored, never summed)
+    const float* gb = g_dst + (on ? ln : 0);
+    float* t = tile[wv];
+    float acc0 = 0.f, acc1 = 0.f;
+    for (int64_t c = e0; c < e1; c += kWgradTile) {
+        const int m = (int)(e1 - c < kWgradTile ? e1 - c : kWgradTile);
+        const int my_d = ln < m ? col[c + ln] : 0;
+        const int my_w = ln < m ? __float_as_int(w[c + ln]) : 0;
+        int k = 0;
+        for (; k + 8 <= m; k += 8) {
+            float g8[8], w8[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int64_t d = __builtin_amdgcn_readlane(my_d, k + j);
+                w8[j] = __int_as_float(__builtin_amdgcn_readlane(my_w, k + j));
+                g8[j] = gb[d * lanes];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                acc0 += w8[j] * g8[j];
+                acc1 += w8[j + 1] * g8[j + 1];
+            }
+            if (mk) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) t[(k + j) * 65 + ln] = g8[j];
+            }
+        }
+        for (; k + 2 <= m; k += 2) {
+            const int64_t d0 = __builtin_amdgcn_readlane(my_d, k);
+            const int64_t d1 = __builtin_amdgcn_readlane(my_d, k + 1);
+            const float w0 = __int_as_float(__builtin_amdgcn_readlane(my_w, k));
+            const float w1 = __int_as_float(__builtin_amdgcn_readlane(my_w, k + 1));
+            const float g0 = gb[d0 * lanes], g1 = gb[d1 * lanes];
+            acc0 += w0 * g0;
+            acc1 += w1 * g1;
+            if (mk) {
+                t[k * 65 + ln] = g0;
+                t[(k + 1) * 65 + ln] = g1;
+            }
+        }
+        if (k < m) {
+            const int64_t d0 = __builtin_amdgcn_readlane(my_d, k);
+            const float w0 = __int_as_float(__builtin_amdgcn_readlane(my_w, k));
+            const float g0 = gb[d0 * lanes];
+            acc0 += w0 * g0;
+            if (mk) t[k * 65 + ln] = g0;
+        }
+        if (mk) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (ln < m) {
+                float acc = 0.f;
+                uint64_t q = mk;
+                while (q) {
+                    const int b = __ffsll((unsigned long long)q) - 1;
+                    acc += t[ln * 65 + b];
+                    q &= q - 1;
+                }
+                gw[c + ln] += acc;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    if (on) g_src[row * lanes + ln] = acc0 + acc1;
+}
+
 // Delayed synapses (HeterSynapsePSC / DelayedPSC): psc_s receives
 // W_sub^T z_{s - age(sub)} through the virtual rows j*csr_d + sub, so the
 // spike z_t of neuron j collects sum over sub of W_sub dL/dpsc_{t + age(sub)}
@@ -386,14 +473,13 @@ BTX_API int btx_adj_lif_chunk(const btx_adj_lif_args* p, int64_t K, const int64_
     const int64_t m = p->n * p->lanes;
     btx_adj_lif_args s = *p;
     for (int64_t t = K - 1; t >= 0; --t) {
-        // dL/dpsc after step t through the transposed product, the weight
-        // gradient of step t's spikes, then the elementwise reverse step
-        if (int rc = btx_adj_transpose(row_ptr, col, w, p->n, p->lanes, p->gpsc,
-                                       (float*)p->gsyn, stream))
-            return rc;
-        if (int rc = btx_adj_wgrad(row_ptr, col, p->n, p->lanes, masks + t * p->n, p->gpsc, gw,
-                                   stream))
-            return rc;
+        // dL/dpsc after step t through the transposed product and the weight
+        // gradient of step t's spikes (one pass), then the elementwise
+        // reverse step
+        hipLaunchKernelGGL(k_adj_transpose_wgrad, dim3((unsigned)((p->n + 3) / 4)), dim3(256), 0,
+                           (hipStream_t)stream, row_ptr, col, w, p->n, p->lanes,
+                           masks + t * p->n, p->gpsc, (float*)p->gsyn, gw);
+        if (int rc = check_launch("btx_adj_lif_chunk")) return rc;
         s.v_prev = p->v_prev + t * m;
         s.v_pre = p->v_pre + t * m;
         s.z = p->z + t * m;
