@@ -226,6 +226,7 @@ struct PushStep {
     int delay0;
     int sub0;
     int nh;                      // pull-only workgroups per lane row (see k_step)
+    int pull_sep;                // the dense pull runs in k_pull, not in the P role
 };
 
 // Arguments of the control workgroup, a block of their own: were they read
@@ -1222,6 +1223,13 @@ __device__ __forceinline__ uint32_t div_u31(const FastDiv& f, uint32_t x) {
 // slot that sub's age points at is set (every slot the step reads was marked
 // when it was written: the pull is only taken then).  Groups of Gp threads
 // (one per lane of a 64-lane group) walk one destination's edges together.
+// WAVE (k_pull only): a 64-lane group takes 64 of a destination's edges at a
+// time, lane k reading edge c + k and its source's lane-bit word, and the
+// wavefront walks them in order out of register lanes (three requests per
+// 64 edges instead of three per edge).  Inside k_step this form put the
+// kernel at 300-400 B of scratch per lane next to the other, so the step
+// keeps the 12-edge rounds.
+template <bool WAVE>
 __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nwg) {
     const int tid = threadIdx.x;
     const FastDiv fd = fast_div(a.csr_d);
@@ -1244,6 +1252,29 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
             float acc = 0.f;
             int64_t e = a.csc_ptr[d];
             const int64_t e1 = a.csc_ptr[d + 1];
+            if (WAVE && Gp == 64) {
+                // one destination per wavefront: e and e1 are uniform
+                const int sh = gl & 31;
+                for (int64_t c = e; c < e1; c += 64) {
+                    const int m =
+                        __builtin_amdgcn_readfirstlane((int)(e1 - c < 64 ? e1 - c : 64));
+                    const bool in = gl < m;
+                    const int r = in ? a.csc_src[c + gl] : 0;
+                    const int wi = in ? __float_as_int(a.csc_w[c + gl]) : 0;
+                    const int src = (int)div_u31(fd, (uint32_t)r);
+                    const int slot = L.slot[r - src * a.csr_d];
+                    const uint64_t wb =
+                        (in && slot >= 0) ? bits[((int64_t)slot * a.n + src) * a.lw] : 0ull;
+                    const int lo = (int)(uint32_t)wb, hi = (int)(uint32_t)(wb >> 32);
+                    for (int k = 0; k < m; ++k) {
+                        const uint32_t wl = (uint32_t)__builtin_amdgcn_readlane(lo, k);
+                        const uint32_t wh = (uint32_t)__builtin_amdgcn_readlane(hi, k);
+                        if (((gl < 32 ? wl : wh) >> sh) & 1u)
+                            acc = acc + __int_as_float(__builtin_amdgcn_readlane(wi, k));
+                    }
+                }
+                e = e1;
+            }
             // 12 edges in flight per round of dependent loads (edges, then
             // lane bits): the most that keeps the step within its registers
             constexpr int U = 12;
@@ -1333,8 +1364,10 @@ __device__ void p_role(const PushStep& a, PushLds& L, int64_t px, int64_t lane) 
     if (tid == 0) L.off[nseg] = total;
     __syncthreads();
     if (ORDERED && pull) {
-        const int64_t npp = np + a.nh;
-        pull_role(a, L, lane * npp + px, npp * a.lanes);
+        if (!a.pull_sep) {
+            const int64_t npp = np + a.nh;
+            pull_role<false>(a, L, lane * npp + px, npp * a.lanes);
+        }
         return;
     }
     if (total == 0) return;
@@ -1482,6 +1515,16 @@ __global__ __launch_bounds__(kThreads, KIND == kLif ? 7 : 1) void k_step(StepArg
     } else {
         f_role<KIND>(a, lds, (int64_t)x - np - nh, lane);
     }
+}
+
+// The dense pull of a step as a launch of its own, after the step's launch
+// (batches of more than 32 lanes): one wavefront per destination and 64-lane
+// group, four destinations per workgroup.  Ends at once unless the previous
+// launch chose the pull for this step.
+__global__ __launch_bounds__(kThreads) void k_pull(PushStep p) {
+    __shared__ PushLds L;
+    if (ld_const(&p.ctl->push_mode[p.clock & 3]) == 0) return;
+    pull_role<true>(p, L, (int64_t)blockIdx.x, (int64_t)gridDim.x);
 }
 
 // FIN: the last push reduced into the synapse (and recorded: psc in `rec`,
@@ -2196,10 +2239,14 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
     const int nc = lane_bits != nullptr ? 1 : 0;
     // pull-only workgroups: a dense pull spreads over at least kPullWgs
     ps.nh = 0;
-    if (lane_bits != nullptr) {
+    // batches of more than 32 lanes pull in k_pull (a wavefront per 64-lane
+    // group and destination)
+    ps.pull_sep = lane_bits != nullptr && lanes > 32 ? 1 : 0;
+    if (lane_bits != nullptr && !ps.pull_sep) {
         const int64_t want = (kPullWgs + lanes - 1) / lanes;
         ps.nh = want > npb ? (int)(want - npb) : 0;
     }
+    const dim3 pull_grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
     ControlArgs ca{};
     ca.ring_cnt = p->ring_cnt;
     ca.ctl = ctl;
@@ -2274,6 +2321,11 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
             const dim3 grid((unsigned)lanes, (unsigned)(nc + npb + ps.nh + nf));
             launch_step(p->ordered, a, ps, ca, npb, nc, group, grid, s);
             kt.end(0);
+            if (ps.pull_sep) {
+                kt.begin(2);
+                hipLaunchKernelGGL(k_pull, pull_grid, dim3(kThreads), 0, s, ps);
+                kt.end(2);
+            }
         } else {
             kt.begin(0);
             const dim3 fgrid((unsigned)lanes, (unsigned)nf);
@@ -2284,6 +2336,7 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
             kt.begin(2);
             const dim3 pgrid((unsigned)lanes, (unsigned)(nc + npb + ps.nh));
             launch_step(p->ordered, a, ps, ca, npb, nc, group, pgrid, s);
+            if (ps.pull_sep) hipLaunchKernelGGL(k_pull, pull_grid, dim3(kThreads), 0, s, ps);
             kt.end(2);
         }
     }
