@@ -1253,18 +1253,27 @@ __device__ void pull_role(const PushStep& a, PushLds& L, int64_t wid, int64_t nw
             int64_t e = a.csc_ptr[d];
             const int64_t e1 = a.csc_ptr[d + 1];
             if (WAVE && Gp == 64) {
-                // one destination per wavefront: e and e1 are uniform
+                // one destination per wavefront: e and e1 are uniform.  The
+                // next 64 edges and their lane-bit words are requested before
+                // the current ones are summed
                 const int sh = gl & 31;
+                auto edges = [&](int64_t c, int& wi) -> uint64_t {
+                    const int m = (int)(e1 - c < 64 ? e1 - c : 64);
+                    const bool in = c < e1 && gl < m;
+                    const int r = in ? a.csc_src[c + gl] : 0;
+                    wi = in ? __float_as_int(a.csc_w[c + gl]) : 0;
+                    const int src = (int)div_u31(fd, (uint32_t)r);
+                    const int slot = L.slot[r - src * a.csr_d];
+                    return (in && slot >= 0) ? bits[((int64_t)slot * a.n + src) * a.lw] : 0ull;
+                };
+                int wi_next = 0;
+                uint64_t wb_next = e < e1 ? edges(e, wi_next) : 0ull;
                 for (int64_t c = e; c < e1; c += 64) {
                     const int m =
                         __builtin_amdgcn_readfirstlane((int)(e1 - c < 64 ? e1 - c : 64));
-                    const bool in = gl < m;
-                    const int r = in ? a.csc_src[c + gl] : 0;
-                    const int wi = in ? __float_as_int(a.csc_w[c + gl]) : 0;
-                    const int src = (int)div_u31(fd, (uint32_t)r);
-                    const int slot = L.slot[r - src * a.csr_d];
-                    const uint64_t wb =
-                        (in && slot >= 0) ? bits[((int64_t)slot * a.n + src) * a.lw] : 0ull;
+                    const int wi = wi_next;
+                    const uint64_t wb = wb_next;
+                    if (c + 64 < e1) wb_next = edges(c + 64, wi_next);
                     const int lo = (int)(uint32_t)wb, hi = (int)(uint32_t)(wb >> 32);
                     for (int k = 0; k < m; ++k) {
                         const uint32_t wl = (uint32_t)__builtin_amdgcn_readlane(lo, k);
