@@ -119,9 +119,10 @@ def test_140k_connectome_teacher_forced_and_free_running_bit_exact():
 
 
 def test_16k_training_gradients_match_reference_autograd():
-    # full N, p and batch; T = 6 in checkpoint chunks of 3 keeps the CPU
-    # oracle's autograd over 13M synapses x 64 lanes near half a minute
-    n, p, B, T = 16_384, 0.05, 64, 6
+    # full N, p and batch; T = 4 in checkpoint chunks of 2 keeps the CPU
+    # oracle's autograd over 13M synapses x 64 lanes within a minute on a
+    # slow host
+    n, p, B, T = 16_384, 0.05, 64, 4
     c = workloads.er_net_fast(n=n, p=p, seed=4)
     rng = np.random.default_rng(4)
     w = rng.normal(0.0, 1.0 / math.sqrt(p * n), c.nnz).astype(np.float32)
@@ -133,7 +134,7 @@ def test_16k_training_gradients_match_reference_autograd():
     model = RecurrentNN(LIF(n, tau=tau, tau_ref=2.0, trainable_param={"tau"},
                             surrogate_function=sur),
                         ExponentialPSC(n, 5.0, conn), step_mode="m", grad_checkpoint=True,
-                        chunk_size=3, unroll=3, update_state_names=[]).to(DEV)
+                        chunk_size=2, unroll=2, update_state_names=[]).to(DEV)
     functional.init_net_state(model, batch_size=B, device=DEV, dtype=torch.float32)
     x = torch.from_numpy(workloads.poisson_input(T, B, n, 400.0, 1.0, seed=4) * 2.0)
     R = torch.randn(T, B, n, generator=torch.Generator().manual_seed(1))
