@@ -2255,7 +2255,11 @@ static int btx::rsnn_run(const btx_rsnn_plan* p, int64_t n_steps, hipStream_t s,
         const int64_t want = (kPullWgs + lanes - 1) / lanes;
         ps.nh = want > npb ? (int)(want - npb) : 0;
     }
-    const dim3 pull_grid((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
+    // a wavefront per destination, at most 4096 workgroups: a launch that
+    // only ends at once (the step pushed) costs ~8 us at 35k workgroups, ~4
+    // at 2048, and the 16k-neuron pull 235 us at 4096 against 258 at 2048
+    const int64_t pull_wgs = (n + kThreads / 64 - 1) / (kThreads / 64);
+    const dim3 pull_grid((unsigned)(pull_wgs < 4096 ? pull_wgs : 4096));
     ControlArgs ca{};
     ca.ring_cnt = p->ring_cnt;
     ca.ctl = ctl;
