@@ -447,7 +447,8 @@ typedef struct btx_rsnn_plan {
      * ring_rebuild rebuilds every slot from ring_words; a caller that empties
      * lanes of the ring clears their bits too.  The decision is made one
      * step ahead from the ring's spike totals (the newest slot estimated by
-     * the one before). */
+     * the one before).  Batches of more than 32 lanes pull in a launch of
+     * their own queued after each step's (it ends at once on a push step). */
     uint64_t* ring_lane_bits;
 } btx_rsnn_plan;
 
